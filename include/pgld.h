@@ -131,6 +131,69 @@ int pgld_count_block(const pgld_panel_desc* desc, const int8_t* d_panels,
                      const int32_t* d_site_stats, int32_t pop, int64_t i0, int64_t ni,
                      int64_t j0, int64_t nj, int32_t* d_counts, void* stream);
 
+/* Heterozygosity statistics H_i_j (i <= j, het_names order) summed over
+ * sites from the pgld_scan_sites tallies, as fixed-point limbs
+ * d_out[e][0..2] (zeroed by the caller); d_nonfinite[e] counts sites whose
+ * term was not finite.  Replaces _compute_heterozygosity
+ * (pg_gpu/moments_ld.py:346-388). */
+int pgld_heterozygosity(const pgld_panel_desc* desc, const int32_t* d_site_stats,
+                        long long* d_out, int32_t* d_nonfinite, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Per-pair API (not the binned hot path)
+ * ------------------------------------------------------------------------- */
+
+/* d_jmax[i] = number of sites j with pos[j] <= pos[i] + max_dist (float64),
+ * i.e. the exclusive right end of anchor i's partners; positions sorted.
+ * Replaces the searchsorted step of iter_pairs_within_distance
+ * (pg_gpu/ld_pipeline.py:85-88). */
+int pgld_pair_bounds(const double* d_pos, int64_t n_sites, double max_dist, int64_t* d_jmax,
+                     void* stream);
+
+/* Writes the n_pairs (i, j) pairs of anchors v_lo .. v_lo+n_anchors-1 in
+ * (i, then j) order; d_offsets[k] is the exclusive prefix count of pairs of
+ * the chunk's first k anchors (n_anchors + 1 entries).  Replaces the
+ * per-chunk pair expansion of pg_gpu/ld_pipeline.py:103-121. */
+int pgld_emit_pairs(const int64_t* d_offsets, int64_t v_lo, int64_t n_anchors, int64_t n_pairs,
+                    int32_t* d_idx_i, int32_t* d_idx_j, void* stream);
+
+/* Counts of population `pop` for an explicit pair list, from packed panels:
+ * haplotype int32 [n][4] = (n11, n10, n01, n00), genotype int32 [n][9] =
+ * (n00, n01, ..., n22), and the valid-sample count per pair (d_nvalid may be
+ * NULL).  Replaces compute_counts_for_pairs (pg_gpu/ld_pipeline.py:130-175)
+ * and compute_genotype_counts_for_pairs with _GENO_COUNTS_KERN
+ * (pg_gpu/ld_pipeline.py:183-263).  Indices must be in [0, n_sites). */
+int pgld_count_pairs(const pgld_panel_desc* desc, const int8_t* d_panels,
+                     const int32_t* d_site_stats, int32_t pop, const int32_t* d_idx_i,
+                     const int32_t* d_idx_j, int64_t n_pairs, int32_t* d_counts,
+                     int32_t* d_nvalid, void* stream);
+
+/* Raw haplotype estimator calls per pair.  d_counts: float64 [n][stride],
+ * population p in columns 4p..4p+3 as (n11, n10, n01, n00); d_nvalid:
+ * float64 [n][n_pops] or NULL; where it is NULL or an entry is negative the
+ * sample size is the sum of that population's counts.  calls: host
+ * int32 [n_calls][5] = (kind, p0, p1, p2, p3), kind 0 = DD(p0,p1),
+ * 1 = Dz(p0,p1,p2), 2 = pi2(p0,p1,p2,p3).  d_out: float64 [n][n_calls].
+ * Replaces ld_statistics.dd / dz / pi2 (pg_gpu/ld_statistics.py:13-129,
+ * :1116-1617) and the per-case kernels of pg_gpu/haplotype_kernels.py:69-304. */
+int pgld_hap_calls(const double* d_counts, int64_t stride, int32_t n_pops,
+                   const double* d_nvalid, int64_t n_pairs, const int32_t* calls,
+                   int32_t n_calls, double* d_out, void* stream);
+
+/* The full moments statistic set (pgld_n_stats(PGLD_MODE_HAP, n_pops)
+ * columns, ld_names order) per pair.  Replaces
+ * compute_single_pop_statistics_batch / compute_two_pop_statistics_batch
+ * (pg_gpu/ld_pipeline.py:271-349) and compute_multi_pop_statistics_batch_hap
+ * (pg_gpu/haplotype_kernels.py:584-640). */
+int pgld_hap_stat_set(const double* d_counts, int64_t stride, int32_t n_pops,
+                      const double* d_nvalid, int64_t n_pairs, double* d_out, void* stream);
+
+/* r (which = 0), r^2 (1) or Lewontin's D' (2) per pair from 4-way counts;
+ * NaN where undefined.  Replaces ld_statistics.r / r_squared / d_prime
+ * (pg_gpu/ld_statistics.py:167-246, haplotype_kernels.py:20-62). */
+int pgld_r_family(const double* d_counts, int64_t stride, const double* d_nvalid,
+                  int64_t n_pairs, int32_t which, double* d_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

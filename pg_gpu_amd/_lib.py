@@ -66,6 +66,19 @@ _SIGS = {
     "pgld_binned_plain": (c_int32, [POINTER(PanelDesc), POINTER(BinnedArgs), c_void_p]),
     "pgld_count_block": (c_int32, [POINTER(PanelDesc), c_void_p, c_void_p, c_int32, c_int64,
                                    c_int64, c_int64, c_int64, c_void_p, c_void_p]),
+    "pgld_pair_bounds": (c_int32, [c_void_p, c_int64, c_double, c_void_p, c_void_p]),
+    "pgld_emit_pairs": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p,
+                                  c_void_p]),
+    "pgld_count_pairs": (c_int32, [POINTER(PanelDesc), c_void_p, c_void_p, c_int32, c_void_p,
+                                   c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    "pgld_hap_calls": (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_int64,
+                                 POINTER(c_int32), c_int32, c_void_p, c_void_p]),
+    "pgld_hap_stat_set": (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_int64, c_void_p,
+                                    c_void_p]),
+    "pgld_r_family": (c_int32, [c_void_p, c_int64, c_void_p, c_int64, c_int32, c_void_p,
+                                c_void_p]),
+    "pgld_heterozygosity": (c_int32, [POINTER(PanelDesc), c_void_p, c_void_p, c_void_p,
+                                      c_void_p]),
 }
 
 _lib = None
@@ -124,3 +137,6 @@ def stream_handle(stream=None):
 def int32_array(values):
     arr = (c_int32 * len(values))(*[int(v) for v in values])
     return arr
+
+
+byref = ctypes.byref

@@ -1,0 +1,393 @@
+// Per-pair entry points behind the reference's per-pair Python API:
+//   pair enumeration       (ld_pipeline.iter_pairs_within_distance)
+//   counts for pair lists  (ld_pipeline.compute_counts_for_pairs /
+//                           compute_genotype_counts_for_pairs)
+//   per-pair estimators    (ld_statistics.dd / dz / pi2, the 2-pop and
+//                           multi-pop batches) and r / r^2 / D'.
+// These are not the binned hot path; they serve callers that want per-pair
+// values.  Estimators divide by the falling factorials exactly as the
+// reference does (numerator / denominator with a den > 0 guard).
+#include "pgld_internal.h"
+#include "stats_hap.inc"
+
+namespace pgld {
+
+// ---------------------------------------------------------------------------
+// Pair enumeration
+// ---------------------------------------------------------------------------
+__global__ void pair_bounds_kernel(const double* __restrict__ pos, int64_t V, double max_dist,
+                                   int64_t* __restrict__ jmax) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V) return;
+  const double ub = pos[i] + max_dist;
+  int64_t lo = i + 1, hi = V;  // positions are sorted: every j <= i has pos <= ub
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (pos[mid] <= ub) lo = mid + 1; else hi = mid;
+  }
+  jmax[i] = lo;
+}
+
+// offs[k] = number of pairs of anchors v_lo .. v_lo+k-1 (exclusive prefix, m+1 entries)
+__global__ void emit_pairs_kernel(const int64_t* __restrict__ offs, int64_t v_lo, int64_t m,
+                                  int64_t n_pairs, int32_t* __restrict__ idx_i,
+                                  int32_t* __restrict__ idx_j) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_pairs) return;
+  int64_t lo = 0, hi = m;  // largest k with offs[k] <= t
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (offs[mid] <= t) lo = mid; else hi = mid;
+  }
+  const int64_t i = v_lo + lo;
+  idx_i[t] = (int32_t)i;
+  idx_j[t] = (int32_t)(i + 1 + (t - offs[lo]));
+}
+
+// ---------------------------------------------------------------------------
+// Counts for explicit pair lists, from the packed panels (dot4 on bytes).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ const uint4* site_half(const int8_t* panels, int64_t block_bytes,
+                                                  int64_t pop_off, int ks, int f, int s,
+                                                  int64_t site, int half) {
+  const int8_t* frag = panels + (site >> 5) * block_bytes + pop_off +
+                       ((int64_t)f * ks + s) * PGLD_FRAG_BYTES;
+  return reinterpret_cast<const uint4*>(frag + ((site & 31) + 32 * half) * 16);
+}
+
+__device__ __forceinline__ int dot16(uint4 a, uint4 b, int acc) {
+  acc = __builtin_amdgcn_udot4(a.x, b.x, acc, false);
+  acc = __builtin_amdgcn_udot4(a.y, b.y, acc, false);
+  acc = __builtin_amdgcn_udot4(a.z, b.z, acc, false);
+  acc = __builtin_amdgcn_udot4(a.w, b.w, acc, false);
+  return acc;
+}
+
+// Feature products P[fa][fb] = sum_k F_fa(i)[k] * F_fb(j)[k] for all feature pairs.
+template <int NF>
+__device__ __forceinline__ void feature_products(const int8_t* panels, int64_t block_bytes,
+                                                 int64_t pop_off, int ks, int64_t i, int64_t j,
+                                                 int (&prod)[NF][NF]) {
+#pragma unroll
+  for (int a = 0; a < NF; ++a)
+#pragma unroll
+    for (int b = 0; b < NF; ++b) prod[a][b] = 0;
+  for (int s = 0; s < ks; ++s) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      uint4 fi[NF], fj[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        fi[f] = *site_half(panels, block_bytes, pop_off, ks, f, s, i, h);
+        fj[f] = *site_half(panels, block_bytes, pop_off, ks, f, s, j, h);
+      }
+#pragma unroll
+      for (int a = 0; a < NF; ++a)
+#pragma unroll
+        for (int b = 0; b < NF; ++b) prod[a][b] = dot16(fi[a], fj[b], prod[a][b]);
+    }
+  }
+}
+
+struct CountArgs {
+  const int8_t* panels;
+  int64_t block_bytes, pop_off;
+  int ks, missing, K, mode;
+  const int32_t* site_stats;  // [3][V] of this pop
+  int64_t V;
+  const int32_t* idx_i;
+  const int32_t* idx_j;
+  int64_t N;
+  int32_t* counts;   // [N][4] or [N][9]
+  int32_t* nvalid;   // [N] or null
+};
+
+__global__ void __launch_bounds__(256) count_pairs_kernel(CountArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.N) return;
+  const int64_t i = a.idx_i[t], j = a.idx_j[t];
+  if (a.mode == PGLD_MODE_HAP) {
+    int hh, hv, vh, vv;
+    if (a.missing) {
+      int pr[2][2];
+      feature_products<2>(a.panels, a.block_bytes, a.pop_off, a.ks, i, j, pr);
+      hh = pr[0][0]; hv = pr[0][1]; vh = pr[1][0]; vv = pr[1][1];
+    } else {
+      int pr[1][1];
+      feature_products<1>(a.panels, a.block_bytes, a.pop_off, a.ks, i, j, pr);
+      hh = pr[0][0]; hv = a.site_stats[i]; vh = a.site_stats[j]; vv = a.K;
+    }
+    int32_t* c = a.counts + 4 * t;
+    c[0] = hh; c[1] = hv - hh; c[2] = vh - hh; c[3] = vv - hv - vh + hh;
+    if (a.nvalid) a.nvalid[t] = vv;
+  } else {
+    // features: b = [g==1], c = [g==2], v = [g>=0]
+    int bb, bc, cb, cc, bv, cv, vb, vc, vv;
+    if (a.missing) {
+      int pr[3][3];
+      feature_products<3>(a.panels, a.block_bytes, a.pop_off, a.ks, i, j, pr);
+      bb = pr[0][0]; bc = pr[0][1]; bv = pr[0][2];
+      cb = pr[1][0]; cc = pr[1][1]; cv = pr[1][2];
+      vb = pr[2][0]; vc = pr[2][1]; vv = pr[2][2];
+    } else {
+      int pr[2][2];
+      feature_products<2>(a.panels, a.block_bytes, a.pop_off, a.ks, i, j, pr);
+      bb = pr[0][0]; bc = pr[0][1]; cb = pr[1][0]; cc = pr[1][1];
+      bv = a.site_stats[i]; cv = a.site_stats[a.V + i];
+      vb = a.site_stats[j]; vc = a.site_stats[a.V + j];
+      vv = a.K;
+    }
+    const int n10 = bv - bb - bc, n20 = cv - cb - cc;
+    const int n01 = vb - bb - cb, n02 = vc - bc - cc;
+    const int n00 = vv - (bv + cv) - n01 - n02;
+    int32_t* c = a.counts + 9 * t;
+    c[0] = n00; c[1] = n01; c[2] = n02;
+    c[3] = n10; c[4] = bb; c[5] = bc;
+    c[6] = n20; c[7] = cb; c[8] = cc;
+    if (a.nvalid) a.nvalid[t] = vv;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-pair haplotype estimators on FP64 counts (reference division form).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ HapPop hap_pop_div(double c1, double c2, double c3, double c4,
+                                              double n) {
+  HapPop p;
+  p.c1 = c1; p.c2 = c2; p.c3 = c3; p.c4 = c4;
+  p.D = c2 * c3 - c1 * c4;
+  p.pA = c1 + c2; p.qA = c3 + c4; p.pB = c1 + c3; p.qB = c2 + c4;
+  const double f1 = n, f2 = f1 * (n - 1.0), f3 = f2 * (n - 2.0), f4 = f3 * (n - 3.0);
+  p.r1 = f1 > 0.0 ? 1.0 / f1 : 0.0;
+  p.r2 = f2 > 0.0 ? 1.0 / f2 : 0.0;
+  p.r3 = f3 > 0.0 ? 1.0 / f3 : 0.0;
+  p.r4 = f4 > 0.0 ? 1.0 / f4 : 0.0;
+  return p;
+}
+
+// call codes: (kind, p0, p1, p2, p3) with kind 0 = dd, 1 = dz, 2 = pi2
+__device__ double hap_raw_call(const HapPop* P, int kind, int i, int j, int k, int l) {
+  if (kind == 0) return i == j ? hap_dd_same(P[i]) : hap_dd_pair(P[i], P[j]);
+  if (kind == 1) {
+    if (i == j && j == k) return hap_dz_same(P[i]);
+    if (i == j) return hap_dz_iik(P[i], P[k]);
+    if (i == k) return hap_dz_iji(P[i], P[j]);
+    if (j == k) return hap_dz_ijj(P[i], P[j]);
+    return hap_dz_ijk(P[i], P[j], P[k]);
+  }
+  int cnt[PGLD_MAX_POPS] = {0, 0, 0, 0};
+  cnt[i]++; cnt[j]++; cnt[k]++; cnt[l]++;
+  int nu = 0, mc = 0, t3 = -1, s1 = -1;
+  for (int q = 0; q < PGLD_MAX_POPS; ++q) {
+    nu += cnt[q] > 0;
+    mc = cnt[q] > mc ? cnt[q] : mc;
+    if (cnt[q] == 3) t3 = q;
+    if (cnt[q] == 1) s1 = q;
+  }
+  if (nu == 1) return hap_pi2_same(P[i]);
+  if (mc == 3) return hap_pi2_triple(P[t3], P[s1]);
+  if (i == j && k == l) return hap_pi2_iikk(P[i], P[k]);
+  if (i == j) return hap_pi2_iikl(P[i], P[k], P[l]);
+  if (k == l) return hap_pi2_ijkk(P[i], P[j], P[k]);
+  if ((i == k && j == l) || (i == l && j == k)) return hap_pi2_ijij(P[i], P[j]);
+  if (nu == 3) {
+    if (i == k) return hap_pi2_shared(P[i], P[j], P[l]);
+    if (i == l) return hap_pi2_shared(P[i], P[j], P[k]);
+    if (j == k) return hap_pi2_shared(P[j], P[i], P[l]);
+    return hap_pi2_shared(P[j], P[i], P[k]);
+  }
+  return hap_pi2_alldiff(P[i], P[j], P[k], P[l]);
+}
+
+struct HapCallArgs {
+  const double* counts;    // [N][stride], pop p at columns 4p..4p+3
+  int64_t stride;
+  const double* nvalid;    // [N][P] or null
+  int64_t N;
+  int P;
+  int n_calls;
+  int calls[64][5];
+  double* out;             // [N][n_calls]
+};
+
+__device__ __forceinline__ void load_pops(const double* counts, int64_t stride, const double* nv,
+                                          int P, int64_t t, HapPop* pp) {
+  for (int p = 0; p < P; ++p) {
+    const double* c = counts + t * stride + 4 * p;
+    const double given = nv ? nv[t * P + p] : -1.0;  // < 0: use the sum of the counts
+    const double n = given >= 0.0 ? given : (c[0] + c[1] + c[2] + c[3]);
+    pp[p] = hap_pop_div(c[0], c[1], c[2], c[3], n);
+  }
+}
+
+__global__ void __launch_bounds__(256) hap_calls_kernel(HapCallArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.N) return;
+  HapPop pp[PGLD_MAX_POPS];
+  load_pops(a.counts, a.stride, a.nvalid, a.P, t, pp);
+  for (int c = 0; c < a.n_calls; ++c)
+    a.out[t * a.n_calls + c] =
+        hap_raw_call(pp, a.calls[c][0], a.calls[c][1], a.calls[c][2], a.calls[c][3], a.calls[c][4]);
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) hap_set_kernel(const double* counts, int64_t stride,
+                                                      const double* nvalid, int64_t N,
+                                                      double* out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  HapPop pp[P];
+  load_pops(counts, stride, nvalid, P, t, pp);
+  double x[P == 1 ? HAP_NSTATS_P1 : P == 2 ? HAP_NSTATS_P2 : P == 3 ? HAP_NSTATS_P3 : HAP_NSTATS_P4];
+  if constexpr (P == 1) hap_stats_P1(pp, x);
+  if constexpr (P == 2) hap_stats_P2(pp, x);
+  if constexpr (P == 3) hap_stats_P3(pp, x);
+  if constexpr (P == 4) hap_stats_P4(pp, x);
+  constexpr int NS = sizeof(x) / sizeof(double);
+  for (int q = 0; q < NS; ++q) out[t * NS + q] = x[q];
+}
+
+// r, r^2, D' (haplotype_kernels.py:20-62 arithmetic)
+__global__ void r_family_kernel(const double* counts, int64_t stride, const double* nvalid,
+                                int64_t N, int which, double* out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  const double* c = counts + t * stride;
+  const double a = c[0], b = c[1], cc = c[2], d = c[3];
+  const double n = nvalid ? nvalid[t] : a + b + cc + d;
+  const double pA = (a + b) / n, pB = (a + cc) / n;
+  const double D = (a * d - b * cc) / (n * n);
+  double v;
+  if (which == 2) {
+    const double qA = 1.0 - pA, qB = 1.0 - pB;
+    double dmax;
+    if (D >= 0.0) {
+      const double v1 = pA * qB, v2 = qA * pB;
+      dmax = v1 < v2 ? v1 : v2;
+    } else {
+      const double v1 = pA * pB, v2 = qA * qB;
+      dmax = v1 < v2 ? v1 : v2;
+    }
+    v = dmax > 0.0 ? D / dmax : __builtin_nan("");
+  } else {
+    const double den = pA * (1.0 - pA) * pB * (1.0 - pB);
+    v = den > 0.0 ? (which == 0 ? D / sqrt(den) : D * D / den) : __builtin_nan("");
+  }
+  out[t] = v;
+}
+
+static unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace pgld
+
+using namespace pgld;
+
+extern "C" int pgld_pair_bounds(const double* d_pos, int64_t n_sites, double max_dist,
+                                int64_t* d_jmax, void* stream) {
+  if (n_sites < 0) return fail(PGLD_EINVAL, "negative n_sites");
+  if (n_sites == 0) return PGLD_OK;
+  hipLaunchKernelGGL(pair_bounds_kernel, dim3(blocks_for(n_sites)), dim3(256), 0,
+                     as_stream(stream), d_pos, n_sites, max_dist, d_jmax);
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}
+
+extern "C" int pgld_emit_pairs(const int64_t* d_offsets, int64_t v_lo, int64_t n_anchors,
+                               int64_t n_pairs, int32_t* d_idx_i, int32_t* d_idx_j, void* stream) {
+  if (n_anchors < 1 || n_pairs < 0) return fail(PGLD_EINVAL, "bad pair chunk");
+  if (n_pairs == 0) return PGLD_OK;
+  hipLaunchKernelGGL(emit_pairs_kernel, dim3(blocks_for(n_pairs)), dim3(256), 0,
+                     as_stream(stream), d_offsets, v_lo, n_anchors, n_pairs, d_idx_i, d_idx_j);
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}
+
+extern "C" int pgld_count_pairs(const pgld_panel_desc* desc, const int8_t* d_panels,
+                                const int32_t* d_site_stats, int32_t pop, const int32_t* d_idx_i,
+                                const int32_t* d_idx_j, int64_t n_pairs, int32_t* d_counts,
+                                int32_t* d_nvalid, void* stream) {
+  PanelLayout L;
+  int rc = make_layout(desc, &L);
+  if (rc) return rc;
+  if (pop < 0 || pop >= desc->n_pops) return fail(PGLD_EINVAL, "pop out of range");
+  if (n_pairs < 0) return fail(PGLD_EINVAL, "negative n_pairs");
+  if (n_pairs == 0) return PGLD_OK;
+  CountArgs a;
+  a.panels = d_panels;
+  a.block_bytes = L.block_bytes;
+  a.pop_off = L.pop_off[pop];
+  a.ks = L.ksteps[pop];
+  a.missing = L.missing[pop];
+  a.K = L.pop_size[pop];
+  a.mode = desc->mode;
+  a.site_stats = d_site_stats + (int64_t)pop * 3 * L.n_sites;
+  a.V = L.n_sites;
+  a.idx_i = d_idx_i;
+  a.idx_j = d_idx_j;
+  a.N = n_pairs;
+  a.counts = d_counts;
+  a.nvalid = d_nvalid;
+  hipLaunchKernelGGL(count_pairs_kernel, dim3(blocks_for(n_pairs)), dim3(256), 0,
+                     as_stream(stream), a);
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}
+
+extern "C" int pgld_hap_calls(const double* d_counts, int64_t stride, int32_t n_pops,
+                              const double* d_nvalid, int64_t n_pairs, const int32_t* calls,
+                              int32_t n_calls, double* d_out, void* stream) {
+  if (n_pops < 1 || n_pops > PGLD_MAX_POPS) return fail(PGLD_EINVAL, "1-4 populations supported");
+  if (stride < 4 * n_pops) return fail(PGLD_EINVAL, "count rows narrower than 4 * n_pops");
+  if (n_calls < 1 || n_calls > 64) return fail(PGLD_EINVAL, "1..64 calls per launch");
+  HapCallArgs a;
+  a.counts = d_counts;
+  a.stride = stride;
+  a.nvalid = d_nvalid;
+  a.N = n_pairs;
+  a.P = n_pops;
+  a.n_calls = n_calls;
+  a.out = d_out;
+  for (int c = 0; c < n_calls; ++c) {
+    const int kind = calls[5 * c];
+    if (kind < 0 || kind > 2) return fail(PGLD_EINVAL, "call kind must be 0 (dd), 1 (dz), 2 (pi2)");
+    const int arity = kind + 2;
+    for (int q = 0; q < 5; ++q) a.calls[c][q] = calls[5 * c + q];
+    for (int q = 1; q <= arity; ++q)
+      if (a.calls[c][q] < 0 || a.calls[c][q] >= n_pops)
+        return fail(PGLD_EINVAL, "population index out of range");
+    for (int q = arity + 1; q < 5; ++q) a.calls[c][q] = 0;
+  }
+  if (n_pairs == 0) return PGLD_OK;
+  hipLaunchKernelGGL(hap_calls_kernel, dim3(blocks_for(n_pairs)), dim3(256), 0, as_stream(stream), a);
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}
+
+extern "C" int pgld_hap_stat_set(const double* d_counts, int64_t stride, int32_t n_pops,
+                                 const double* d_nvalid, int64_t n_pairs, double* d_out,
+                                 void* stream) {
+  if (n_pops < 1 || n_pops > PGLD_MAX_POPS) return fail(PGLD_EINVAL, "1-4 populations supported");
+  if (stride < 4 * n_pops) return fail(PGLD_EINVAL, "count rows narrower than 4 * n_pops");
+  if (n_pairs == 0) return PGLD_OK;
+  const dim3 g(blocks_for(n_pairs)), b(256);
+  hipStream_t s = as_stream(stream);
+  switch (n_pops) {
+    case 1: hipLaunchKernelGGL(hap_set_kernel<1>, g, b, 0, s, d_counts, stride, d_nvalid, n_pairs, d_out); break;
+    case 2: hipLaunchKernelGGL(hap_set_kernel<2>, g, b, 0, s, d_counts, stride, d_nvalid, n_pairs, d_out); break;
+    case 3: hipLaunchKernelGGL(hap_set_kernel<3>, g, b, 0, s, d_counts, stride, d_nvalid, n_pairs, d_out); break;
+    default: hipLaunchKernelGGL(hap_set_kernel<4>, g, b, 0, s, d_counts, stride, d_nvalid, n_pairs, d_out); break;
+  }
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}
+
+extern "C" int pgld_r_family(const double* d_counts, int64_t stride, const double* d_nvalid,
+                             int64_t n_pairs, int32_t which, double* d_out, void* stream) {
+  if (which < 0 || which > 2) return fail(PGLD_EINVAL, "which must be 0 (r), 1 (r^2), 2 (D')");
+  if (stride < 4) return fail(PGLD_EINVAL, "count rows narrower than 4");
+  if (n_pairs == 0) return PGLD_OK;
+  hipLaunchKernelGGL(r_family_kernel, dim3(blocks_for(n_pairs)), dim3(256), 0, as_stream(stream),
+                     d_counts, stride, d_nvalid, n_pairs, which, d_out);
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}

@@ -1,0 +1,111 @@
+"""Pins the genotype half of the oracle (CPU only).
+
+* the single-population polynomials read from the two reference files
+  (readable module and production kernels) are identical coefficient tables;
+* every readable-form estimator is exactly unbiased for its haplotype-level
+  target under random mating (factorial-moment expectation, exact rationals);
+* the production (kernel) form -- the parity target of the GPU code -- agrees
+  with the readable form on random counts for every raw call at P = 1..4;
+* a planted coefficient error is caught by the unbiasedness check.
+"""
+import copy
+from fractions import Fraction as F
+
+import numpy as np
+import pytest
+import sympy as sp
+
+from oracle import geno_oracle as go
+from oracle import ld_oracle as o
+
+
+def test_single_pop_tables_agree_between_reference_files():
+    T = go.tables()
+    for name in ("dd_geno_single", "dz_geno_single", "pi2_geno_single"):
+        a = {tuple(m): F(c) for m, c in T["kernel_single"][name]}
+        b = {tuple(m): F(c) for m, c in T["readable"][name]["numer"]}
+        assert a == b, name
+
+
+def _freqs(seed):
+    rng = np.random.default_rng(seed + 100)
+    w = [F(int(x)) for x in rng.integers(1, 10, 4)]
+    s = sum(w)
+    return [x / s for x in w]
+
+
+def _target(kind, pops, fs):
+    D = lambda f: f[0] * f[3] - f[1] * f[2]
+    p = lambda f: f[0] + f[1]
+    q = lambda f: f[0] + f[2]
+    if kind == "dd":
+        return D(fs[pops[0]]) * D(fs[pops[1]])
+    if kind == "dz":
+        return D(fs[pops[0]]) * (1 - 2 * p(fs[pops[1]])) * (1 - 2 * q(fs[pops[2]]))
+    i, j, k, l = pops
+    H = lambda u, v, g: g(u) * (1 - g(v)) + g(v) * (1 - g(u))
+    return F(1, 4) * H(fs[i], fs[j], p) * H(fs[k], fs[l], q)
+
+
+# readable function -> (statistic, target populations in terms of its argument slots)
+CASES = {
+    "dd_geno_single": ("dd", (0, 0)), "dd_geno_between": ("dd", (0, 1)),
+    "dz_geno_single": ("dz", (0, 0, 0)), "dz_geno_p1p2": ("dz", (0, 0, 1)),
+    "dz_geno_p1p3": ("dz", (0, 1, 0)), "dz_geno_p2p3": ("dz", (0, 1, 1)),
+    "dz_geno_alldiff": ("dz", (0, 1, 2)), "pi2_geno_single": ("pi2", (0, 0, 0, 0)),
+    "pi2_geno_triple_123": ("pi2", (0, 0, 0, 1)), "pi2_geno_triple_134": ("pi2", (0, 1, 0, 0)),
+    "pi2_geno_iikk": ("pi2", (0, 0, 1, 1)), "pi2_geno_ijij": ("pi2", (0, 1, 0, 1)),
+    "pi2_geno_iikl": ("pi2", (0, 0, 1, 2)), "pi2_geno_ijkk": ("pi2", (1, 2, 0, 0)),
+    "pi2_geno_shared": ("pi2", (0, 1, 0, 2)), "pi2_geno_alldiff": ("pi2", (0, 1, 2, 3)),
+}
+
+
+def _estimate_expectation(t, ns, Qs):
+    e = go.expected_table(t["numer"], Qs, ns)
+    den = sp.sympify(t["denom"]).subs({sp.Symbol(f"n{k}"): ns[k] for k in range(len(ns))})
+    return e * F(t["scale"]) / F(int(den))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_genotype_estimators_unbiased(name):
+    T = go.tables()["readable"]
+    t = T[name]
+    kind, pops = CASES[name]
+    P = t["n_pops"]
+    fs = [_freqs(s) for s in range(P)]
+    Qs = [go.genotype_class_probs(f) for f in fs]
+    ns = [6] * P
+    assert _estimate_expectation(t, ns, Qs) == _target(kind, pops, fs)
+
+
+def test_unbiasedness_catches_planted_coefficient_error():
+    t = copy.deepcopy(go.tables()["readable"]["dd_geno_single"])
+    t["numer"][5][1] = str(F(t["numer"][5][1]) + F(1, 16))
+    fs = [_freqs(0)]
+    Qs = [go.genotype_class_probs(f) for f in fs]
+    assert _estimate_expectation(t, [6], Qs) != _target("dd", (0, 0), fs)
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4])
+def test_kernel_form_matches_readable_form(P):
+    rng = np.random.default_rng(P)
+    N = 200
+    cs = [rng.integers(0, 12, (N, 9)) for _ in range(P)]
+    cs[0][:5] = rng.integers(0, 2, (5, 9))          # small samples: guard paths
+    Pk = [go.GenoPop(c) for c in cs]
+    G = [go.to_moments(c) for c in cs]
+    ns = [g.sum(1) for g in G]
+    calls = sorted({(t, pp) for _, c in o.stat_specs(P) for _, t, pp in c})
+    for kind, pp in calls:
+        a = go.readable_call(G, ns, kind, pp)
+        b = go.kernel_call(Pk, kind, pp)
+        ok = np.isfinite(a) & np.isfinite(b) & (np.minimum.reduce([n for n in ns]) >= 4)
+        np.testing.assert_allclose(b[ok], a[ok], rtol=1e-12, atol=1e-300, err_msg=str((kind, pp)))
+
+
+def test_moments_order_mapping():
+    c = np.zeros((1, 9))
+    c[0, 8] = 5      # n22 -> g1
+    c[0, 0] = 2      # n00 -> g9
+    g = go.to_moments(c)
+    assert g[0, 0] == 5 and g[0, 8] == 2
