@@ -122,7 +122,7 @@ def main():
 
     import torch
 
-    from pg_gpu_amd.engine import LDPanels, band_tiles
+    from pg_gpu_amd.engine import TILE, LDPanels, band_tiles
 
     dev = torch.device("cuda:0")
     mat = torch.as_tensor(w.mat, device=dev)
@@ -180,7 +180,7 @@ def main():
     prec = None
     if cpu:
         X = cpu["X"]
-        sub = tiles[tiles[:, 0] < X // 64]
+        sub = tiles[tiles[:, 0] < X // TILE]
         rg = pan.binned(w.positions, w.bins, w.max_dist, kernel="fused", tiles=sub)
         scale = np.maximum(np.abs(cpu["sums"]), 1e-3 * cpu["abs"])
         err = np.abs(rg.sums - cpu["sums"]) / np.maximum(scale, 1e-300)

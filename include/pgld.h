@@ -101,6 +101,8 @@ typedef struct {
   int64_t n_tiles;
   long long* d_out;            /* [n_bins][n_stats + 1][3] fixed-point limbs, zeroed */
   int32_t grid;                /* workgroups for the persistent fused kernel (0 = auto) */
+  int32_t n_lo[PGLD_MAX_POPS]; /* per pop: a lower bound on any pair's valid-sample count
+                                  (e.g. size - 2 * max missing per site); 0 if unknown */
 } pgld_binned_args;
 
 /* Number of statistics produced per bin for (mode, n_pops). */

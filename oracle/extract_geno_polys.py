@@ -109,17 +109,32 @@ def kernel_tables(src_text):
     a2 = g[0] + g[1] / 2 + g[3] + g[4] / 2 + g[6] + g[7] / 2
     a3 = g[1] / 2 + g[2] + g[4] / 2 + g[5] + g[7] / 2 + g[8]
     a4 = (g[3] + g[4] + g[5]) / 2 + g[6] + g[7] + g[8]
-    dd = grab("_DD_GENO_SINGLE_KERN", "poly") / 16 + Dg ** 2
-    dz = grab("_DZ_III_KERN", "quartic") / 4 + A * B * (x00 * x11 - x01 * x10)
-    pi2 = a1 * a2 * a3 * a4 + grab("_PI2_IIII_KERN", "corr")  # corr already carries 1/16
-    return {"dd_geno_single": _poly_table(dd, 1), "dz_geno_single": _poly_table(dz, 1),
+    poly = grab("_DD_GENO_SINGLE_KERN", "poly")
+    quartic = grab("_DZ_III_KERN", "quartic")
+    corr = grab("_PI2_IIII_KERN", "corr") * 16   # the kernel's corr carries a 1/16 factor
+    dd = poly / 16 + Dg ** 2
+    dz = quartic / 4 + A * B * (x00 * x11 - x01 * x10)
+    pi2 = a1 * a2 * a3 * a4 + corr / 16
+    full = {"dd_geno_single": _poly_table(dd, 1), "dz_geno_single": _poly_table(dz, 1),
             "pi2_geno_single": _poly_table(pi2, 1)}
+    # integer sub-polynomials the device code evaluates (genotype_kernels.py
+    # structure: DD = poly/16 + Dg^2, Dz = quartic/4 + A B D, pi2 = a1a2a3a4 + corr/16)
+    parts = {"dd_poly": _poly_table(poly, 1), "dz_quartic": _poly_table(quartic, 1),
+             "pi2_corr": _poly_table(corr, 1)}
+    return full, parts
 
 
 def main(ref_root):
     ref = Path(ref_root)
     readable = readable_tables((ref / "pg_gpu/ld_statistics_genotype.py").read_text())
-    kern = kernel_tables((ref / "pg_gpu/genotype_kernels.py").read_text())
+    kern, parts = kernel_tables((ref / "pg_gpu/genotype_kernels.py").read_text())
+    # product-side data for the HIP code generator (pg_gpu_amd/csrc/gen_stats.py)
+    csrc = HERE.parent / "pg_gpu_amd" / "csrc" / "geno_kernel_polys.json"
+    csrc.write_text(json.dumps({
+        "source": "integer sub-polynomials of the single-population genotype estimators "
+                  "(pg_gpu/genotype_kernels.py:27-71, :235-272, :364-405); variables g0_1..g0_9 "
+                  "= moments genotype classes g1 (n22) .. g9 (n00)",
+        **parts}, separators=(",", ":")))
     doc = {
         "source": "pg_gpu/ld_statistics_genotype.py (readable) and pg_gpu/genotype_kernels.py "
                   "(kernel single-pop polynomials); variables g{p}_{m}: population slot p, "

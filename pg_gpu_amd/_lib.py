@@ -49,6 +49,7 @@ class BinnedArgs(ctypes.Structure):
         ("n_tiles", c_int64),
         ("d_out", c_void_p),
         ("grid", c_int32),
+        ("n_lo", c_int32 * MAX_POPS),
     ]
 
 

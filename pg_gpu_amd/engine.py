@@ -18,7 +18,8 @@ import numpy as np
 from . import _lib
 from ._lib import MODE_GENO, MODE_HAP
 
-TILE = 64
+TILE = 32  # sites per tile side: one 32x32 MFMA block per wave
+SUPER = 16  # tiles per side of the L2-sharing groups of band_tiles
 
 
 def _torch():
@@ -39,7 +40,7 @@ def reciprocal_table(kmax):
 
 
 def band_tiles(pos, max_dist):
-    """(row tile, col tile) pairs of 64 sites covering every i<j pair with
+    """(row tile, col tile) pairs of TILE sites covering every i<j pair with
     pos[j] <= pos[i] + max_dist, row-major (pg_gpu/ld_pipeline.py:85-88)."""
     pos = np.asarray(pos, np.float64)
     V = len(pos)
@@ -55,7 +56,12 @@ def band_tiles(pos, max_dist):
     rows = np.repeat(rb, n_per)
     start = np.repeat(np.cumsum(n_per) - n_per, n_per)
     cols = rows + (np.arange(int(n_per.sum())) - start)
-    return np.stack([rows, cols], axis=1).astype(np.int32)
+    # Group the band into SUPER x SUPER squares of tiles, squares in row-major
+    # order and tiles row-major inside a square: the waves running at the same
+    # time on one XCD then share ~2 * SUPER operand panels in that XCD's L2.
+    key = (rows // SUPER) * (n_rt + SUPER) + (cols // SUPER)
+    order = np.lexsort((cols, rows, key))
+    return np.stack([rows[order], cols[order]], axis=1).astype(np.int32)
 
 
 def decode_limbs(raw, n_stats):
@@ -147,6 +153,8 @@ class BinnedPlan:
         a.n_tiles = len(self.tiles)
         a.d_out = _lib.ptr(out).value
         a.grid = int(grid)
+        for p in range(pan.P):
+            a.n_lo[p] = int(pan.n_lo[p])
         return a
 
     def run(self, kernel="fused", grid=0):
@@ -211,6 +219,10 @@ class LDPanels:
             alph = "0, 1 or -1" if self.mode == MODE_HAP else "0, 1, 2 or -1"
             raise ValueError(f"matrix values must be {alph}")
         self.has_missing = [int(m > 0) for m in miss]
+        # lower bound of any pair's valid-sample count: size - 2 * (max missing at a site)
+        mx = (self.site_stats[:, 2, :].max(dim=1).values.cpu().numpy() if self.V
+              else np.zeros(self.P, np.int64))
+        self.n_lo = [max(0, int(k) - 2 * int(m)) for k, m in zip(sizes, mx)]
         self.desc = self._desc(self.has_missing)
         nbytes = L.pgld_panel_bytes(ctypes.byref(self.desc))
         if nbytes < 0:

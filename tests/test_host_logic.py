@@ -34,7 +34,7 @@ def test_band_tiles_cover_every_pair_once(seed, V, md):
     assert len({tuple(t) for t in tiles}) == len(tiles)
     ii, jj = o.pairs_within_distance(pos, md)
     have = {tuple(t) for t in tiles}
-    for i, j in zip(ii // 64, jj // 64):
+    for i, j in zip(ii // engine.TILE, jj // engine.TILE):
         assert (i, j) in have
     assert np.all(tiles[:, 1] >= tiles[:, 0]) if len(tiles) else True
 
