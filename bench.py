@@ -35,6 +35,14 @@ PEAK_I8_TOPS = 5000.0      # MI355X dense int8 MFMA, TOP/s
 PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
+WORKLOADS = {
+    "c0": "c0: 1 pop x 200 haplotypes, 5k SNPs / 1 Mb, pairs <= 100 kb, 10 log bins, 3 stats",
+    "c1": "c1: 2 pops x 500 haplotypes, 200k SNPs / 20 Mb, 5% missing, pairs <= 200 kb, "
+          "20 log bins 100 bp-200 kb, 15 stats",
+    "c2": "c2: 1 pop x 1000 diploid genotypes, 100k SNPs / 10 Mb, 2% missing, pairs <= 100 kb, "
+          "15 log bins, 3 stats",
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -50,8 +58,9 @@ def _cpu_worker(block):
 
     w = _W
     t0 = time.perf_counter()
-    sums, counts, n, absr = o.binned_sums_hap(w.mat, w.positions, w.pop_rows, w.bins, w.max_dist,
-                                              chunk=100_000, with_abs=True, anchors=block)
+    sums, counts, n, absr = o.binned_sums(w.mat, w.positions, w.pop_rows, w.bins, w.max_dist,
+                                          chunk=100_000, with_abs=True, anchors=block,
+                                          mode=w.mode)
     return sums, counts, n, absr, time.perf_counter() - t0
 
 
@@ -65,8 +74,8 @@ def cpu_baseline(w, target_s=15.0, max_workers=16):
     cores = max(1, min(os.cpu_count() or 1, max_workers))
     # calibrate on one small block in this process
     t0 = time.perf_counter()
-    _, _, n_cal, _ = o.binned_sums_hap(w.mat, w.positions, w.pop_rows, w.bins, w.max_dist,
-                                       chunk=100_000, with_abs=True, anchors=(0, 64))
+    _, _, n_cal, _ = o.binned_sums(w.mat, w.positions, w.pop_rows, w.bins, w.max_dist,
+                                   chunk=100_000, with_abs=True, anchors=(0, 64), mode=w.mode)
     rate = n_cal / max(time.perf_counter() - t0, 1e-6)
     jmax = np.searchsorted(w.positions, w.positions + w.max_dist, side="right")
     per = np.maximum(0, jmax - np.arange(len(w.positions)) - 1)
@@ -192,8 +201,12 @@ def main():
         log(f"precision vs oracle on {cpu['n_pairs']:,} pairs: {prec}")
 
     k_ms = float(np.mean(kern_ms))
-    g = 4 if any(pan.has_missing) else 1
-    ops_per_pair = 2 * sum((4 if m else 1) * k for m, k in zip(pan.has_missing, pan.pop_sizes))
+    # int8 MFMA products per pair and population: haplotypes H.H (+ H.V, V.H,
+    # V.V with missing calls); genotypes the 2x2 (or, with missing, 3x3)
+    # feature products.  2 ops (multiply + add) per sample and product.
+    geno = w.mode == "geno"
+    nprod = [((9 if m else 4) if geno else (4 if m else 1)) for m in pan.has_missing]
+    ops_per_pair = 2 * sum(q * k for q, k in zip(nprod, pan.pop_sizes))
     achieved = ops_per_pair * n_pairs / (k_ms * 1e-3) / 1e12
     line = {
         "metric": "site-pairs/sec binned DD/Dz/pi2, I8-MFMA fused vs plain FP64; max rel err",
@@ -208,14 +221,14 @@ def main():
         "vs_baseline": None,
         "dtype": "i8 counts -> i32 accum, f64 estimators",
         "data": "synthetic (founder-copying generator, seeded)",
-        "config": {"workload": f"{w.name}: 2 pops x 500 haplotypes, 200k SNPs / 20 Mb, 5% missing, "
-                               "pairs <= 200 kb, 20 log bins 100 bp-200 kb, 15 stats"
-                   if w.name == "c1" else w.name,
+        "config": {"workload": WORKLOADS.get(w.name, w.name) if args.scale == 1.0 else
+                   f"{w.name} at scale {args.scale}",
                    "n_sites": V, "pairs": n_pairs, "n_bins": int(len(w.bins) - 1),
                    "n_stats": ns, "scale": args.scale, "params": w.params},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_I8_TOPS,
                      "unit": "TOP/s (int8)", "frac": achieved / PEAK_I8_TOPS, "traffic": None,
-                     "kernel": "fused_hap_kernel<2>", "ops_per_pair": ops_per_pair,
+                     "kernel": f"fused_kernel<{'Geno' if geno else 'Hap'}Traits<{pan.P}>>",
+                     "ops_per_pair": ops_per_pair,
                      "kernel_ms": k_ms},
         "plain_fp64": plain,
         "speedup_vs_plain": (plain["ms"] / k_ms) if plain else None,

@@ -127,7 +127,8 @@ int pgld_binned_plain(const pgld_panel_desc* desc, const pgld_binned_args* args,
 /* Dense counts for a rectangle of sites [i0, i0+ni) x [j0, j0+nj) of one
  * population through the MFMA path (used to check counts bit-exactly and to
  * time the counting stage).  d_counts is int32 [q][ni][nj] with q = 4
- * (n11, n10, n01, n00) for haplotypes or 9 (n00..n22) for genotypes; i0 and
+ * (n11, n10, n01, n00) for haplotypes or 9 in moments class order (n22, n21,
+ * n20, n12, n11, n10, n02, n01, n00) for genotypes; i0 and
  * j0 must be multiples of 32. */
 int pgld_count_block(const pgld_panel_desc* desc, const int8_t* d_panels,
                      const int32_t* d_site_stats, int32_t pop, int64_t i0, int64_t ni,

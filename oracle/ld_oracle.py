@@ -554,9 +554,20 @@ def digitize_bins(dist, bins):
     return np.digitize(dist, np.asarray(bins, np.float64)) - 1
 
 
-def binned_sums_hap(hap, positions, pop_rows, bins, max_dist=None, coords=None,
-                    chunk=200_000, exact=True, with_abs=False, anchors=None):
-    """Binned LD sums for haplotype data (moments_ld.py:248-343, haplotype branch).
+def binned_sums_hap(hap, positions, pop_rows, bins, **kw):
+    """Binned LD sums for haplotype data (moments_ld.py:248-343, haplotype branch)."""
+    return binned_sums(hap, positions, pop_rows, bins, mode="hap", **kw)
+
+
+def binned_sums_geno(geno, positions, pop_rows, bins, **kw):
+    """Binned LD sums for genotype data (moments_ld.py:248-343, genotype branch:
+    9-way counts and the kernel-form estimators of every population count)."""
+    return binned_sums(geno, positions, pop_rows, bins, mode="geno", **kw)
+
+
+def binned_sums(mat, positions, pop_rows, bins, max_dist=None, coords=None,
+                chunk=200_000, exact=True, with_abs=False, anchors=None, mode="hap"):
+    """Binned LD sums (moments_ld.py:248-343).
 
     Returns (sums[n_bins, n_stats], pair_counts[n_bins], n_enumerated) and,
     with ``with_abs``, also the per-bin sums of |value| (the scale against which
@@ -591,10 +602,15 @@ def binned_sums_hap(hap, positions, pop_rows, bins, max_dist=None, coords=None,
         b = digitize_bins(d, bins)
         cl, nl = [], []
         for rows in pop_rows:
-            c, nv = hap_counts(hap, ii, jj, rows)
+            c, nv = (hap_counts if mode == "hap" else geno_counts)(mat, ii, jj, rows)
             cl.append(c)
             nl.append(nv)
-        st = hap_stats_for_pairs(cl, nl)
+        if mode == "hap":
+            st = hap_stats_for_pairs(cl, nl)
+        else:
+            from .geno_oracle import multi_pop_stats_geno
+
+            st = multi_pop_stats_geno(cl, nl)
         ok = (b >= 0) & (b < nb)
         counts += np.bincount(b[ok], minlength=nb)[:nb]
         np.add.at(abs_sums, b[ok], np.abs(st[ok]))

@@ -24,6 +24,7 @@
 // epilogue.  No pair list and no per-pair statistic ever reaches HBM.
 #include "pgld_internal.h"
 #include "stats_hap.inc"
+#include "stats_geno.inc"
 
 namespace pgld {
 
@@ -49,16 +50,6 @@ struct FusedParams {
   int32_t rt_lo[PGLD_MAX_POPS];   // smallest n kept in the LDS table of pop p
   int32_t rt_off[PGLD_MAX_POPS];  // first entry of pop p in the LDS table
   int32_t rt_len;                 // total LDS table entries
-};
-
-template <int P> struct HapSet;
-template <> struct HapSet<1> {
-  static constexpr int NS = HAP_NSTATS_P1;
-  __device__ static void eval(const HapPop* p, double* o) { hap_stats_P1(p, o); }
-};
-template <> struct HapSet<2> {
-  static constexpr int NS = HAP_NSTATS_P2;
-  __device__ static void eval(const HapPop* p, double* o) { hap_stats_P2(p, o); }
 };
 
 __device__ __forceinline__ v16i zero16() {
@@ -149,22 +140,183 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
   }
 }
 
-template <int P>
-__device__ __forceinline__ void pair_stats(const FusedParams& a, const double* srt,
-                                           const uint32_t (&w0)[P][16],
-                                           const uint32_t (&w1)[P][16], int r, double* x) {
-  HapPop pp[P];
+// ---------------------------------------------------------------------------
+// Genotype tile counts.  Features per sample: b = [g==1], c = [g==2] and, with
+// missing calls, v = [g>=0].  The 9 products give the 9 class counts:
+//   n11 = b.b, n12 = b.c, n21 = c.b, n22 = c.c, n10 = b.v - n11 - n12,
+//   n20 = c.v - n21 - n22, n01 = v.b - n11 - n21, n02 = v.c - n12 - n22,
+//   n00 = v.v - (b.v + c.v) - n01 - n02.
+// Packed per slot in moments order g1..g9 (n22, n21, n20, n12, n11, n10, n02,
+// n01, n00) as five 16-bit pairs.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, int64_t rb,
+                                                 int64_t cb, int lane, uint32_t (&w)[16][5]) {
+  const int ks = a.ksteps[p];
+  const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[p]) + lane;
+  const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[p]) + lane;
+  const int fs = ks * 64;  // feature stride in v4i
+  v16i bb = zero16(), bc = zero16(), cbm = zero16(), cc = zero16();
+  int cnt[16][9];
+  if (a.missing[p]) {
+    v16i bv = zero16(), cv = zero16(), vb = zero16(), vc = zero16(), vv = zero16();
+    constexpr int PF = 2;
+    v4i fa[PF][3], fb[PF][3];
 #pragma unroll
-  for (int p = 0; p < P; ++p) {
-    const int n11 = w0[p][r] & 0xffff, n10 = w0[p][r] >> 16;
-    const int n01 = w1[p][r] & 0xffff, n00 = w1[p][r] >> 16;
-    const int n = n11 + n10 + n01 + n00;
-    int e = n - a.rt_lo[p];
-    e = e < 0 ? 0 : e;
-    pp[p] = hap_pop(n11, n10, n01, n00, e, srt + 4 * a.rt_off[p]);
+    for (int d = 0; d < PF; ++d)
+      if (d < ks)
+#pragma unroll
+        for (int f = 0; f < 3; ++f) { fa[d][f] = A[f * fs + d * 64]; fb[d][f] = B[f * fs + d * 64]; }
+    for (int s0 = 0; s0 < ks; s0 += PF) {
+#pragma unroll
+      for (int d = 0; d < PF; ++d) {
+        const int s = s0 + d;
+        if (s < ks) {
+          bb = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][0], fb[d][0], bb, 0, 0, 0);
+          bc = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][0], fb[d][1], bc, 0, 0, 0);
+          bv = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][0], fb[d][2], bv, 0, 0, 0);
+          cbm = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][1], fb[d][0], cbm, 0, 0, 0);
+          cc = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][1], fb[d][1], cc, 0, 0, 0);
+          cv = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][1], fb[d][2], cv, 0, 0, 0);
+          vb = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][2], fb[d][0], vb, 0, 0, 0);
+          vc = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][2], fb[d][1], vc, 0, 0, 0);
+          vv = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][2], fb[d][2], vv, 0, 0, 0);
+          if (s + PF < ks) {
+            const int n = (s + PF) * 64;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) { fa[d][f] = A[f * fs + n]; fb[d][f] = B[f * fs + n]; }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int n11 = bb[r], n12 = bc[r], n21 = cbm[r], n22 = cc[r];
+      const int n10 = bv[r] - n11 - n12, n20 = cv[r] - n21 - n22;
+      const int n01 = vb[r] - n11 - n21, n02 = vc[r] - n12 - n22;
+      const int n00 = vv[r] - bv[r] - cv[r] - n01 - n02;
+      cnt[r][0] = n22; cnt[r][1] = n21; cnt[r][2] = n20; cnt[r][3] = n12; cnt[r][4] = n11;
+      cnt[r][5] = n10; cnt[r][6] = n02; cnt[r][7] = n01; cnt[r][8] = n00;
+    }
+  } else {
+    constexpr int PF = 4;
+    v4i fa[PF][2], fb[PF][2];
+#pragma unroll
+    for (int d = 0; d < PF; ++d)
+      if (d < ks)
+#pragma unroll
+        for (int f = 0; f < 2; ++f) { fa[d][f] = A[f * fs + d * 64]; fb[d][f] = B[f * fs + d * 64]; }
+    for (int s0 = 0; s0 < ks; s0 += PF) {
+#pragma unroll
+      for (int d = 0; d < PF; ++d) {
+        const int s = s0 + d;
+        if (s < ks) {
+          bb = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][0], fb[d][0], bb, 0, 0, 0);
+          bc = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][0], fb[d][1], bc, 0, 0, 0);
+          cbm = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][1], fb[d][0], cbm, 0, 0, 0);
+          cc = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][1], fb[d][1], cc, 0, 0, 0);
+          if (s + PF < ks) {
+            const int n = (s + PF) * 64;
+#pragma unroll
+            for (int f = 0; f < 2; ++f) { fa[d][f] = A[f * fs + n]; fb[d][f] = B[f * fs + n]; }
+          }
+        }
+      }
+    }
+    const int32_t* st = a.site_stats + (int64_t)p * 3 * a.V;
+    const int64_t j = cb * 32 + (lane & 31);
+    const int vbj = j < a.V ? st[j] : 0, vcj = j < a.V ? st[a.V + j] : 0;
+    const int K = a.K[p];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int64_t i = rb * 32 + mfma_row(lane, r);
+      const int bvi = i < a.V ? st[i] : 0, cvi = i < a.V ? st[a.V + i] : 0;
+      const int n11 = bb[r], n12 = bc[r], n21 = cbm[r], n22 = cc[r];
+      const int n10 = bvi - n11 - n12, n20 = cvi - n21 - n22;
+      const int n01 = vbj - n11 - n21, n02 = vcj - n12 - n22;
+      const int n00 = K - bvi - cvi - n01 - n02;
+      cnt[r][0] = n22; cnt[r][1] = n21; cnt[r][2] = n20; cnt[r][3] = n12; cnt[r][4] = n11;
+      cnt[r][5] = n10; cnt[r][6] = n02; cnt[r][7] = n01; cnt[r][8] = n00;
+    }
   }
-  HapSet<P>::eval(pp, x);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      w[r][q] = (uint32_t)cnt[r][2 * q] | ((uint32_t)cnt[r][2 * q + 1] << 16);
+    w[r][4] = (uint32_t)cnt[r][8];
+  }
 }
+
+// ---------------------------------------------------------------------------
+// Traits: what a tile's counts are and how a pair's statistics follow.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ const double* rt_entry(const FusedParams& a, const double* srt, int p,
+                                                  int n) {
+  int e = n - a.rt_lo[p];
+  e = e < 0 ? 0 : e;
+  return srt + 4 * (a.rt_off[p] + e);
+}
+
+template <int P>
+struct HapTraits {
+  static constexpr int NP = P;
+  static constexpr int NS = P == 1 ? HAP_NSTATS_P1 : P == 2 ? HAP_NSTATS_P2 : P == 3 ? HAP_NSTATS_P3
+                                                                                     : HAP_NSTATS_P4;
+  struct Counts { uint32_t w0[P][16], w1[P][16]; };
+  __device__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane, Counts& c) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) tile_counts(a, p, rb, cb, lane, c.w0[p], c.w1[p]);
+  }
+  __device__ static void stats(const FusedParams& a, const double* srt, const Counts& c, int r,
+                               double* x) {
+    HapPop pp[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int n11 = c.w0[p][r] & 0xffff, n10 = c.w0[p][r] >> 16;
+      const int n01 = c.w1[p][r] & 0xffff, n00 = c.w1[p][r] >> 16;
+      pp[p] = hap_pop(n11, n10, n01, n00, 0, rt_entry(a, srt, p, n11 + n10 + n01 + n00));
+    }
+    if constexpr (P == 1) hap_stats_P1(pp, x);
+    if constexpr (P == 2) hap_stats_P2(pp, x);
+    if constexpr (P == 3) hap_stats_P3(pp, x);
+    if constexpr (P == 4) hap_stats_P4(pp, x);
+  }
+};
+
+template <int P>
+struct GenoTraits {
+  static constexpr int NP = P;
+  static constexpr int NS = P == 1 ? GENO_NSTATS_P1 : P == 2 ? GENO_NSTATS_P2 : P == 3 ? GENO_NSTATS_P3
+                                                                                        : GENO_NSTATS_P4;
+  struct Counts { uint32_t w[P][16][5]; };
+  __device__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane, Counts& c) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) geno_tile_counts(a, p, rb, cb, lane, c.w[p]);
+  }
+  __device__ static void stats(const FusedParams& a, const double* srt, const Counts& c, int r,
+                               double* x) {
+    GenoPop pp[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      int g[9];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        g[2 * q] = c.w[p][r][q] & 0xffff;
+        g[2 * q + 1] = c.w[p][r][q] >> 16;
+      }
+      g[8] = c.w[p][r][4];
+      int n = 0;
+#pragma unroll
+      for (int m = 0; m < 9; ++m) n += g[m];
+      pp[p] = geno_pop(g, 0, rt_entry(a, srt, p, n));
+      pp[p].n = n;
+    }
+    if constexpr (P == 1) geno_stats_P1(pp, x);
+    if constexpr (P == 2) geno_stats_P2(pp, x);
+    if constexpr (P == 3) geno_stats_P3(pp, x);
+    if constexpr (P == 4) geno_stats_P4(pp, x);
+  }
+};
 
 template <int NS>
 __device__ __forceinline__ void flush_lane(unsigned long long* slimb, int bin, const double* acc,
@@ -236,9 +388,10 @@ __device__ __forceinline__ double pair_dist(const FusedParams& a, int64_t i, int
   return a.coord_is_pos ? (a.pos[j] - a.pos[i]) : fabs(a.coord[j] - a.coord[i]);
 }
 
-template <int P>
-__global__ void __launch_bounds__(256, 2) fused_hap_kernel(FusedParams a) {
-  constexpr int NS = HapSet<P>::NS;
+template <class T>
+__global__ void __launch_bounds__(256, 2) fused_kernel(FusedParams a) {
+  constexpr int NS = T::NS;
+  constexpr int P = T::NP;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   double* sedges = reinterpret_cast<double*>(smem);
   double* srt = sedges + PGLD_MAX_BINS + 8;
@@ -267,9 +420,8 @@ __global__ void __launch_bounds__(256, 2) fused_hap_kernel(FusedParams a) {
   for (int64_t tile = seg_lo + (blockIdx.x / nx) * 4 + wave; tile < seg_hi;
        tile += waves_per_xcd) {
     const int64_t rb = a.tiles[2 * tile], cb = a.tiles[2 * tile + 1];
-    uint32_t w0[P][16], w1[P][16];
-#pragma unroll
-    for (int p = 0; p < P; ++p) tile_counts(a, p, rb, cb, lane, w0[p], w1[p]);
+    typename T::Counts cts;
+    T::counts(a, rb, cb, lane, cts);
 
     // ---- tile classification (wave-uniform)
     const int64_t i0 = rb * 32, j0 = cb * 32;
@@ -290,7 +442,7 @@ __global__ void __launch_bounds__(256, 2) fused_hap_kernel(FusedParams a) {
 #pragma unroll 1
       for (int r = 0; r < 16; ++r) {
         double x[NS];
-        pair_stats<P>(a, srt, w0, w1, r, x);
+        T::stats(a, srt, cts, r, x);
 #pragma unroll
         for (int q = 0; q < NS; ++q) acc[q] += x[q];
       }
@@ -316,7 +468,7 @@ __global__ void __launch_bounds__(256, 2) fused_hap_kernel(FusedParams a) {
         }
         if (ok) {
           double x[NS];
-          pair_stats<P>(a, srt, w0, w1, r, x);
+          T::stats(a, srt, cts, r, x);
           if (b != cur) {
             if (cur >= 0) flush_lane<NS>(slimb, cur, acc, count);
             cur = b;
@@ -349,9 +501,10 @@ static int num_cus() {
   return g_num_cus;
 }
 
-template <int P>
-static int launch_fused_hap(const PanelLayout& L, const pgld_binned_args* g, hipStream_t s) {
-  constexpr int NS = HapSet<P>::NS;
+template <class T>
+static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStream_t s) {
+  constexpr int NS = T::NS;
+  constexpr int P = T::NP;
   FusedParams a;
   a.panels = g->d_panels;
   a.site_stats = g->d_site_stats;
@@ -386,14 +539,14 @@ static int launch_fused_hap(const PanelLayout& L, const pgld_binned_args* g, hip
   const size_t lds = (PGLD_MAX_BINS + 8) * sizeof(double) + (size_t)off * 4 * sizeof(double) +
                      (size_t)g->n_bins * (NS + 1) * 3 * sizeof(unsigned long long);
   if (lds > 160 * 1024) return fail(PGLD_EINVAL, "fused kernel: LDS tables exceed 160 KiB");
-  PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(fused_hap_kernel<P>),
+  PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(fused_kernel<T>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int grid = g->grid > 0 ? g->grid : 2 * num_cus();
   grid = ((grid + 7) / 8) * 8;
   const int64_t need = ((g->n_tiles + 31) / 32) * 8;  // at least 4 tiles per wave
   if (grid > need) grid = (int)need;
   if (grid < 8) grid = 8;
-  hipLaunchKernelGGL(fused_hap_kernel<P>, dim3(grid), dim3(256), lds, s, a);
+  hipLaunchKernelGGL(fused_kernel<T>, dim3(grid), dim3(256), lds, s, a);
   PGLD_LAUNCH_CHECK();
   return PGLD_OK;
 }
@@ -416,7 +569,9 @@ using namespace pgld;
 extern "C" int pgld_n_stats(int32_t mode, int32_t n_pops) {
   if (n_pops < 1 || n_pops > 4) return -1;
   static const int hap[5] = {0, HAP_NSTATS_P1, HAP_NSTATS_P2, HAP_NSTATS_P3, HAP_NSTATS_P4};
+  static const int geno[5] = {0, GENO_NSTATS_P1, GENO_NSTATS_P2, GENO_NSTATS_P3, GENO_NSTATS_P4};
   if (mode == PGLD_MODE_HAP) return hap[n_pops];
+  if (mode == PGLD_MODE_GENO) return geno[n_pops];
   return -1;
 }
 
@@ -431,8 +586,14 @@ extern "C" int pgld_binned_fused(const pgld_panel_desc* desc, const pgld_binned_
   hipStream_t s = as_stream(stream);
   if (desc->mode == PGLD_MODE_HAP) {
     switch (desc->n_pops) {
-      case 1: return launch_fused_hap<1>(L, args, s);
-      case 2: return launch_fused_hap<2>(L, args, s);
+      case 1: return launch_fused<HapTraits<1>>(L, args, s);
+      case 2: return launch_fused<HapTraits<2>>(L, args, s);
+      default: break;
+    }
+  } else {
+    switch (desc->n_pops) {
+      case 1: return launch_fused<GenoTraits<1>>(L, args, s);
+      case 2: return launch_fused<GenoTraits<2>>(L, args, s);
       default: break;
     }
   }

@@ -12,6 +12,7 @@ Run:  python pg_gpu_amd/csrc/gen_stats.py   (writes stats_hap.inc beside it)
 """
 from __future__ import annotations
 
+import json
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
@@ -138,6 +139,101 @@ def emit_set(prefix, popty, P, keyfn):
     return "\n".join(lines)
 
 
+def geno_key(kind, pp):
+    """Canonical device function for one raw genotype call (case dispatch of
+    pg_gpu/genotype_kernels.py:561-878)."""
+    if kind == "dd":
+        i, j = pp
+        return ("geno_dd_same", (i,)) if i == j else ("geno_dd_pair", tuple(sorted(pp)))
+    if kind == "dz":
+        i, j, k = pp
+        if i == j == k:
+            return ("geno_dz_same", (i,))
+        if i == j:
+            return ("geno_dz_iik", (i, k))
+        if i == k:
+            return ("geno_dz_iji", (i, j))
+        if j == k:
+            return ("geno_dz_ijj", (i, j))
+        return ("geno_dz_ijk", (i, j, k))
+    i, j, k, l = pp
+    cnt = {}
+    for p in pp:
+        cnt[p] = cnt.get(p, 0) + 1
+    if len(cnt) == 1:
+        return ("geno_pi2_same", (i,))
+    if max(cnt.values()) == 3:
+        t = [p for p, c in cnt.items() if c == 3][0]
+        s = [p for p, c in cnt.items() if c == 1][0]
+        return ("geno_pi2_iiij" if (i == t and j == t) else "geno_pi2_ijii", (t, s))
+    if i == j and k == l:
+        return ("geno_pi2_iikk", (i, k))
+    if i == j:
+        return ("geno_pi2_iikl", (i, min(k, l), max(k, l)))
+    if k == l:
+        return ("geno_pi2_ijkk", (min(i, j), max(i, j), k))
+    if (i == k and j == l) or (i == l and j == k):
+        return ("geno_pi2_ijij", (i, j))
+    if len(cnt) == 3:
+        if i == k:
+            s, a, b = i, j, l
+        elif i == l:
+            s, a, b = i, j, k
+        elif j == k:
+            s, a, b = j, i, l
+        else:
+            s, a, b = j, i, k
+        return ("geno_pi2_shared", (s, a, b))
+    return ("geno_pi2_alldiff", (min(i, j), max(i, j), min(k, l), max(k, l)))
+
+
+def nested_poly(table, var="g"):
+    """C expression for sum(coef * prod var[m]^e_m), nested by the lowest
+    variable index (Horner in several variables).  With integer coefficients
+    and integer variables every intermediate is an integer, so the FP64
+    evaluation is exact while the magnitudes stay below 2^53."""
+    monos = []
+    for exps, coef in table:
+        c = int(coef)
+        assert str(c) == coef, "sub-polynomials have integer coefficients"
+        idx = tuple(m for m, e in enumerate(exps) for _ in range(e))
+        monos.append((idx, c))
+
+    def emit(ms):
+        const = sum(c for idx, c in ms if not idx)
+        groups = {}
+        for idx, c in ms:
+            if idx:
+                groups.setdefault(idx[0], []).append((idx[1:], c))
+        terms = []
+        for m in sorted(groups):
+            inner = emit(groups[m])
+            terms.append(f"{var}[{m}] * ({inner})" if inner not in ("1.0",) else f"{var}[{m}]")
+        if const:
+            terms.append(f"{float(const)!r}")
+        if not terms:
+            return "0.0"
+        return " + ".join(terms)
+
+    return emit(monos)
+
+
+def emit_geno_polys():
+    data = json.loads((HERE / "geno_kernel_polys.json").read_text())
+    out = ["// Generated by gen_stats.py from geno_kernel_polys.json -- do not edit by hand.",
+           "// Integer sub-polynomials of the single-population genotype estimators",
+           "// (pg_gpu/genotype_kernels.py:27-71 poly, :235-272 quartic, :364-405 corr),",
+           "// in moments class order g[0] = n22 ... g[8] = n00.",
+           "#pragma once", ""]
+    for name, fn in (("dd_poly", "geno_dd_poly"), ("dz_quartic", "geno_dz_quartic"),
+                     ("pi2_corr", "geno_pi2_corr")):
+        out.append(f"__device__ __forceinline__ double {fn}(const double* g) {{")
+        out.append(f"  return {nested_poly(data[name])};")
+        out.append("}")
+        out.append("")
+    (HERE / "geno_polys.inc").write_text("\n".join(out))
+
+
 def main():
     parts = ["// Generated by gen_stats.py -- do not edit by hand.",
              "#pragma once",
@@ -146,6 +242,14 @@ def main():
         parts.append(emit_set("hap", "HapPop", P, hap_key))
         parts.append("")
     (HERE / "stats_hap.inc").write_text("\n".join(parts))
+    emit_geno_polys()
+    parts = ["// Generated by gen_stats.py -- do not edit by hand.",
+             "#pragma once",
+             '#include "geno_estimators.h"', ""]
+    for P in range(1, 5):
+        parts.append(emit_set("geno", "GenoPop", P, geno_key))
+        parts.append("")
+    (HERE / "stats_geno.inc").write_text("\n".join(parts))
 
 
 if __name__ == "__main__":

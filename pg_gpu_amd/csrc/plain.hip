@@ -1,14 +1,15 @@
 // Plain per-pair FP64 comparator and the dense count-block kernel.
 //
-// plain_hap_kernel is the straightforward GPU form of the reference's chunk
+// plain_kernel is the straightforward GPU form of the reference's chunk
 // loop: one workgroup per anchor site i, one thread per partner j, a loop over
 // the population's sample rows of the raw int8 matrix to form the four
-// haplotype counts (the reference does this with gathers and boolean
-// reductions, pg_gpu/ld_pipeline.py:155-172), the same FP64 estimators as the
-// fused kernel, and per-workgroup bin partials.  It reports the same
+// haplotype (or nine genotype) counts (the reference does this with gathers
+// and boolean reductions, pg_gpu/ld_pipeline.py:155-172 and :216-250), the
+// same FP64 estimators as the fused kernel, and per-workgroup bin partials.  It reports the same
 // fixed-point output as the fused kernel so the two can be compared exactly.
 #include "pgld_internal.h"
 #include "stats_hap.inc"
+#include "stats_geno.inc"
 
 namespace pgld {
 
@@ -30,14 +31,47 @@ struct PlainParams {
   int64_t V;
 };
 
-template <int P> struct PlainSet;
-template <> struct PlainSet<1> {
-  static constexpr int NS = HAP_NSTATS_P1;
-  __device__ static void eval(const HapPop* p, double* o) { hap_stats_P1(p, o); }
+// Per-pair counting straight from the int8 matrix, and the statistic set.
+template <int MODE, int P> struct PlainSet;
+template <int P> struct PlainSet<PGLD_MODE_HAP, P> {
+  static constexpr int NS = P == 1 ? HAP_NSTATS_P1 : HAP_NSTATS_P2;
+  using Pop = HapPop;
+  __device__ static Pop pop(const PlainParams& a, const int32_t* srows, int p, int64_t i,
+                            int64_t j) {
+    int n11 = 0, n10 = 0, n01 = 0, n00 = 0;
+    for (int r = a.row_offset[p]; r < a.row_offset[p + 1]; ++r) {
+      const int8_t* row = a.mat + (int64_t)srows[r] * a.ld;
+      const int x = row[i], y = row[j];
+      n11 += (x == 1) & (y == 1);
+      n10 += (x == 1) & (y == 0);
+      n01 += (x == 0) & (y == 1);
+      n00 += (x == 0) & (y == 0);
+    }
+    return hap_pop(n11, n10, n01, n00, n11 + n10 + n01 + n00, a.rtab);
+  }
+  __device__ static void eval(const Pop* p, double* o) {
+    if constexpr (P == 1) hap_stats_P1(p, o); else hap_stats_P2(p, o);
+  }
 };
-template <> struct PlainSet<2> {
-  static constexpr int NS = HAP_NSTATS_P2;
-  __device__ static void eval(const HapPop* p, double* o) { hap_stats_P2(p, o); }
+template <int P> struct PlainSet<PGLD_MODE_GENO, P> {
+  static constexpr int NS = P == 1 ? GENO_NSTATS_P1 : GENO_NSTATS_P2;
+  using Pop = GenoPop;
+  __device__ static Pop pop(const PlainParams& a, const int32_t* srows, int p, int64_t i,
+                            int64_t j) {
+    // class of (x, y) in moments order: n22, n21, n20, n12, n11, n10, n02, n01, n00
+    int c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int r = a.row_offset[p]; r < a.row_offset[p + 1]; ++r) {
+      const int8_t* row = a.mat + (int64_t)srows[r] * a.ld;
+      const int x = row[i], y = row[j];
+      if (x >= 0 && y >= 0) c[(2 - x) * 3 + (2 - y)] += 1;
+    }
+    int n = 0;
+    for (int m = 0; m < 9; ++m) n += c[m];
+    return geno_pop(c, n, a.rtab);
+  }
+  __device__ static void eval(const Pop* p, double* o) {
+    if constexpr (P == 1) geno_stats_P1(p, o); else geno_stats_P2(p, o);
+  }
 };
 
 template <int NS>
@@ -54,9 +88,10 @@ __device__ __forceinline__ void plain_flush(unsigned long long* slimb, int bin, 
   atomicAdd(dst + 3 * NS, (unsigned long long)count);
 }
 
-template <int P>
-__global__ void __launch_bounds__(256) plain_hap_kernel(PlainParams a) {
-  constexpr int NS = PlainSet<P>::NS;
+template <int MODE, int P>
+__global__ void __launch_bounds__(256) plain_kernel(PlainParams a) {
+  using S = PlainSet<MODE, P>;
+  constexpr int NS = S::NS;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   double* sedges = reinterpret_cast<double*>(smem);
   unsigned long long* slimb = reinterpret_cast<unsigned long long*>(sedges + PGLD_MAX_BINS + 8);
@@ -90,21 +125,10 @@ __global__ void __launch_bounds__(256) plain_hap_kernel(PlainParams a) {
     const int b = pgld_bin_scan(d, sedges, nb, guess, nb - 1);
     if (b < 0 || b >= nb) continue;
     guess = b;
-    HapPop pp[P];
-    for (int p = 0; p < P; ++p) {
-      int n11 = 0, n10 = 0, n01 = 0, n00 = 0;
-      for (int r = a.row_offset[p]; r < a.row_offset[p + 1]; ++r) {
-        const int8_t* row = a.mat + (int64_t)srows[r] * a.ld;
-        const int x = row[i], y = row[j];
-        n11 += (x == 1) & (y == 1);
-        n10 += (x == 1) & (y == 0);
-        n01 += (x == 0) & (y == 1);
-        n00 += (x == 0) & (y == 0);
-      }
-      pp[p] = hap_pop(n11, n10, n01, n00, n11 + n10 + n01 + n00, a.rtab);
-    }
+    typename S::Pop pp[P];
+    for (int p = 0; p < P; ++p) pp[p] = S::pop(a, srows, p, i, j);
     double x[NS];
-    PlainSet<P>::eval(pp, x);
+    S::eval(pp, x);
     if (b != cur) {
       if (cur >= 0) plain_flush<NS>(slimb, cur, acc, count);
       cur = b;
@@ -130,9 +154,9 @@ __global__ void __launch_bounds__(256) plain_hap_kernel(PlainParams a) {
   }
 }
 
-template <int P>
-static int launch_plain_hap(const PanelLayout& L, const pgld_binned_args* g, hipStream_t s) {
-  constexpr int NS = PlainSet<P>::NS;
+template <int MODE, int P>
+static int launch_plain(const PanelLayout& L, const pgld_binned_args* g, hipStream_t s) {
+  constexpr int NS = PlainSet<MODE, P>::NS;
   PlainParams a;
   a.mat = g->d_mat;
   a.ld = g->ld;
@@ -154,10 +178,10 @@ static int launch_plain_hap(const PanelLayout& L, const pgld_binned_args* g, hip
                      (size_t)g->n_bins * (NS + 1) * 3 * sizeof(unsigned long long) +
                      (size_t)a.row_offset[P] * sizeof(int32_t);
   if (lds > 160 * 1024) return fail(PGLD_EINVAL, "plain kernel: too many sample rows for LDS");
-  PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(plain_hap_kernel<P>),
+  PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(plain_kernel<MODE, P>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (L.n_sites < 2) return PGLD_OK;
-  hipLaunchKernelGGL(plain_hap_kernel<P>, dim3((unsigned)(L.n_sites - 1)), dim3(256), lds, s, a);
+  hipLaunchKernelGGL((plain_kernel<MODE, P>), dim3((unsigned)(L.n_sites - 1)), dim3(256), lds, s, a);
   PGLD_LAUNCH_CHECK();
   return PGLD_OK;
 }
@@ -206,6 +230,48 @@ __global__ void __launch_bounds__(64) count_block_hap_kernel(
   }
 }
 
+// Genotype variant: nine planes in moments class order n22 .. n00.
+__global__ void __launch_bounds__(64) count_block_geno_kernel(
+    const int8_t* __restrict__ panels, int64_t block_bytes, int64_t pop_off, int ks, int missing,
+    const int32_t* __restrict__ st, int K, int64_t V, int64_t i0, int64_t ni, int64_t j0,
+    int64_t nj, int32_t* __restrict__ out) {
+  const int lane = threadIdx.x;
+  const int64_t rb = i0 / 32 + blockIdx.y, cb = j0 / 32 + blockIdx.x;
+  const v4i* A = reinterpret_cast<const v4i*>(panels + rb * block_bytes + pop_off) + lane;
+  const v4i* B = reinterpret_cast<const v4i*>(panels + cb * block_bytes + pop_off) + lane;
+  const int nf = missing ? 3 : 2;
+  v16i pr[3][3];
+  for (int x = 0; x < 3; ++x)
+    for (int y = 0; y < 3; ++y)
+      for (int q = 0; q < 16; ++q) pr[x][y][q] = 0;
+  for (int s = 0; s < ks; ++s) {
+    v4i fa[3], fb[3];
+    for (int f = 0; f < nf; ++f) { fa[f] = A[(f * ks + s) * 64]; fb[f] = B[(f * ks + s) * 64]; }
+    for (int x = 0; x < nf; ++x)
+      for (int y = 0; y < nf; ++y)
+        pr[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[x], fb[y], pr[x][y], 0, 0, 0);
+  }
+  const int64_t j = cb * 32 + (lane & 31);
+  for (int r = 0; r < 16; ++r) {
+    const int64_t i = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (i >= i0 + ni || j >= j0 + nj || i >= V || j >= V) continue;
+    const int n11 = pr[0][0][r], n12 = pr[0][1][r], n21 = pr[1][0][r], n22 = pr[1][1][r];
+    int bv, cv, vb, vc, vv;
+    if (missing) {
+      bv = pr[0][2][r]; cv = pr[1][2][r]; vb = pr[2][0][r]; vc = pr[2][1][r]; vv = pr[2][2][r];
+    } else {
+      bv = st[i]; cv = st[V + i]; vb = st[j]; vc = st[V + j]; vv = K;
+    }
+    const int n10 = bv - n11 - n12, n20 = cv - n21 - n22;
+    const int n01 = vb - n11 - n21, n02 = vc - n12 - n22;
+    const int n00 = vv - bv - cv - n01 - n02;
+    const int c[9] = {n22, n21, n20, n12, n11, n10, n02, n01, n00};
+    const int64_t o = (i - i0) * nj + (j - j0);
+    const int64_t plane = ni * nj;
+    for (int m = 0; m < 9; ++m) out[m * plane + o] = c[m];
+  }
+}
+
 }  // namespace pgld
 
 using namespace pgld;
@@ -219,8 +285,14 @@ extern "C" int pgld_binned_plain(const pgld_panel_desc* desc, const pgld_binned_
   hipStream_t s = as_stream(stream);
   if (desc->mode == PGLD_MODE_HAP) {
     switch (desc->n_pops) {
-      case 1: return launch_plain_hap<1>(L, args, s);
-      case 2: return launch_plain_hap<2>(L, args, s);
+      case 1: return launch_plain<PGLD_MODE_HAP, 1>(L, args, s);
+      case 2: return launch_plain<PGLD_MODE_HAP, 2>(L, args, s);
+      default: break;
+    }
+  } else {
+    switch (desc->n_pops) {
+      case 1: return launch_plain<PGLD_MODE_GENO, 1>(L, args, s);
+      case 2: return launch_plain<PGLD_MODE_GENO, 2>(L, args, s);
       default: break;
     }
   }
@@ -237,11 +309,9 @@ extern "C" int pgld_count_block(const pgld_panel_desc* desc, const int8_t* d_pan
   if (i0 % 32 || j0 % 32 || i0 < 0 || j0 < 0 || ni < 0 || nj < 0)
     return fail(PGLD_EINVAL, "i0/j0 must be non-negative multiples of 32");
   if (i0 + ni > L.n_sites || j0 + nj > L.n_sites) return fail(PGLD_EINVAL, "block out of range");
-  if (desc->mode != PGLD_MODE_HAP)
-    return fail(PGLD_EUNSUPPORTED, "count_block: genotype mode not built yet");
   if (ni == 0 || nj == 0) return PGLD_OK;
   dim3 grid((unsigned)((nj + 31) / 32), (unsigned)((ni + 31) / 32));
-  hipLaunchKernelGGL(count_block_hap_kernel, grid, dim3(64), 0, as_stream(stream), d_panels,
+  hipLaunchKernelGGL(desc->mode == PGLD_MODE_HAP ? count_block_hap_kernel : count_block_geno_kernel, grid, dim3(64), 0, as_stream(stream), d_panels,
                      L.block_bytes, L.pop_off[pop], L.ksteps[pop], L.missing[pop],
                      d_site_stats + (int64_t)pop * 3 * L.n_sites, L.pop_size[pop], L.n_sites, i0,
                      ni, j0, nj, d_counts);

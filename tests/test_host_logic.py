@@ -108,6 +108,38 @@ def test_generated_stat_plan_matches_specs(P):
     assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)) < 1e-13
 
 
+@pytest.mark.parametrize("P", [1, 2, 3, 4])
+def test_generated_geno_stat_plan_matches_specs(P):
+    """Same for the genotype plan: merging raw calls by geno_key (one device
+    function per case of genotype_kernels.py) loses nothing."""
+    from oracle import geno_oracle as go
+
+    gen = load_gen()
+    values, stats = gen.stat_set_plan(P, gen.geno_key)
+    rng = np.random.default_rng(10 + P)
+    counts = [rng.integers(0, 12, (200, 9)) for _ in range(P)]
+    pops = [go.GenoPop(c) for c in counts]
+    rep = {
+        "geno_dd_same": ("dd", (0, 0)), "geno_dd_pair": ("dd", (0, 1)),
+        "geno_dz_same": ("dz", (0, 0, 0)), "geno_dz_iik": ("dz", (0, 0, 1)),
+        "geno_dz_iji": ("dz", (0, 1, 0)), "geno_dz_ijj": ("dz", (0, 1, 1)),
+        "geno_dz_ijk": ("dz", (0, 1, 2)), "geno_pi2_same": ("pi2", (0, 0, 0, 0)),
+        "geno_pi2_iiij": ("pi2", (0, 0, 0, 1)), "geno_pi2_ijii": ("pi2", (0, 1, 0, 0)),
+        "geno_pi2_iikk": ("pi2", (0, 0, 1, 1)), "geno_pi2_iikl": ("pi2", (0, 0, 1, 2)),
+        "geno_pi2_ijkk": ("pi2", (0, 1, 2, 2)), "geno_pi2_ijij": ("pi2", (0, 1, 0, 1)),
+        "geno_pi2_shared": ("pi2", (0, 1, 0, 2)), "geno_pi2_alldiff": ("pi2", (0, 1, 2, 3)),
+    }
+    vals = []
+    for f, args in values:
+        kind, pp = rep[f]
+        vals.append(go.kernel_call([pops[x] for x in args], kind, pp))
+    got = np.stack([sum(w * vals[v] for v, w in terms) for terms in stats], axis=1)
+    ref = go.multi_pop_stats_geno(counts, [None] * P)
+    assert got.shape == ref.shape
+    scale = np.maximum(np.abs(ref), 1e-12)
+    assert np.max(np.abs(got - ref) / scale) < 1e-12
+
+
 def header_functions():
     text = (ROOT / "include/pgld.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
