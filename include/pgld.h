@@ -91,7 +91,8 @@ typedef struct {
   const int32_t* d_rows;       /* raw matrix rows per pop, concatenated (plain) */
   int32_t row_offset[PGLD_MAX_POPS + 1]; /* pop p uses d_rows[row_offset[p]..) */
   const double* d_pos;         /* n_sites, sorted, the cutoff coordinate (bp) */
-  const double* d_coord;       /* n_sites, the binning coordinate (bp or Morgans) */
+  const double* d_coord;       /* n_sites, the binning coordinate (bp or Morgans),
+                                  non-decreasing along the sites */
   int32_t coord_is_pos;        /* 1: d_coord == d_pos, distances are pos_j - pos_i */
   double max_dist;             /* pairs with pos_j <= pos_i + max_dist (float64) */
   const double* d_edges;       /* n_bins + 1 increasing bin edges */

@@ -1,4 +1,4 @@
-// Fused binned two-locus LD for haplotype data on gfx950.
+// Fused binned two-locus LD for haplotype and genotype data on gfx950.
 //
 // Every wave owns whole 32 x 32 site tiles of the distance band (row block rb,
 // column block cb >= rb) and processes each one end to end, with no
@@ -20,8 +20,9 @@
 //      atomics; at exit each workgroup adds its limbs to the global result.
 //      Integer addition is associative, so the bin sums are bitwise
 //      reproducible whatever the schedule.
-// With two waves per SIMD one wave's MFMA/loads overlap the other's FP64
-// epilogue.  No pair list and no per-pair statistic ever reaches HBM.
+// With two waves per SIMD (haplotypes, and genotypes of one population) one
+// wave's MFMA/loads overlap the other's FP64 epilogue; two-population
+// genotype sets need the full register file of a one-wave-per-SIMD launch.  No pair list and no per-pair statistic ever reaches HBM.
 #include "pgld_internal.h"
 #include "stats_hap.inc"
 #include "stats_geno.inc"
@@ -52,6 +53,11 @@ struct FusedParams {
   int32_t rt_len;                 // total LDS table entries
 };
 
+// 16 packed count words per lane (one per MFMA output slot).  Kept as a
+// vector value so that the epilogue's wave-uniform slot index r selects a
+// register (index mode) instead of spilling the tile's counts to scratch.
+typedef uint32_t v16u __attribute__((ext_vector_type(16)));
+
 __device__ __forceinline__ v16i zero16() {
   v16i z;
 #pragma unroll
@@ -68,7 +74,7 @@ __device__ __forceinline__ int mfma_row(int lane, int r) {
 // Counts of population p for tile (rb, cb), packed per slot as
 // w0 = n11 | n10 << 16, w1 = n01 | n00 << 16.
 __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t rb, int64_t cb,
-                                            int lane, uint32_t (&w0)[16], uint32_t (&w1)[16]) {
+                                            int lane, v16u& w0, v16u& w1) {
   const int ks = a.ksteps[p];
   const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[p]) + lane;
   const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[p]) + lane;
@@ -149,8 +155,9 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
 // Packed per slot in moments order g1..g9 (n22, n21, n20, n12, n11, n10, n02,
 // n01, n00) as five 16-bit pairs.
 // ---------------------------------------------------------------------------
+template <int W>
 __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, int64_t rb,
-                                                 int64_t cb, int lane, uint32_t (&w)[16][5]) {
+                                                 int64_t cb, int lane, v16u (&w)[W]) {
   const int ks = a.ksteps[p];
   const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[p]) + lane;
   const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[p]) + lane;
@@ -240,10 +247,17 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
   }
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
+    if constexpr (W == 3) {  // 10-bit fields: every count <= population size <= 1023
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-      w[r][q] = (uint32_t)cnt[r][2 * q] | ((uint32_t)cnt[r][2 * q + 1] << 16);
-    w[r][4] = (uint32_t)cnt[r][8];
+      for (int q = 0; q < 3; ++q)
+        w[q][r] = (uint32_t)cnt[r][3 * q] | ((uint32_t)cnt[r][3 * q + 1] << 10) |
+                  ((uint32_t)cnt[r][3 * q + 2] << 20);
+    } else {  // 16-bit fields
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        w[q][r] = (uint32_t)cnt[r][2 * q] | ((uint32_t)cnt[r][2 * q + 1] << 16);
+      w[4][r] = (uint32_t)cnt[r][8];
+    }
   }
 }
 
@@ -260,9 +274,10 @@ __device__ __forceinline__ const double* rt_entry(const FusedParams& a, const do
 template <int P>
 struct HapTraits {
   static constexpr int NP = P;
+  static constexpr int MINB = 2;  // workgroups per CU: 2 waves per SIMD
   static constexpr int NS = P == 1 ? HAP_NSTATS_P1 : P == 2 ? HAP_NSTATS_P2 : P == 3 ? HAP_NSTATS_P3
                                                                                      : HAP_NSTATS_P4;
-  struct Counts { uint32_t w0[P][16], w1[P][16]; };
+  struct Counts { v16u w0[P], w1[P]; };
   __device__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane, Counts& c) {
 #pragma unroll
     for (int p = 0; p < P; ++p) tile_counts(a, p, rb, cb, lane, c.w0[p], c.w1[p]);
@@ -283,15 +298,20 @@ struct HapTraits {
   }
 };
 
-template <int P>
+// W = 3: nine 10-bit counts in three words per slot (population sizes up to
+// 1023); W = 5: 16-bit counts in five words (up to 32767).
+template <int P, int W>
 struct GenoTraits {
   static constexpr int NP = P;
+  // Two populations need more than the 256 registers a wave gets at two
+  // waves per SIMD; they run one wave per SIMD with the full register file.
+  static constexpr int MINB = P == 1 ? 2 : 1;
   static constexpr int NS = P == 1 ? GENO_NSTATS_P1 : P == 2 ? GENO_NSTATS_P2 : P == 3 ? GENO_NSTATS_P3
                                                                                         : GENO_NSTATS_P4;
-  struct Counts { uint32_t w[P][16][5]; };
+  struct Counts { v16u w[P][W]; };
   __device__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane, Counts& c) {
 #pragma unroll
-    for (int p = 0; p < P; ++p) geno_tile_counts(a, p, rb, cb, lane, c.w[p]);
+    for (int p = 0; p < P; ++p) geno_tile_counts<W>(a, p, rb, cb, lane, c.w[p]);
   }
   __device__ static void stats(const FusedParams& a, const double* srt, const Counts& c, int r,
                                double* x) {
@@ -299,12 +319,23 @@ struct GenoTraits {
 #pragma unroll
     for (int p = 0; p < P; ++p) {
       int g[9];
+      if constexpr (W == 3) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        g[2 * q] = c.w[p][r][q] & 0xffff;
-        g[2 * q + 1] = c.w[p][r][q] >> 16;
+        for (int q = 0; q < 3; ++q) {
+          const uint32_t v = c.w[p][q][r];
+          g[3 * q] = v & 0x3ff;
+          g[3 * q + 1] = (v >> 10) & 0x3ff;
+          g[3 * q + 2] = v >> 20;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t v = c.w[p][q][r];
+          g[2 * q] = v & 0xffff;
+          g[2 * q + 1] = v >> 16;
+        }
+        g[8] = c.w[p][4][r];
       }
-      g[8] = c.w[p][r][4];
       int n = 0;
 #pragma unroll
       for (int m = 0; m < 9; ++m) n += g[m];
@@ -318,52 +349,55 @@ struct GenoTraits {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Binning.  Each wave owns an FP64 staging array stage[bin][stat] and a pair
+// counter per bin in LDS.  Inside a tile, lanes add their running per-bin
+// sums to the staging array (LDS FP64 atomics, wave-private region); at the
+// tile end the wave converts the touched bins to 3-limb fixed point, adds
+// them to the workgroup's integer limbs and clears the staging entries.  The
+// FP64 rounding is thus confined to the 1024 pairs of one tile, and the
+// integer limb sums across tiles, waves and workgroups are exact, so the
+// result does not depend on how tiles are dealt to waves.
+// ---------------------------------------------------------------------------
 template <int NS>
-__device__ __forceinline__ void flush_lane(unsigned long long* slimb, int bin, const double* acc,
-                                           int count) {
-  unsigned long long* dst = slimb + (int64_t)bin * (NS + 1) * 3;
+__device__ __forceinline__ void stage_add(double* stg, int* scnt, int bin, const double* acc,
+                                          int count) {
+  double* d = stg + bin * NS;
 #pragma unroll
-  for (int q = 0; q < NS; ++q) {
-    long long x2, x1, x0;
-    pgld_to_limbs(acc[q], x2, x1, x0);
-    atomicAdd(dst + 3 * q + 0, (unsigned long long)x0);
-    atomicAdd(dst + 3 * q + 1, (unsigned long long)x1);
-    atomicAdd(dst + 3 * q + 2, (unsigned long long)x2);
-  }
-  atomicAdd(dst + 3 * NS, (unsigned long long)count);
+  for (int q = 0; q < NS; ++q) atomicAdd(d + q, acc[q]);
+  atomicAdd(scnt + bin, count);
 }
 
-// Sum the lanes whose running bin is `bin` and add it once (lane 0).
+// Move staged bins [blo, bhi] of this wave into the workgroup limbs.
 template <int NS>
-__device__ __forceinline__ void flush_wave_bin(unsigned long long* slimb, int lane, int bin,
-                                               bool mine, const double* acc, int count) {
-  unsigned long long* dst = slimb + (int64_t)bin * (NS + 1) * 3;
-#pragma unroll
-  for (int q = 0; q < NS; ++q) {
-    const double v = pgld_wave_sum(mine ? acc[q] : 0.0);
-    if (lane == 0) {
-      long long x2, x1, x0;
-      pgld_to_limbs(v, x2, x1, x0);
-      atomicAdd(dst + 3 * q + 0, (unsigned long long)x0);
-      atomicAdd(dst + 3 * q + 1, (unsigned long long)x1);
-      atomicAdd(dst + 3 * q + 2, (unsigned long long)x2);
+__device__ __forceinline__ void stage_drain(double* stg, int* scnt, unsigned long long* slimb,
+                                            int lane, int blo, int bhi) {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const int E = (bhi - blo + 1) * (NS + 1);
+  for (int e = lane; e < E; e += 64) {
+    const int b = blo + e / (NS + 1), q = e - (b - blo) * (NS + 1);
+    unsigned long long* dst = slimb + ((int64_t)b * (NS + 1) + q) * 3;
+    if (q < NS) {
+      const double v = stg[b * NS + q];
+      if (v != 0.0) {
+        stg[b * NS + q] = 0.0;
+        long long x2, x1, x0;
+        pgld_to_limbs(v, x2, x1, x0);
+        atomicAdd(dst + 0, (unsigned long long)x0);
+        atomicAdd(dst + 1, (unsigned long long)x1);
+        atomicAdd(dst + 2, (unsigned long long)x2);
+      }
+    } else {
+      const int c = scnt[b];
+      if (c) {
+        scnt[b] = 0;
+        atomicAdd(dst, (unsigned long long)c);
+      }
     }
   }
-  const int c = pgld_wave_sum_int(mine ? count : 0);
-  if (lane == 0) atomicAdd(dst + 3 * NS, (unsigned long long)c);
-}
-
-template <int NS>
-__device__ __forceinline__ void flush_wave(unsigned long long* slimb, int lane, int cur,
-                                           const double* acc, int count) {
-  int bmax = cur;
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) bmax = max(bmax, __shfl_xor(bmax, off, 64));
-  const bool uniform = __all(cur < 0 || cur == bmax);
-  if (bmax >= 0 && uniform)
-    flush_wave_bin<NS>(slimb, lane, bmax, cur == bmax, acc, count);
-  else if (cur >= 0)
-    flush_lane<NS>(slimb, cur, acc, count);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 }
 
 // Adds the workgroup's LDS limbs to the global result (carry-normalised).
@@ -384,33 +418,43 @@ __device__ __forceinline__ void flush_global(unsigned long long* slimb, unsigned
   }
 }
 
-__device__ __forceinline__ double pair_dist(const FusedParams& a, int64_t i, int64_t j) {
-  return a.coord_is_pos ? (a.pos[j] - a.pos[i]) : fabs(a.coord[j] - a.coord[i]);
+// Signed coordinate difference c[j] - c[i]; for i < j it is the pair's
+// distance (coordinates are non-decreasing along the sites), and over a tile
+// it is smallest at (i1, j0) and largest at (i0, j1).
+__device__ __forceinline__ double coord_diff(const FusedParams& a, int64_t i, int64_t j) {
+  return a.coord_is_pos ? (a.pos[j] - a.pos[i]) : (a.coord[j] - a.coord[i]);
 }
 
 template <class T>
-__global__ void __launch_bounds__(256, 2) fused_kernel(FusedParams a) {
+__global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
   constexpr int NS = T::NS;
-  constexpr int P = T::NP;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   double* sedges = reinterpret_cast<double*>(smem);
   double* srt = sedges + PGLD_MAX_BINS + 8;
-  unsigned long long* slimb = reinterpret_cast<unsigned long long*>(srt + 4 * a.rt_len);
   const int nb = a.n_bins;
+  double* sstage = srt + 4 * a.rt_len;                       // [4 waves][nb][NS]
+  unsigned long long* slimb =
+      reinterpret_cast<unsigned long long*>(sstage + 4 * nb * NS);  // [nb][NS+1][3]
+  int* scnt = reinterpret_cast<int*>(slimb + nb * (NS + 1) * 3);    // [4 waves][nb]
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   for (int e = t; e <= nb; e += blockDim.x) sedges[e] = a.edges[e];
-  for (int p = 0; p < P; ++p) {
+  for (int p = 0; p < T::NP; ++p) {
     const int len = a.K[p] - a.rt_lo[p] + 1;
     for (int e = t; e < 4 * len; e += blockDim.x)
       srt[4 * a.rt_off[p] + e] = a.rtab[4 * a.rt_lo[p] + e];
   }
+  for (int e = t; e < 4 * nb * NS; e += blockDim.x) sstage[e] = 0.0;
   for (int e = t; e < nb * (NS + 1) * 3; e += blockDim.x) slimb[e] = 0ull;
+  for (int e = t; e < 4 * nb; e += blockDim.x) scnt[e] = 0;
   __syncthreads();
+  double* stg = sstage + wave * nb * NS;
+  int* scw = scnt + wave * nb;
 
   // XCD-aware split: workgroups are dealt round-robin to the 8 XCDs, so each
-  // XCD gets a contiguous stretch of the (row-major) tile list; neighbouring
-  // tiles share row panels, which then stay in that XCD's L2.
+  // XCD gets a contiguous stretch of the tile list (ordered in L2-sized
+  // squares of tiles); the waves of one XCD then share operand panels in
+  // that XCD's L2.
   const int nx = 8;
   const int xcd = blockIdx.x % nx;
   const int waves_per_xcd = (gridDim.x / nx) * 4;
@@ -423,22 +467,20 @@ __global__ void __launch_bounds__(256, 2) fused_kernel(FusedParams a) {
     typename T::Counts cts;
     T::counts(a, rb, cb, lane, cts);
 
-    // ---- tile classification (wave-uniform)
+    // ---- tile classification (wave-uniform).  Coordinates are
+    // non-decreasing along the sites, so the corner pairs (i1, j0) and
+    // (i0, j1) bound the bins of every pair in the tile.
     const int64_t i0 = rb * 32, j0 = cb * 32;
     const int64_t i1 = min(i0 + 31, a.V - 1), j1 = min(j0 + 31, a.V - 1);
-    bool fast = (cb > rb) && (j1 == j0 + 31) && (a.pos[j1] <= a.pos[i0] + a.max_dist);
-    int bfast = -1;
-    if (fast) {
-      const int b0 = pgld_bin_scan(pair_dist(a, i1, j0), sedges, nb, 0, nb - 1);
-      const int b1 = pgld_bin_scan(pair_dist(a, i0, j1), sedges, nb, b0 < 0 ? 0 : b0, nb - 1);
-      fast = (b0 == b1) && b0 >= 0 && b0 < nb;
-      bfast = b0;
-    }
+    const int bmin = pgld_bin_scan(coord_diff(a, i1, j0), sedges, nb, 0, nb - 1);
+    const int bmax = pgld_bin_scan(coord_diff(a, i0, j1), sedges, nb, bmin < 0 ? 0 : bmin, nb - 1);
+    const bool fast = (cb > rb) && (j1 == j0 + 31) && (a.pos[j1] <= a.pos[i0] + a.max_dist) &&
+                      bmin == bmax && bmin >= 0 && bmin < nb;
     double acc[NS];
 #pragma unroll
     for (int q = 0; q < NS; ++q) acc[q] = 0.0;
     if (fast) {
-      // every pair of the tile is valid and falls in bin bfast
+      // every pair of the tile is valid and falls in bin bmin
 #pragma unroll 1
       for (int r = 0; r < 16; ++r) {
         double x[NS];
@@ -446,7 +488,17 @@ __global__ void __launch_bounds__(256, 2) fused_kernel(FusedParams a) {
 #pragma unroll
         for (int q = 0; q < NS; ++q) acc[q] += x[q];
       }
-      flush_wave_bin<NS>(slimb, lane, bfast, true, acc, 16);
+#pragma unroll
+      for (int q = 0; q < NS; ++q) acc[q] = pgld_wave_sum(acc[q]);
+      if (lane < NS) {
+        double v = acc[0];
+#pragma unroll
+        for (int q = 1; q < NS; ++q) v = lane == q ? acc[q] : v;
+        stg[bmin * NS + lane] += v;
+      } else if (lane == NS) {
+        scw[bmin] += 1024;
+      }
+      stage_drain<NS>(stg, scw, slimb, lane, bmin, bmin);
     } else {
       const int64_t j = j0 + col;
       const bool jok = j < a.V;
@@ -470,7 +522,7 @@ __global__ void __launch_bounds__(256, 2) fused_kernel(FusedParams a) {
           double x[NS];
           T::stats(a, srt, cts, r, x);
           if (b != cur) {
-            if (cur >= 0) flush_lane<NS>(slimb, cur, acc, count);
+            if (cur >= 0) stage_add<NS>(stg, scw, cur, acc, count);
             cur = b;
             count = 0;
 #pragma unroll
@@ -481,9 +533,11 @@ __global__ void __launch_bounds__(256, 2) fused_kernel(FusedParams a) {
           ++count;
         }
       }
-      flush_wave<NS>(slimb, lane, cur, acc, count);
+      if (cur >= 0) stage_add<NS>(stg, scw, cur, acc, count);
+      stage_drain<NS>(stg, scw, slimb, lane, bmin < 0 ? 0 : bmin, bmax >= nb ? nb - 1 : bmax);
     }
   }
+  stage_drain<NS>(stg, scw, slimb, lane, 0, nb - 1);  // nothing may stay staged
   __syncthreads();
   flush_global<NS>(slimb, a.out, nb);
 }
@@ -537,11 +591,13 @@ static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStre
   }
   a.rt_len = off;
   const size_t lds = (PGLD_MAX_BINS + 8) * sizeof(double) + (size_t)off * 4 * sizeof(double) +
-                     (size_t)g->n_bins * (NS + 1) * 3 * sizeof(unsigned long long);
+                     (size_t)4 * g->n_bins * NS * sizeof(double) +
+                     (size_t)g->n_bins * (NS + 1) * 3 * sizeof(unsigned long long) +
+                     (size_t)4 * g->n_bins * sizeof(int);
   if (lds > 160 * 1024) return fail(PGLD_EINVAL, "fused kernel: LDS tables exceed 160 KiB");
   PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(fused_kernel<T>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  int grid = g->grid > 0 ? g->grid : 2 * num_cus();
+  int grid = g->grid > 0 ? g->grid : T::MINB * num_cus();
   grid = ((grid + 7) / 8) * 8;
   const int64_t need = ((g->n_tiles + 31) / 32) * 8;  // at least 4 tiles per wave
   if (grid > need) grid = (int)need;
@@ -591,9 +647,15 @@ extern "C" int pgld_binned_fused(const pgld_panel_desc* desc, const pgld_binned_
       default: break;
     }
   } else {
+    bool narrow = true;
+    for (int p = 0; p < desc->n_pops; ++p) narrow = narrow && L.pop_size[p] <= 1023;
     switch (desc->n_pops) {
-      case 1: return launch_fused<GenoTraits<1>>(L, args, s);
-      case 2: return launch_fused<GenoTraits<2>>(L, args, s);
+      case 1:
+        return narrow ? launch_fused<GenoTraits<1, 3>>(L, args, s)
+                      : launch_fused<GenoTraits<1, 5>>(L, args, s);
+      case 2:
+        return narrow ? launch_fused<GenoTraits<2, 3>>(L, args, s)
+                      : launch_fused<GenoTraits<2, 5>>(L, args, s);
       default: break;
     }
   }

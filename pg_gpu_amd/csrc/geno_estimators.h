@@ -26,7 +26,7 @@ __device__ __forceinline__ GenoPop geno_pop(const int* c /* moments order */, in
                                             const double* __restrict__ rtab) {
   GenoPop p;
 #pragma unroll
-  for (int m = 0; m < 9; ++m) p.g[m] = c[m];
+  for (int m = 0; m < 9; ++m) p.g[m] = pgld_count_f64(c[m]);
   p.n = n;
   const double2* t = reinterpret_cast<const double2*>(rtab + 4 * n);
   const double2 a = t[0], b = t[1];

@@ -23,7 +23,8 @@ struct HapPop {
 __device__ __forceinline__ HapPop hap_pop(int n11, int n10, int n01, int n00, int n,
                                           const double* __restrict__ rtab) {
   HapPop p;
-  p.c1 = n11; p.c2 = n10; p.c3 = n01; p.c4 = n00;
+  p.c1 = pgld_count_f64(n11); p.c2 = pgld_count_f64(n10);
+  p.c3 = pgld_count_f64(n01); p.c4 = pgld_count_f64(n00);
   p.D = p.c2 * p.c3 - p.c1 * p.c4;
   p.pA = p.c1 + p.c2; p.qA = p.c3 + p.c4;
   p.pB = p.c1 + p.c3; p.qB = p.c2 + p.c4;

@@ -20,6 +20,15 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 // are added with plain integer adds; integer addition is associative, so the
 // sums do not depend on the order in which workgroups, waves or lanes add
 // their contributions.
+
+// Exact conversion of a count (< 2^31) to FP64 through the bit pattern of
+// 2^52 + v.  The result is opaque to the compiler's integer reasoning, so
+// FP64 products and sums of counts stay single FP64 instructions instead of
+// being rewritten as integer multiplies followed by conversions.
+__device__ __forceinline__ double pgld_count_f64(uint32_t v) {
+  return __hiloint2double(0x43300000, (int)v) - 4503599627370496.0;
+}
+
 #define PGLD_LIMBS 3
 
 __device__ __forceinline__ void pgld_to_limbs(double v, long long& x2, long long& x1,

@@ -112,7 +112,15 @@ class BinnedPlan:
         self.nb = nb
         self.d_pos = torch.as_tensor(pos_np, device=pan.device)
         self.coords = coords
-        self.d_coord = (torch.as_tensor(np.asarray(coords, np.float64), device=pan.device)
+        if coords is not None:
+            co_np = np.asarray(coords.cpu() if hasattr(coords, "cpu") else coords, np.float64)
+            if len(co_np) != pan.V:
+                raise ValueError("coords length differs from the number of sites")
+            # tiles are classified by their corner pairs, which bound every
+            # pair's distance only when the coordinate grows with position
+            if pan.V > 1 and np.any(np.diff(co_np) < 0):
+                raise ValueError("coords must be non-decreasing along the sorted positions")
+        self.d_coord = (torch.as_tensor(co_np, device=pan.device)
                         if coords is not None else self.d_pos)
         self.d_edges = torch.as_tensor(bins, device=pan.device)
         self.tiles = band_tiles(pos_np, self.max_dist) if tiles is None else np.asarray(tiles)

@@ -126,3 +126,23 @@ def test_geno_dense_counts_bit_exact(cuda, missing):
     c, _ = o.geno_counts(geno, ii.ravel(), jj.ravel(), pops[0])
     want = go.to_moments(c).T.reshape(9, V, V).astype(np.int32)
     np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("sizes,missing", [([1100], 0.02), ([1030, 40], 0.0)])
+def test_geno_large_population_16bit_counts(cuda, sizes, missing):
+    """Populations above 1023 individuals take the 16-bit count packing."""
+    rng = np.random.default_rng(19)
+    geno, pos, pops = make_geno(rng, sizes, 200, 20_000, missing)
+    run_geno(cuda, geno, pos, pops, synth.log_bins(100, 10_000, 5), 10_000.0)
+
+
+def test_coords_must_be_non_decreasing(cuda):
+    import torch
+
+    rng = np.random.default_rng(20)
+    geno, pos, pops = make_geno(rng, [20], 64, 5_000, 0.0)
+    pan = LDPanels(torch.as_tensor(geno, device=cuda), pops, "geno")
+    coords = np.linspace(0.0, 1e-3, len(pos))
+    coords[10] = coords[11] + 1e-4
+    with pytest.raises(ValueError):
+        pan.binned(pos, np.array([0.0, 1e-4, 1e-3]), 5_000.0, coords=coords)
