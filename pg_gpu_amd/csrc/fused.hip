@@ -488,17 +488,22 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
 #pragma unroll
         for (int q = 0; q < NS; ++q) acc[q] += x[q];
       }
+      // wave totals (lanes 0..31), then lane q < NS adds statistic q's limbs
 #pragma unroll
       for (int q = 0; q < NS; ++q) acc[q] = pgld_wave_sum(acc[q]);
-      if (lane < NS) {
-        double v = acc[0];
+      double v = acc[0];
 #pragma unroll
-        for (int q = 1; q < NS; ++q) v = lane == q ? acc[q] : v;
-        stg[bmin * NS + lane] += v;
+      for (int q = 1; q < NS; ++q) v = lane == q ? acc[q] : v;
+      unsigned long long* dst = slimb + ((int64_t)bmin * (NS + 1) + lane) * 3;
+      if (lane < NS) {
+        long long x2, x1, x0;
+        pgld_to_limbs(v, x2, x1, x0);
+        atomicAdd(dst + 0, (unsigned long long)x0);
+        atomicAdd(dst + 1, (unsigned long long)x1);
+        atomicAdd(dst + 2, (unsigned long long)x2);
       } else if (lane == NS) {
-        scw[bmin] += 1024;
+        atomicAdd(dst, 1024ull);
       }
-      stage_drain<NS>(stg, scw, slimb, lane, bmin, bmin);
     } else {
       const int64_t j = j0 + col;
       const bool jok = j < a.V;

@@ -54,11 +54,35 @@ __device__ __forceinline__ void pgld_normalize_limbs(long long& l2, long long& l
   l2 += c;
 }
 
-// Wave-wide (wave64) sum of a double; every lane receives the total.
+// DPP move of both halves of a double (row-local lane permutation `ctrl`).
+template <int CTRL>
+__device__ __forceinline__ double pgld_dpp_f64(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)u, CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Wave-wide (wave64) sum of a double, valid in lanes 0..31.  Within each
+// 16-lane row: DPP quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_ror 4 and
+// row_ror 8 leave the row total in every lane of the row; a ds_swizzle
+// exchange of the two rows of each 32-lane half and a readlane of lane 32
+// complete the sum.  The summation order is fixed, so the result is
+// deterministic.
 __device__ __forceinline__ double pgld_wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += pgld_dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
+  v += pgld_dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
+  v += pgld_dpp_f64<0x124>(v);  // row_ror 4
+  v += pgld_dpp_f64<0x128>(v);  // row_ror 8
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  // ds_swizzle bitmask mode: and 0x1f, or 0, xor 0x10 -> lane ^ 16 within 32
+  const int lo = __builtin_amdgcn_ds_swizzle((int)(uint32_t)u, 0x401F);
+  const int hi = __builtin_amdgcn_ds_swizzle((int)(uint32_t)(u >> 32), 0x401F);
+  v += __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+  const uint64_t w = __builtin_bit_cast(uint64_t, v);
+  const uint32_t hl = __builtin_amdgcn_readlane((int)(uint32_t)w, 32);
+  const uint32_t hh = __builtin_amdgcn_readlane((int)(uint32_t)(w >> 32), 32);
+  return v + __builtin_bit_cast(double, ((uint64_t)hh << 32) | hl);
 }
 
 __device__ __forceinline__ int pgld_wave_sum_int(int v) {
