@@ -274,14 +274,22 @@ __device__ __forceinline__ const double* rt_entry(const FusedParams& a, const do
 template <int P>
 struct HapTraits {
   static constexpr int NP = P;
-  static constexpr int MINB = 2;  // workgroups per CU: 2 waves per SIMD
+  // Workgroups per CU: two waves per SIMD while the live state fits the 256
+  // registers of a wave at that occupancy; three or four populations keep
+  // more counts and run one wave per SIMD with the full register file.
+  static constexpr int MINB = P <= 2 ? 2 : 1;
   static constexpr int NS = P == 1 ? HAP_NSTATS_P1 : P == 2 ? HAP_NSTATS_P2 : P == 3 ? HAP_NSTATS_P3
                                                                                      : HAP_NSTATS_P4;
+  static constexpr int NG = P == 1 ? HAP_NGROUPS_P1 : P == 2 ? HAP_NGROUPS_P2 : P == 3 ? HAP_NGROUPS_P3
+                                                                                       : HAP_NGROUPS_P4;
+  static constexpr int GS = NS < STATS_GROUP ? NS : STATS_GROUP;  // statistics per group
   struct Counts { v16u w0[P], w1[P]; };
   __device__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane, Counts& c) {
 #pragma unroll
     for (int p = 0; p < P; ++p) tile_counts(a, p, rb, cb, lane, c.w0[p], c.w1[p]);
   }
+  // statistics G*GS .. G*GS+GS-1 of the pair in accumulator slot r
+  template <int G>
   __device__ static void stats(const FusedParams& a, const double* srt, const Counts& c, int r,
                                double* x) {
     HapPop pp[P];
@@ -291,10 +299,7 @@ struct HapTraits {
       const int n01 = c.w1[p][r] & 0xffff, n00 = c.w1[p][r] >> 16;
       pp[p] = hap_pop(n11, n10, n01, n00, 0, rt_entry(a, srt, p, n11 + n10 + n01 + n00));
     }
-    if constexpr (P == 1) hap_stats_P1(pp, x);
-    if constexpr (P == 2) hap_stats_P2(pp, x);
-    if constexpr (P == 3) hap_stats_P3(pp, x);
-    if constexpr (P == 4) hap_stats_P4(pp, x);
+    hap_stats_group<P, G>(pp, x);
   }
 };
 
@@ -303,16 +308,20 @@ struct HapTraits {
 template <int P, int W>
 struct GenoTraits {
   static constexpr int NP = P;
-  // Two populations need more than the 256 registers a wave gets at two
-  // waves per SIMD; they run one wave per SIMD with the full register file.
+  // One population fits two waves per SIMD; more need the full register file
+  // of a one-wave-per-SIMD launch.
   static constexpr int MINB = P == 1 ? 2 : 1;
   static constexpr int NS = P == 1 ? GENO_NSTATS_P1 : P == 2 ? GENO_NSTATS_P2 : P == 3 ? GENO_NSTATS_P3
                                                                                         : GENO_NSTATS_P4;
+  static constexpr int NG = P == 1 ? GENO_NGROUPS_P1 : P == 2 ? GENO_NGROUPS_P2 : P == 3 ? GENO_NGROUPS_P3
+                                                                                          : GENO_NGROUPS_P4;
+  static constexpr int GS = NS < STATS_GROUP ? NS : STATS_GROUP;
   struct Counts { v16u w[P][W]; };
   __device__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane, Counts& c) {
 #pragma unroll
     for (int p = 0; p < P; ++p) geno_tile_counts<W>(a, p, rb, cb, lane, c.w[p]);
   }
+  template <int G>
   __device__ static void stats(const FusedParams& a, const double* srt, const Counts& c, int r,
                                double* x) {
     GenoPop pp[P];
@@ -342,46 +351,45 @@ struct GenoTraits {
       pp[p] = geno_pop(g, 0, rt_entry(a, srt, p, n));
       pp[p].n = n;
     }
-    if constexpr (P == 1) geno_stats_P1(pp, x);
-    if constexpr (P == 2) geno_stats_P2(pp, x);
-    if constexpr (P == 3) geno_stats_P3(pp, x);
-    if constexpr (P == 4) geno_stats_P4(pp, x);
+    geno_stats_group<P, G>(pp, x);
   }
 };
 
 // ---------------------------------------------------------------------------
-// Binning.  Each wave owns an FP64 staging array stage[bin][stat] and a pair
-// counter per bin in LDS.  Inside a tile, lanes add their running per-bin
-// sums to the staging array (LDS FP64 atomics, wave-private region); at the
-// tile end the wave converts the touched bins to 3-limb fixed point, adds
-// them to the workgroup's integer limbs and clears the staging entries.  The
-// FP64 rounding is thus confined to the 1024 pairs of one tile, and the
-// integer limb sums across tiles, waves and workgroups are exact, so the
-// result does not depend on how tiles are dealt to waves.
+// Binning.  Each wave owns an FP64 staging array stage[bin][slot] for the GS
+// statistics of the group being evaluated, and a pair counter per bin, in
+// LDS.  Inside a tile, lanes add their running per-bin sums to the staging
+// array (LDS FP64 atomics, wave-private region); at the end of the group's
+// pass the wave converts the touched bins to 3-limb fixed point, adds them to
+// the workgroup's integer limbs and clears the staging entries.  The FP64
+// rounding is thus confined to the 1024 pairs of one tile, and the integer
+// limb sums across tiles, waves and workgroups are exact, so the result does
+// not depend on how tiles are dealt to waves.
 // ---------------------------------------------------------------------------
-template <int NS>
+template <int GS, bool COUNT>
 __device__ __forceinline__ void stage_add(double* stg, int* scnt, int bin, const double* acc,
                                           int count) {
-  double* d = stg + bin * NS;
+  double* d = stg + bin * GS;
 #pragma unroll
-  for (int q = 0; q < NS; ++q) atomicAdd(d + q, acc[q]);
-  atomicAdd(scnt + bin, count);
+  for (int q = 0; q < GS; ++q) atomicAdd(d + q, acc[q]);
+  if constexpr (COUNT) atomicAdd(scnt + bin, count);
 }
 
-// Move staged bins [blo, bhi] of this wave into the workgroup limbs.
-template <int NS>
+// Move staged bins [blo, bhi] of this wave into the workgroup limbs of
+// statistics s0 .. s0+GS-1 (and the pair counters).
+template <int NS, int GS>
 __device__ __forceinline__ void stage_drain(double* stg, int* scnt, unsigned long long* slimb,
-                                            int lane, int blo, int bhi) {
+                                            int lane, int blo, int bhi, int s0) {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  const int E = (bhi - blo + 1) * (NS + 1);
+  const int E = (bhi - blo + 1) * (GS + 1);
   for (int e = lane; e < E; e += 64) {
-    const int b = blo + e / (NS + 1), q = e - (b - blo) * (NS + 1);
-    unsigned long long* dst = slimb + ((int64_t)b * (NS + 1) + q) * 3;
-    if (q < NS) {
-      const double v = stg[b * NS + q];
+    const int b = blo + e / (GS + 1), q = e - (b - blo) * (GS + 1);
+    if (q < GS) {
+      const double v = stg[b * GS + q];
       if (v != 0.0) {
-        stg[b * NS + q] = 0.0;
+        stg[b * GS + q] = 0.0;
+        unsigned long long* dst = slimb + ((int64_t)b * (NS + 1) + s0 + q) * 3;
         long long x2, x1, x0;
         pgld_to_limbs(v, x2, x1, x0);
         atomicAdd(dst + 0, (unsigned long long)x0);
@@ -392,7 +400,7 @@ __device__ __forceinline__ void stage_drain(double* stg, int* scnt, unsigned lon
       const int c = scnt[b];
       if (c) {
         scnt[b] = 0;
-        atomicAdd(dst, (unsigned long long)c);
+        atomicAdd(slimb + ((int64_t)b * (NS + 1) + NS) * 3, (unsigned long long)c);
       }
     }
   }
@@ -425,31 +433,130 @@ __device__ __forceinline__ double coord_diff(const FusedParams& a, int64_t i, in
   return a.coord_is_pos ? (a.pos[j] - a.pos[i]) : (a.coord[j] - a.coord[i]);
 }
 
+struct TileCtx {
+  const double* sedges;
+  const double* srt;
+  double* stg;   // this wave's staging [nb][GS]
+  int* scw;      // this wave's pair counters [nb]
+  unsigned long long* slimb;
+  int lane, bmin, bmax;
+  bool fast;
+  int64_t i0, j0;
+};
+
+// One pass over the tile's 1024 pairs for statistic group G.
+template <class T, int G>
+__device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& t,
+                                           const typename T::Counts& cts) {
+  constexpr int NS = T::NS, GS = T::GS, S0 = G * GS;
+  const int nb = a.n_bins, lane = t.lane;
+  double acc[GS];
+#pragma unroll
+  for (int q = 0; q < GS; ++q) acc[q] = 0.0;
+  if (t.fast) {
+    // every pair of the tile is valid and falls in bin bmin
+#pragma unroll 1
+    for (int r = 0; r < 16; ++r) {
+      double x[GS];
+      T::template stats<G>(a, t.srt, cts, r, x);
+#pragma unroll
+      for (int q = 0; q < GS; ++q) acc[q] += x[q];
+    }
+    // wave totals (lanes 0..31), then lane q < GS adds statistic S0+q's limbs
+#pragma unroll
+    for (int q = 0; q < GS; ++q) acc[q] = pgld_wave_sum(acc[q]);
+    double v = acc[0];
+#pragma unroll
+    for (int q = 1; q < GS; ++q) v = lane == q ? acc[q] : v;
+    if (lane < GS) {
+      unsigned long long* dst = t.slimb + ((int64_t)t.bmin * (NS + 1) + S0 + lane) * 3;
+      long long x2, x1, x0;
+      pgld_to_limbs(v, x2, x1, x0);
+      atomicAdd(dst + 0, (unsigned long long)x0);
+      atomicAdd(dst + 1, (unsigned long long)x1);
+      atomicAdd(dst + 2, (unsigned long long)x2);
+    } else if (G == 0 && lane == GS) {
+      atomicAdd(t.slimb + ((int64_t)t.bmin * (NS + 1) + NS) * 3, 1024ull);
+    }
+  } else {
+    const int64_t j = t.j0 + (lane & 31);
+    const bool jok = j < a.V;
+    const double pos_j = jok ? a.pos[j] : 0.0;
+    const double co_j = jok ? a.coord[j] : 0.0;
+    int cur = -1, count = 0, guess = nb - 1;
+#pragma unroll 1
+    for (int r = 0; r < 16; ++r) {
+      const int64_t i = t.i0 + mfma_row(lane, r);
+      bool ok = jok && (i < j);
+      int b = -1;
+      if (ok) {
+        const double pi = a.pos[i];
+        ok = pos_j <= pi + a.max_dist;
+        const double d = a.coord_is_pos ? (pos_j - pi) : fabs(co_j - a.coord[i]);
+        b = pgld_bin_scan(d, t.sedges, nb, guess, nb - 1);
+        ok = ok && b >= 0 && b < nb;
+        guess = b < 0 ? 0 : (b >= nb ? nb - 1 : b);
+      }
+      if (ok) {
+        double x[GS];
+        T::template stats<G>(a, t.srt, cts, r, x);
+        if (b != cur) {
+          if (cur >= 0) stage_add<GS, G == 0>(t.stg, t.scw, cur, acc, count);
+          cur = b;
+          count = 0;
+#pragma unroll
+          for (int q = 0; q < GS; ++q) acc[q] = 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < GS; ++q) acc[q] += x[q];
+        ++count;
+      }
+    }
+    if (cur >= 0) stage_add<GS, G == 0>(t.stg, t.scw, cur, acc, count);
+    stage_drain<NS, GS>(t.stg, t.scw, t.slimb, lane, t.bmin < 0 ? 0 : t.bmin,
+                        t.bmax >= nb ? nb - 1 : t.bmax, S0);
+  }
+}
+
+template <class T, int G>
+__device__ __forceinline__ void tile_groups(const FusedParams& a, const TileCtx& t,
+                                            const typename T::Counts& cts) {
+  if constexpr (G < T::NG) {
+    tile_group<T, G>(a, t, cts);
+    tile_groups<T, G + 1>(a, t, cts);
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
-  constexpr int NS = T::NS;
+  constexpr int NS = T::NS, GS = T::GS;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   double* sedges = reinterpret_cast<double*>(smem);
   double* srt = sedges + PGLD_MAX_BINS + 8;
   const int nb = a.n_bins;
-  double* sstage = srt + 4 * a.rt_len;                       // [4 waves][nb][NS]
+  double* sstage = srt + 4 * a.rt_len;                       // [4 waves][nb][GS]
   unsigned long long* slimb =
-      reinterpret_cast<unsigned long long*>(sstage + 4 * nb * NS);  // [nb][NS+1][3]
+      reinterpret_cast<unsigned long long*>(sstage + 4 * nb * GS);  // [nb][NS+1][3]
   int* scnt = reinterpret_cast<int*>(slimb + nb * (NS + 1) * 3);    // [4 waves][nb]
-  const int t = threadIdx.x, lane = t & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-  for (int e = t; e <= nb; e += blockDim.x) sedges[e] = a.edges[e];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int e = tid; e <= nb; e += blockDim.x) sedges[e] = a.edges[e];
   for (int p = 0; p < T::NP; ++p) {
     const int len = a.K[p] - a.rt_lo[p] + 1;
-    for (int e = t; e < 4 * len; e += blockDim.x)
+    for (int e = tid; e < 4 * len; e += blockDim.x)
       srt[4 * a.rt_off[p] + e] = a.rtab[4 * a.rt_lo[p] + e];
   }
-  for (int e = t; e < 4 * nb * NS; e += blockDim.x) sstage[e] = 0.0;
-  for (int e = t; e < nb * (NS + 1) * 3; e += blockDim.x) slimb[e] = 0ull;
-  for (int e = t; e < 4 * nb; e += blockDim.x) scnt[e] = 0;
+  for (int e = tid; e < 4 * nb * GS; e += blockDim.x) sstage[e] = 0.0;
+  for (int e = tid; e < nb * (NS + 1) * 3; e += blockDim.x) slimb[e] = 0ull;
+  for (int e = tid; e < 4 * nb; e += blockDim.x) scnt[e] = 0;
   __syncthreads();
-  double* stg = sstage + wave * nb * NS;
-  int* scw = scnt + wave * nb;
+  TileCtx t;
+  t.sedges = sedges;
+  t.srt = srt;
+  t.stg = sstage + wave * nb * GS;
+  t.scw = scnt + wave * nb;
+  t.slimb = slimb;
+  t.lane = lane;
 
   // XCD-aware split: workgroups are dealt round-robin to the 8 XCDs, so each
   // XCD gets a contiguous stretch of the tile list (ordered in L2-sized
@@ -459,7 +566,6 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
   const int xcd = blockIdx.x % nx;
   const int waves_per_xcd = (gridDim.x / nx) * 4;
   const int64_t seg_lo = a.n_tiles * xcd / nx, seg_hi = a.n_tiles * (xcd + 1) / nx;
-  const int col = lane & 31;
 
   for (int64_t tile = seg_lo + (blockIdx.x / nx) * 4 + wave; tile < seg_hi;
        tile += waves_per_xcd) {
@@ -472,77 +578,16 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
     // (i0, j1) bound the bins of every pair in the tile.
     const int64_t i0 = rb * 32, j0 = cb * 32;
     const int64_t i1 = min(i0 + 31, a.V - 1), j1 = min(j0 + 31, a.V - 1);
-    const int bmin = pgld_bin_scan(coord_diff(a, i1, j0), sedges, nb, 0, nb - 1);
-    const int bmax = pgld_bin_scan(coord_diff(a, i0, j1), sedges, nb, bmin < 0 ? 0 : bmin, nb - 1);
-    const bool fast = (cb > rb) && (j1 == j0 + 31) && (a.pos[j1] <= a.pos[i0] + a.max_dist) &&
-                      bmin == bmax && bmin >= 0 && bmin < nb;
-    double acc[NS];
-#pragma unroll
-    for (int q = 0; q < NS; ++q) acc[q] = 0.0;
-    if (fast) {
-      // every pair of the tile is valid and falls in bin bmin
-#pragma unroll 1
-      for (int r = 0; r < 16; ++r) {
-        double x[NS];
-        T::stats(a, srt, cts, r, x);
-#pragma unroll
-        for (int q = 0; q < NS; ++q) acc[q] += x[q];
-      }
-      // wave totals (lanes 0..31), then lane q < NS adds statistic q's limbs
-#pragma unroll
-      for (int q = 0; q < NS; ++q) acc[q] = pgld_wave_sum(acc[q]);
-      double v = acc[0];
-#pragma unroll
-      for (int q = 1; q < NS; ++q) v = lane == q ? acc[q] : v;
-      unsigned long long* dst = slimb + ((int64_t)bmin * (NS + 1) + lane) * 3;
-      if (lane < NS) {
-        long long x2, x1, x0;
-        pgld_to_limbs(v, x2, x1, x0);
-        atomicAdd(dst + 0, (unsigned long long)x0);
-        atomicAdd(dst + 1, (unsigned long long)x1);
-        atomicAdd(dst + 2, (unsigned long long)x2);
-      } else if (lane == NS) {
-        atomicAdd(dst, 1024ull);
-      }
-    } else {
-      const int64_t j = j0 + col;
-      const bool jok = j < a.V;
-      const double pos_j = jok ? a.pos[j] : 0.0;
-      const double co_j = jok ? a.coord[j] : 0.0;
-      int cur = -1, count = 0, guess = nb - 1;
-#pragma unroll 1
-      for (int r = 0; r < 16; ++r) {
-        const int64_t i = i0 + mfma_row(lane, r);
-        bool ok = jok && (i < j);
-        int b = -1;
-        if (ok) {
-          const double pi = a.pos[i];
-          ok = pos_j <= pi + a.max_dist;
-          const double d = a.coord_is_pos ? (pos_j - pi) : fabs(co_j - a.coord[i]);
-          b = pgld_bin_scan(d, sedges, nb, guess, nb - 1);
-          ok = ok && b >= 0 && b < nb;
-          guess = b < 0 ? 0 : (b >= nb ? nb - 1 : b);
-        }
-        if (ok) {
-          double x[NS];
-          T::stats(a, srt, cts, r, x);
-          if (b != cur) {
-            if (cur >= 0) stage_add<NS>(stg, scw, cur, acc, count);
-            cur = b;
-            count = 0;
-#pragma unroll
-            for (int q = 0; q < NS; ++q) acc[q] = 0.0;
-          }
-#pragma unroll
-          for (int q = 0; q < NS; ++q) acc[q] += x[q];
-          ++count;
-        }
-      }
-      if (cur >= 0) stage_add<NS>(stg, scw, cur, acc, count);
-      stage_drain<NS>(stg, scw, slimb, lane, bmin < 0 ? 0 : bmin, bmax >= nb ? nb - 1 : bmax);
-    }
+    t.i0 = i0;
+    t.j0 = j0;
+    t.bmin = pgld_bin_scan(coord_diff(a, i1, j0), sedges, nb, 0, nb - 1);
+    t.bmax = pgld_bin_scan(coord_diff(a, i0, j1), sedges, nb, t.bmin < 0 ? 0 : t.bmin, nb - 1);
+    t.fast = (cb > rb) && (j1 == j0 + 31) && (a.pos[j1] <= a.pos[i0] + a.max_dist) &&
+             t.bmin == t.bmax && t.bmin >= 0 && t.bmin < nb;
+    tile_groups<T, 0>(a, t, cts);
   }
-  stage_drain<NS>(stg, scw, slimb, lane, 0, nb - 1);  // nothing may stay staged
+  // nothing may stay staged
+  stage_drain<NS, GS>(t.stg, t.scw, slimb, lane, 0, nb - 1, 0);
   __syncthreads();
   flush_global<NS>(slimb, a.out, nb);
 }
@@ -596,7 +641,7 @@ static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStre
   }
   a.rt_len = off;
   const size_t lds = (PGLD_MAX_BINS + 8) * sizeof(double) + (size_t)off * 4 * sizeof(double) +
-                     (size_t)4 * g->n_bins * NS * sizeof(double) +
+                     (size_t)4 * g->n_bins * T::GS * sizeof(double) +
                      (size_t)g->n_bins * (NS + 1) * 3 * sizeof(unsigned long long) +
                      (size_t)4 * g->n_bins * sizeof(int);
   if (lds > 160 * 1024) return fail(PGLD_EINVAL, "fused kernel: LDS tables exceed 160 KiB");
@@ -649,6 +694,8 @@ extern "C" int pgld_binned_fused(const pgld_panel_desc* desc, const pgld_binned_
     switch (desc->n_pops) {
       case 1: return launch_fused<HapTraits<1>>(L, args, s);
       case 2: return launch_fused<HapTraits<2>>(L, args, s);
+      case 3: return launch_fused<HapTraits<3>>(L, args, s);
+      case 4: return launch_fused<HapTraits<4>>(L, args, s);
       default: break;
     }
   } else {
@@ -661,6 +708,12 @@ extern "C" int pgld_binned_fused(const pgld_panel_desc* desc, const pgld_binned_
       case 2:
         return narrow ? launch_fused<GenoTraits<2, 3>>(L, args, s)
                       : launch_fused<GenoTraits<2, 5>>(L, args, s);
+      case 3:
+        return narrow ? launch_fused<GenoTraits<3, 3>>(L, args, s)
+                      : launch_fused<GenoTraits<3, 5>>(L, args, s);
+      case 4:
+        return narrow ? launch_fused<GenoTraits<4, 3>>(L, args, s)
+                      : launch_fused<GenoTraits<4, 5>>(L, args, s);
       default: break;
     }
   }

@@ -121,20 +121,42 @@ def stat_set_plan(P, keyfn):
     return values, stats
 
 
+GROUP = 15  # statistics evaluated per pass of the fused epilogue
+
+
 def emit_set(prefix, popty, P, keyfn):
+    """One device function per group of GROUP consecutive statistics (ld_names
+    order); consecutive groups share no raw-call value, so splitting costs no
+    re-evaluation.  Group k writes out[0 .. size-1] for statistics
+    k*GROUP .. k*GROUP+size-1."""
     values, stats = stat_set_plan(P, keyfn)
-    lines = [f"// P = {P}: {len(stats)} statistics from {len(values)} distinct raw-call values",
+    ng = (len(stats) + GROUP - 1) // GROUP
+    lines = [f"// P = {P}: {len(stats)} statistics from {len(values)} distinct raw-call values, "
+             f"{ng} group(s) of up to {GROUP}",
              f"#define {prefix.upper()}_NSTATS_P{P} {len(stats)}",
-             f"__device__ __forceinline__ void {prefix}_stats_P{P}(const {popty}* p, double* out) {{"]
-    for v, (fn, args) in enumerate(values):
-        a = ", ".join(f"p[{x}]" for x in args)
-        lines.append(f"  const double v{v} = {fn}({a});")
-    for s, terms in enumerate(stats):
-        if len(terms) == 1 and terms[0][1] == 1.0:
-            expr = f"v{terms[0][0]}"
-        else:
-            expr = " + ".join(f"{w!r} * v{v}" for v, w in terms)
-        lines.append(f"  out[{s}] = {expr};")
+             f"#define {prefix.upper()}_NGROUPS_P{P} {ng}"]
+    for k in range(ng):
+        grp = stats[k * GROUP:(k + 1) * GROUP]
+        used = sorted(set(v for terms in grp for v, _ in terms))
+        lines.append(f"__device__ __forceinline__ void {prefix}_stats_P{P}_g{k}(const {popty}* p, "
+                     "double* out) {")
+        for v in used:
+            fn, args = values[v]
+            a = ", ".join(f"p[{x}]" for x in args)
+            lines.append(f"  const double v{v} = {fn}({a});")
+        for s, terms in enumerate(grp):
+            if len(terms) == 1 and terms[0][1] == 1.0:
+                expr = f"v{terms[0][0]}"
+            else:
+                expr = " + ".join(f"{w!r} * v{v}" for v, w in terms)
+            lines.append(f"  out[{s}] = {expr};")
+        lines.append("}")
+        lines.append(f"template <> __device__ __forceinline__ void {prefix}_stats_group<{P}, {k}>("
+                     f"const {popty}* p, double* out) {{ {prefix}_stats_P{P}_g{k}(p, out); }}")
+    # whole set (used by the per-pair and plain kernels)
+    lines.append(f"__device__ __forceinline__ void {prefix}_stats_P{P}(const {popty}* p, double* out) {{")
+    for k in range(ng):
+        lines.append(f"  {prefix}_stats_P{P}_g{k}(p, out + {k * GROUP});")
     lines.append("}")
     return "\n".join(lines)
 
@@ -237,7 +259,10 @@ def emit_geno_polys():
 def main():
     parts = ["// Generated by gen_stats.py -- do not edit by hand.",
              "#pragma once",
-             '#include "hap_estimators.h"', ""]
+             '#include "hap_estimators.h"', "",
+             f"#define STATS_GROUP {GROUP}",
+             "template <int P, int G> __device__ void hap_stats_group(const HapPop* p, double* out);",
+             ""]
     for P in range(1, 5):
         parts.append(emit_set("hap", "HapPop", P, hap_key))
         parts.append("")
@@ -245,7 +270,9 @@ def main():
     emit_geno_polys()
     parts = ["// Generated by gen_stats.py -- do not edit by hand.",
              "#pragma once",
-             '#include "geno_estimators.h"', ""]
+             '#include "geno_estimators.h"', "",
+             "template <int P, int G> __device__ void geno_stats_group(const GenoPop* p, double* out);",
+             ""]
     for P in range(1, 5):
         parts.append(emit_set("geno", "GenoPop", P, geno_key))
         parts.append("")

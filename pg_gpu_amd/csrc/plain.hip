@@ -34,7 +34,8 @@ struct PlainParams {
 // Per-pair counting straight from the int8 matrix, and the statistic set.
 template <int MODE, int P> struct PlainSet;
 template <int P> struct PlainSet<PGLD_MODE_HAP, P> {
-  static constexpr int NS = P == 1 ? HAP_NSTATS_P1 : HAP_NSTATS_P2;
+  static constexpr int NS = P == 1 ? HAP_NSTATS_P1 : P == 2 ? HAP_NSTATS_P2 : P == 3 ? HAP_NSTATS_P3
+                                                                                     : HAP_NSTATS_P4;
   using Pop = HapPop;
   __device__ static Pop pop(const PlainParams& a, const int32_t* srows, int p, int64_t i,
                             int64_t j) {
@@ -50,11 +51,15 @@ template <int P> struct PlainSet<PGLD_MODE_HAP, P> {
     return hap_pop(n11, n10, n01, n00, n11 + n10 + n01 + n00, a.rtab);
   }
   __device__ static void eval(const Pop* p, double* o) {
-    if constexpr (P == 1) hap_stats_P1(p, o); else hap_stats_P2(p, o);
+    if constexpr (P == 1) hap_stats_P1(p, o);
+    if constexpr (P == 2) hap_stats_P2(p, o);
+    if constexpr (P == 3) hap_stats_P3(p, o);
+    if constexpr (P == 4) hap_stats_P4(p, o);
   }
 };
 template <int P> struct PlainSet<PGLD_MODE_GENO, P> {
-  static constexpr int NS = P == 1 ? GENO_NSTATS_P1 : GENO_NSTATS_P2;
+  static constexpr int NS = P == 1 ? GENO_NSTATS_P1 : P == 2 ? GENO_NSTATS_P2 : P == 3 ? GENO_NSTATS_P3
+                                                                                        : GENO_NSTATS_P4;
   using Pop = GenoPop;
   __device__ static Pop pop(const PlainParams& a, const int32_t* srows, int p, int64_t i,
                             int64_t j) {
@@ -70,7 +75,10 @@ template <int P> struct PlainSet<PGLD_MODE_GENO, P> {
     return geno_pop(c, n, a.rtab);
   }
   __device__ static void eval(const Pop* p, double* o) {
-    if constexpr (P == 1) geno_stats_P1(p, o); else geno_stats_P2(p, o);
+    if constexpr (P == 1) geno_stats_P1(p, o);
+    if constexpr (P == 2) geno_stats_P2(p, o);
+    if constexpr (P == 3) geno_stats_P3(p, o);
+    if constexpr (P == 4) geno_stats_P4(p, o);
   }
 };
 
@@ -287,12 +295,16 @@ extern "C" int pgld_binned_plain(const pgld_panel_desc* desc, const pgld_binned_
     switch (desc->n_pops) {
       case 1: return launch_plain<PGLD_MODE_HAP, 1>(L, args, s);
       case 2: return launch_plain<PGLD_MODE_HAP, 2>(L, args, s);
+      case 3: return launch_plain<PGLD_MODE_HAP, 3>(L, args, s);
+      case 4: return launch_plain<PGLD_MODE_HAP, 4>(L, args, s);
       default: break;
     }
   } else {
     switch (desc->n_pops) {
       case 1: return launch_plain<PGLD_MODE_GENO, 1>(L, args, s);
       case 2: return launch_plain<PGLD_MODE_GENO, 2>(L, args, s);
+      case 3: return launch_plain<PGLD_MODE_GENO, 3>(L, args, s);
+      case 4: return launch_plain<PGLD_MODE_GENO, 4>(L, args, s);
       default: break;
     }
   }

@@ -129,3 +129,21 @@ def test_dense_counts_bit_exact(cuda, missing):
     ii, jj = np.meshgrid(np.arange(V), np.arange(V), indexing="ij")
     c, _ = o.hap_counts(hap, ii.ravel(), jj.ravel())
     np.testing.assert_array_equal(got.reshape(4, -1).T, c)
+
+
+def test_three_pops_missing(cuda):
+    rng = np.random.default_rng(31)
+    hap, pos, pops = small_pops(rng, 0, 300, 30_000, 0.03, sizes=[20, 45, 70])
+    run_case(cuda, hap, pos, pops, synth.log_bins(100, 12_000, 6), 12_000.0)
+
+
+def test_four_pops_workload_c4_scaled(cuda):
+    """40/120/300/800 haplotypes, 3% missing, rare-allele sites: the 105-statistic set."""
+    w = synth.make("c4", scale=0.0008)
+    run_case(cuda, w.mat, w.positions, w.pop_rows, w.bins, w.max_dist)
+
+
+def test_four_pops_no_missing_ragged(cuda):
+    rng = np.random.default_rng(32)
+    hap, pos, pops = small_pops(rng, 0, 150, 20_000, 0.0, sizes=[3, 33, 64, 97])
+    run_case(cuda, hap, pos, pops, np.array([0.0, 300.0, 2_000.0, 8_000.0]), 8_000.0)

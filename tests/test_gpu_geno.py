@@ -146,3 +146,10 @@ def test_coords_must_be_non_decreasing(cuda):
     coords[10] = coords[11] + 1e-4
     with pytest.raises(ValueError):
         pan.binned(pos, np.array([0.0, 1e-4, 1e-3]), 5_000.0, coords=coords)
+
+
+@pytest.mark.parametrize("sizes,missing", [([20, 30, 25], 0.03), ([9, 16, 21, 40], 0.0)])
+def test_geno_three_and_four_pops(cuda, sizes, missing):
+    rng = np.random.default_rng(21)
+    geno, pos, pops = make_geno(rng, sizes, 200, 20_000, missing)
+    run_geno(cuda, geno, pos, pops, synth.log_bins(100, 8_000, 4), 8_000.0)
