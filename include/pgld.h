@@ -104,6 +104,9 @@ typedef struct {
   int32_t grid;                /* workgroups for the persistent fused kernel (0 = auto) */
   int32_t n_lo[PGLD_MAX_POPS]; /* per pop: a lower bound on any pair's valid-sample count
                                   (e.g. size - 2 * max missing per site); 0 if unknown */
+  int32_t with_r2;             /* 1 (one population only): two more sums per bin, the
+                                  sum of r^2 over pairs where it is defined and the
+                                  number of such pairs (mean r^2 = ratio) */
 } pgld_binned_args;
 
 /* Number of statistics produced per bin for (mode, n_pops). */
@@ -118,8 +121,45 @@ int pgld_n_stats(int32_t mode, int32_t n_pops);
  * loop in HaplotypeMatrix.compute_ld_statistics_gpu_{single_pop,two_pops}
  * (haplotype_matrix.py:1592-1866).  Output: d_out[b][s][0..2] fixed-point
  * limbs of the bin sum of statistic s (value = (L2*2^64 + L1*2^32 + L0)/2^72),
- * d_out[b][n_stats][0] = number of pairs in bin b. */
+ * d_out[b][n_out][0] = number of pairs in bin b, where n_out = n_stats, or
+ * n_stats + 2 with with_r2 (r^2 sum and defined-pair count at n_stats and
+ * n_stats + 1; haplotype r^2 as ld_statistics.r_squared,
+ * pg_gpu/ld_statistics.py:190-217, genotype r^2 the pairwise-complete dosage
+ * Pearson r^2, equal to Rogers-Huff r^2 without missing calls, :492-521). */
 int pgld_binned_fused(const pgld_panel_desc* desc, const pgld_binned_args* args, void* stream);
+
+/* Arguments of the ragged-window kernels (one population). */
+typedef struct {
+  const int8_t* d_panels;      /* from pgld_pack (fused kernel) */
+  const int32_t* d_site_stats; /* from pgld_scan_sites */
+  const int8_t* d_mat;         /* raw matrix (plain kernel only) */
+  int64_t ld;                  /* raw matrix row stride (plain kernel only) */
+  const int32_t* d_rows;       /* raw matrix rows of the population (plain kernel) */
+  const double* d_rtab;        /* 4*(pop size + 1) reciprocal falling factorials */
+  const int64_t* d_windows;    /* [n_windows][2] site ranges [lo, hi), lo <= hi */
+  int64_t n_windows;
+  const int32_t* d_work;       /* fused: [n_work][3] (window, row block, col block) with
+                                  32-site blocks covering each window's upper triangle */
+  int64_t n_work;
+  long long* d_out;            /* [n_windows][n_stats + 3][3] limbs, zeroed */
+  int32_t grid;                /* workgroups (0 = auto) */
+  int32_t n_lo[PGLD_MAX_POPS]; /* as in pgld_binned_args */
+} pgld_window_args;
+
+/* Per-window sums over every i < j pair of sites inside each window (no
+ * distance cutoff): the single-population DD, Dz, pi2 (as pgld_binned_fused),
+ * then the r^2 sum and the number of pairs where r^2 is defined, then the
+ * pair count at column n_stats + 2.  Replaces, per window, the reference's
+ * pair loop over a window's sites (ld_pipeline.py:57-175 with
+ * max_dist >= window span) followed by ld_statistics.dd / r_squared
+ * (ld_statistics.py:13-65, :190-217); all windows in one persistent launch
+ * over the (window, tile) work list. */
+int pgld_windows_fused(const pgld_panel_desc* desc, const pgld_window_args* args, void* stream);
+
+/* Plain comparator of pgld_windows_fused: one workgroup per anchor site, one
+ * thread per partner in the anchor's window, per-pair loops over samples of
+ * the raw matrix.  d_work is not used.  Same output. */
+int pgld_windows_plain(const pgld_panel_desc* desc, const pgld_window_args* args, void* stream);
 
 /* The plain comparator: one thread per pair, a loop over samples of the raw
  * int8 matrix, FP64 estimators, per-workgroup bin partials.  Same output. */

@@ -408,3 +408,28 @@ def expected_table(table, Qs, ns):
                 break
         tot += val
     return tot
+
+
+# ---------------------------------------------------------------------------
+# r^2 of dosages (pairwise-complete Pearson; Rogers-Huff without missing calls,
+# ld_statistics.py:492-521)
+# ---------------------------------------------------------------------------
+
+
+def r2_dosage(counts):
+    """(N, 9) counts (n00..n22, first digit locus i) -> r^2 [N], NaN where a
+    locus has no dosage variance among the individuals valid at both."""
+    c = np.asarray(counts, np.float64).reshape(-1, 3, 3)   # [N, x, y]
+    dose = np.array([0.0, 1.0, 2.0])
+    n = c.sum(axis=(1, 2))
+    cx = c.sum(axis=2)                                     # [N, x]
+    cy = c.sum(axis=1)
+    sx, sxx = cx @ dose, cx @ (dose * dose)
+    sy, syy = cy @ dose, cy @ (dose * dose)
+    sxy = np.einsum("nxy,x,y->n", c, dose, dose)
+    num = n * sxy - sx * sy
+    vx, vy = n * sxx - sx * sx, n * syy - sy * sy
+    ok = (vx > 0) & (vy > 0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.where(ok, num * num / np.where(ok, vx * vy, 1.0), np.nan)
+

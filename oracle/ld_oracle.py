@@ -566,8 +566,13 @@ def binned_sums_geno(geno, positions, pop_rows, bins, **kw):
 
 
 def binned_sums(mat, positions, pop_rows, bins, max_dist=None, coords=None,
-                chunk=200_000, exact=True, with_abs=False, anchors=None, mode="hap"):
+                chunk=200_000, exact=True, with_abs=False, anchors=None, mode="hap", r2=False):
     """Binned LD sums (moments_ld.py:248-343).
+
+    With ``r2`` two columns follow the LD statistics: the per-bin sum of r^2
+    of the first population over pairs where it is defined (haplotypes:
+    ld_statistics.r_squared, ld_statistics.py:190-217; genotypes: the
+    pairwise-complete dosage r^2) and the number of such pairs.
 
     Returns (sums[n_bins, n_stats], pair_counts[n_bins], n_enumerated) and,
     with ``with_abs``, also the per-bin sums of |value| (the scale against which
@@ -583,7 +588,7 @@ def binned_sums(mat, positions, pop_rows, bins, max_dist=None, coords=None,
     pos = np.asarray(positions)
     co = pos.astype(np.float64) if coords is None else np.asarray(coords, np.float64)
     P = len(pop_rows)
-    n_stats = len(ld_names(P))
+    n_stats = len(ld_names(P)) + (2 if r2 else 0)
     parts = [[[] for _ in range(n_stats)] for _ in range(nb)]
     sums = np.zeros((nb, n_stats))
     abs_sums = np.zeros((nb, n_stats))
@@ -611,6 +616,16 @@ def binned_sums(mat, positions, pop_rows, bins, max_dist=None, coords=None,
             from .geno_oracle import multi_pop_stats_geno
 
             st = multi_pop_stats_geno(cl, nl)
+        if r2:
+            if mode == "hap":
+                rr = r_family(cl[0], nl[0])[1]
+            else:
+                from .geno_oracle import r2_dosage
+
+                rr = r2_dosage(cl[0])
+            fin = np.isfinite(rr)
+            st = np.concatenate([st, np.where(fin, rr, 0.0)[:, None],
+                                 fin.astype(np.float64)[:, None]], axis=1)
         ok = (b >= 0) & (b < nb)
         counts += np.bincount(b[ok], minlength=nb)[:nb]
         np.add.at(abs_sums, b[ok], np.abs(st[ok]))
@@ -633,6 +648,39 @@ def binned_sums(mat, positions, pop_rows, bins, max_dist=None, coords=None,
     if with_abs:
         return sums, counts, n_enum, abs_sums
     return sums, counts, n_enum
+
+
+def window_sums(mat, windows, rows=None, mode="hap", exact=True):
+    """Per-window sums over every i < j pair inside each site range [lo, hi):
+    columns DD, Dz, pi2 (single population), r^2 sum where defined, number of
+    pairs with r^2 defined; plus the pair count per window.  The reference
+    form: pairs of a window (ld_pipeline.py:57-122 with the cutoff spanning
+    the window), counts (:130-263), ld_statistics dd/dz/pi2/r_squared."""
+    windows = np.asarray(windows, np.int64).reshape(-1, 2)
+    sums = np.zeros((len(windows), 5))
+    counts = np.zeros(len(windows), np.int64)
+    for w, (lo, hi) in enumerate(windows):
+        k = hi - lo
+        if k < 2:
+            continue
+        ii, jj = np.triu_indices(k, 1)
+        ii, jj = ii + lo, jj + lo
+        if mode == "hap":
+            c, nv = hap_counts(mat, ii, jj, rows)
+            st = hap_stats_for_pairs([c], [nv])
+            rr = r_family(c, nv)[1]
+        else:
+            from .geno_oracle import multi_pop_stats_geno, r2_dosage
+
+            c, nv = geno_counts(mat, ii, jj, rows)
+            st = multi_pop_stats_geno([c], [nv])
+            rr = r2_dosage(c)
+        fin = np.isfinite(rr)
+        cols = [st[:, 0], st[:, 1], st[:, 2], np.where(fin, rr, 0.0), fin.astype(np.float64)]
+        for q, v in enumerate(cols):
+            sums[w, q] = math.fsum(v) if exact else float(np.sum(v))
+        counts[w] = len(ii)
+    return sums, counts
 
 
 def heterozygosity_hap(hap, pop_rows):

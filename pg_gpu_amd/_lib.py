@@ -50,6 +50,25 @@ class BinnedArgs(ctypes.Structure):
         ("d_out", c_void_p),
         ("grid", c_int32),
         ("n_lo", c_int32 * MAX_POPS),
+        ("with_r2", c_int32),
+    ]
+
+
+class WindowArgs(ctypes.Structure):
+    _fields_ = [
+        ("d_panels", c_void_p),
+        ("d_site_stats", c_void_p),
+        ("d_mat", c_void_p),
+        ("ld", c_int64),
+        ("d_rows", c_void_p),
+        ("d_rtab", c_void_p),
+        ("d_windows", c_void_p),
+        ("n_windows", c_int64),
+        ("d_work", c_void_p),
+        ("n_work", c_int64),
+        ("d_out", c_void_p),
+        ("grid", c_int32),
+        ("n_lo", c_int32 * MAX_POPS),
     ]
 
 
@@ -65,6 +84,8 @@ _SIGS = {
     "pgld_n_stats": (c_int32, [c_int32, c_int32]),
     "pgld_binned_fused": (c_int32, [POINTER(PanelDesc), POINTER(BinnedArgs), c_void_p]),
     "pgld_binned_plain": (c_int32, [POINTER(PanelDesc), POINTER(BinnedArgs), c_void_p]),
+    "pgld_windows_fused": (c_int32, [POINTER(PanelDesc), POINTER(WindowArgs), c_void_p]),
+    "pgld_windows_plain": (c_int32, [POINTER(PanelDesc), POINTER(WindowArgs), c_void_p]),
     "pgld_count_block": (c_int32, [POINTER(PanelDesc), c_void_p, c_void_p, c_int32, c_int64,
                                    c_int64, c_int64, c_int64, c_void_p, c_void_p]),
     "pgld_pair_bounds": (c_int32, [c_void_p, c_int64, c_double, c_void_p, c_void_p]),

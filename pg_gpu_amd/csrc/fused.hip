@@ -271,9 +271,14 @@ __device__ __forceinline__ const double* rt_entry(const FusedParams& a, const do
   return srt + 4 * (a.rt_off[p] + e);
 }
 
-template <int P>
+// Statistic groups of a traits class T: groups 0 .. NG-1 hold the LD
+// statistics (GS each, consecutive), and with R2 one more group holds
+// (sum of r^2 over pairs where it is defined, number of such pairs) of the
+// first population.  Output row per bin: NSO = NS (+2) sums, then the pair count.
+template <int P, bool R2 = false>
 struct HapTraits {
   static constexpr int NP = P;
+  static constexpr bool WITH_R2 = R2;
   // Workgroups per CU: two waves per SIMD while the live state fits the 256
   // registers of a wave at that occupancy; three or four populations keep
   // more counts and run one wave per SIMD with the full register file.
@@ -283,6 +288,8 @@ struct HapTraits {
   static constexpr int NG = P == 1 ? HAP_NGROUPS_P1 : P == 2 ? HAP_NGROUPS_P2 : P == 3 ? HAP_NGROUPS_P3
                                                                                        : HAP_NGROUPS_P4;
   static constexpr int GS = NS < STATS_GROUP ? NS : STATS_GROUP;  // statistics per group
+  static constexpr int NSO = NS + (R2 ? 2 : 0);
+  static constexpr int NGT = NG + (R2 ? 1 : 0);
   struct Counts { v16u w0[P], w1[P]; };
   __device__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane, Counts& c) {
 #pragma unroll
@@ -299,15 +306,17 @@ struct HapTraits {
       const int n01 = c.w1[p][r] & 0xffff, n00 = c.w1[p][r] >> 16;
       pp[p] = hap_pop(n11, n10, n01, n00, 0, rt_entry(a, srt, p, n11 + n10 + n01 + n00));
     }
-    hap_stats_group<P, G>(pp, x);
+    if constexpr (G < NG) hap_stats_group<P, G>(pp, x);
+    else hap_r2(pp[0], x);
   }
 };
 
 // W = 3: nine 10-bit counts in three words per slot (population sizes up to
 // 1023); W = 5: 16-bit counts in five words (up to 32767).
-template <int P, int W>
+template <int P, int W, bool R2 = false>
 struct GenoTraits {
   static constexpr int NP = P;
+  static constexpr bool WITH_R2 = R2;
   // One population fits two waves per SIMD; more need the full register file
   // of a one-wave-per-SIMD launch.
   static constexpr int MINB = P == 1 ? 2 : 1;
@@ -316,6 +325,8 @@ struct GenoTraits {
   static constexpr int NG = P == 1 ? GENO_NGROUPS_P1 : P == 2 ? GENO_NGROUPS_P2 : P == 3 ? GENO_NGROUPS_P3
                                                                                           : GENO_NGROUPS_P4;
   static constexpr int GS = NS < STATS_GROUP ? NS : STATS_GROUP;
+  static constexpr int NSO = NS + (R2 ? 2 : 0);
+  static constexpr int NGT = NG + (R2 ? 1 : 0);
   struct Counts { v16u w[P][W]; };
   __device__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane, Counts& c) {
 #pragma unroll
@@ -351,7 +362,8 @@ struct GenoTraits {
       pp[p] = geno_pop(g, 0, rt_entry(a, srt, p, n));
       pp[p].n = n;
     }
-    geno_stats_group<P, G>(pp, x);
+    if constexpr (G < NG) geno_stats_group<P, G>(pp, x);
+    else geno_r2(pp[0], x);
   }
 };
 
@@ -448,7 +460,10 @@ struct TileCtx {
 template <class T, int G>
 __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& t,
                                            const typename T::Counts& cts) {
-  constexpr int NS = T::NS, GS = T::GS, S0 = G * GS;
+  // NS below is the output row width; GS the size of this group, S0 its start
+  constexpr int NS = T::NSO;
+  constexpr int GS = G < T::NG ? T::GS : 2;
+  constexpr int S0 = G < T::NG ? G * T::GS : T::NS;
   const int nb = a.n_bins, lane = t.lane;
   double acc[GS];
 #pragma unroll
@@ -521,7 +536,7 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
 template <class T, int G>
 __device__ __forceinline__ void tile_groups(const FusedParams& a, const TileCtx& t,
                                             const typename T::Counts& cts) {
-  if constexpr (G < T::NG) {
+  if constexpr (G < T::NGT) {
     tile_group<T, G>(a, t, cts);
     tile_groups<T, G + 1>(a, t, cts);
   }
@@ -529,7 +544,7 @@ __device__ __forceinline__ void tile_groups(const FusedParams& a, const TileCtx&
 
 template <class T>
 __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
-  constexpr int NS = T::NS, GS = T::GS;
+  constexpr int NS = T::NSO, GS = T::GS;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   double* sedges = reinterpret_cast<double*>(smem);
   double* srt = sedges + PGLD_MAX_BINS + 8;
@@ -607,7 +622,7 @@ static int num_cus() {
 
 template <class T>
 static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStream_t s) {
-  constexpr int NS = T::NS;
+  constexpr int NS = T::NSO;
   constexpr int P = T::NP;
   FusedParams a;
   a.panels = g->d_panels;
@@ -657,6 +672,142 @@ static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStre
   return PGLD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Ragged windows: every i < j pair inside each window [lo_w, hi_w) of sites,
+// summed per window.  The work list holds (window, row block, column block)
+// triples covering each window's upper triangle; a tile that straddles a
+// window edge appears once per window it touches, masked to that window.
+// ---------------------------------------------------------------------------
+struct WindowParams {
+  FusedParams f;           // panels, site stats, reciprocal table, V, layout
+  const int32_t* work;     // [n_work][3]
+  int64_t n_work;
+  const int64_t* win;      // [n_windows][2]
+  unsigned long long* out; // [n_windows][NSO + 1][3]
+};
+
+template <class T, int G>
+__device__ __forceinline__ void window_group(const WindowParams& w, const double* srt,
+                                             const typename T::Counts& cts, int lane, bool full,
+                                             int64_t i0, int64_t j0, int64_t lo, int64_t hi,
+                                             unsigned long long* dst_w) {
+  constexpr int NS = T::NSO;
+  constexpr int GS = G < T::NG ? T::GS : 2;
+  constexpr int S0 = G < T::NG ? G * T::GS : T::NS;
+  double acc[GS];
+#pragma unroll
+  for (int q = 0; q < GS; ++q) acc[q] = 0.0;
+  int count = 0;
+  const int64_t j = j0 + (lane & 31);
+#pragma unroll 1
+  for (int r = 0; r < 16; ++r) {
+    const int64_t i = i0 + mfma_row(lane, r);
+    if (full || (i >= lo && i < j && j < hi)) {
+      double x[GS];
+      T::template stats<G>(w.f, srt, cts, r, x);
+#pragma unroll
+      for (int q = 0; q < GS; ++q) acc[q] += x[q];
+      ++count;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < GS; ++q) acc[q] = pgld_wave_sum(acc[q]);
+  double v = acc[0];
+#pragma unroll
+  for (int q = 1; q < GS; ++q) v = lane == q ? acc[q] : v;
+  const int c = G == 0 ? pgld_wave_sum_int(count) : 0;
+  if (lane < GS) {
+    unsigned long long* dst = dst_w + (int64_t)(S0 + lane) * 3;
+    long long x2, x1, x0;
+    pgld_to_limbs(v, x2, x1, x0);
+    if (x0) atomicAdd(dst + 0, (unsigned long long)x0);
+    if (x1) atomicAdd(dst + 1, (unsigned long long)x1);
+    if (x2) atomicAdd(dst + 2, (unsigned long long)x2);
+  } else if (G == 0 && lane == GS && c) {
+    atomicAdd(dst_w + (int64_t)NS * 3, (unsigned long long)c);
+  }
+}
+
+template <class T, int G>
+__device__ __forceinline__ void window_groups(const WindowParams& w, const double* srt,
+                                              const typename T::Counts& cts, int lane, bool full,
+                                              int64_t i0, int64_t j0, int64_t lo, int64_t hi,
+                                              unsigned long long* dst_w) {
+  if constexpr (G < T::NGT) {
+    window_group<T, G>(w, srt, cts, lane, full, i0, j0, lo, hi, dst_w);
+    window_groups<T, G + 1>(w, srt, cts, lane, full, i0, j0, lo, hi, dst_w);
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256, T::MINB) window_kernel(WindowParams w) {
+  constexpr int NS = T::NSO;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  double* srt = reinterpret_cast<double*>(smem);
+  const FusedParams& a = w.f;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int p = 0; p < T::NP; ++p) {
+    const int len = a.K[p] - a.rt_lo[p] + 1;
+    for (int e = tid; e < 4 * len; e += blockDim.x)
+      srt[4 * a.rt_off[p] + e] = a.rtab[4 * a.rt_lo[p] + e];
+  }
+  __syncthreads();
+  // XCD-aware split as in fused_kernel: each XCD walks a contiguous stretch
+  // of the work list, whose consecutive items share a window's panels.
+  const int nx = 8;
+  const int xcd = blockIdx.x % nx;
+  const int waves_per_xcd = (gridDim.x / nx) * 4;
+  const int64_t seg_lo = w.n_work * xcd / nx, seg_hi = w.n_work * (xcd + 1) / nx;
+  for (int64_t k = seg_lo + (blockIdx.x / nx) * 4 + wave; k < seg_hi; k += waves_per_xcd) {
+    const int64_t win = w.work[3 * k], rb = w.work[3 * k + 1], cb = w.work[3 * k + 2];
+    const int64_t lo = w.win[2 * win], hi = w.win[2 * win + 1];
+    typename T::Counts cts;
+    T::counts(a, rb, cb, lane, cts);
+    const int64_t i0 = rb * 32, j0 = cb * 32;
+    const bool full = cb > rb && i0 >= lo && j0 + 32 <= hi;
+    window_groups<T, 0>(w, srt, cts, lane, full, i0, j0, lo, hi,
+                        w.out + (int64_t)win * (NS + 1) * 3);
+  }
+}
+
+template <class T>
+static int launch_windows(const PanelLayout& L, const pgld_window_args* g, hipStream_t s) {
+  WindowParams w;
+  FusedParams& a = w.f;
+  a = FusedParams{};
+  a.panels = g->d_panels;
+  a.site_stats = g->d_site_stats;
+  a.rtab = g->d_rtab;
+  a.V = L.n_sites;
+  a.block_bytes = L.block_bytes;
+  a.pop_off[0] = L.pop_off[0];
+  a.ksteps[0] = L.ksteps[0];
+  a.missing[0] = L.missing[0];
+  a.K[0] = L.pop_size[0];
+  const int lo = L.missing[0] ? g->n_lo[0] : L.pop_size[0];
+  if (lo < 0 || lo > L.pop_size[0]) return fail(PGLD_EINVAL, "n_lo out of range");
+  a.rt_lo[0] = lo;
+  a.rt_off[0] = 0;
+  a.rt_len = L.pop_size[0] - lo + 1;
+  w.work = g->d_work;
+  w.n_work = g->n_work;
+  w.win = g->d_windows;
+  w.out = reinterpret_cast<unsigned long long*>(g->d_out);
+  const size_t lds = (size_t)a.rt_len * 4 * sizeof(double);
+  if (lds > 160 * 1024) return fail(PGLD_EINVAL, "window kernel: reciprocal table exceeds LDS");
+  PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(window_kernel<T>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int grid = g->grid > 0 ? g->grid : T::MINB * num_cus();
+  grid = ((grid + 7) / 8) * 8;
+  const int64_t need = ((g->n_work + 31) / 32) * 8;
+  if (grid > need) grid = (int)need;
+  if (grid < 8) grid = 8;
+  hipLaunchKernelGGL(window_kernel<T>, dim3(grid), dim3(256), lds, s, w);
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}
+
 int validate_binned(const pgld_panel_desc* desc, const pgld_binned_args* g, PanelLayout* L) {
   int rc = make_layout(desc, L);
   if (rc) return rc;
@@ -665,6 +816,9 @@ int validate_binned(const pgld_panel_desc* desc, const pgld_binned_args* g, Pane
     return fail(PGLD_EINVAL, "n_bins must be in [1, 64]");
   if (!g->d_pos || !g->d_edges || !g->d_rtab || !g->d_out)
     return fail(PGLD_EINVAL, "null device pointer in binned args");
+  if (g->with_r2 != 0 && g->with_r2 != 1) return fail(PGLD_EINVAL, "with_r2 must be 0 or 1");
+  if (g->with_r2 && desc->n_pops != 1)
+    return fail(PGLD_EUNSUPPORTED, "with_r2 is built for one population");
   return PGLD_OK;
 }
 
@@ -690,6 +844,11 @@ extern "C" int pgld_binned_fused(const pgld_panel_desc* desc, const pgld_binned_
     return fail(PGLD_EINVAL, "fused kernel needs panels, site stats and a tile list");
   if (args->n_tiles == 0) return PGLD_OK;
   hipStream_t s = as_stream(stream);
+  if (args->with_r2) {  // one population (checked above)
+    if (desc->mode == PGLD_MODE_HAP) return launch_fused<HapTraits<1, true>>(L, args, s);
+    return L.pop_size[0] <= 1023 ? launch_fused<GenoTraits<1, 3, true>>(L, args, s)
+                                 : launch_fused<GenoTraits<1, 5, true>>(L, args, s);
+  }
   if (desc->mode == PGLD_MODE_HAP) {
     switch (desc->n_pops) {
       case 1: return launch_fused<HapTraits<1>>(L, args, s);
@@ -719,3 +878,22 @@ extern "C" int pgld_binned_fused(const pgld_panel_desc* desc, const pgld_binned_
   }
   return fail(PGLD_EUNSUPPORTED, "fused kernel: this (mode, n_pops) is not built yet");
 }
+
+extern "C" int pgld_windows_fused(const pgld_panel_desc* desc, const pgld_window_args* args,
+                                  void* stream) {
+  PanelLayout L;
+  int rc = make_layout(desc, &L);
+  if (rc) return rc;
+  if (!args) return fail(PGLD_EINVAL, "null window args");
+  if (desc->n_pops != 1) return fail(PGLD_EUNSUPPORTED, "window sums are built for one population");
+  if (args->n_work < 0 || args->n_windows < 0) return fail(PGLD_EINVAL, "negative sizes");
+  if (args->n_work == 0) return PGLD_OK;
+  if (!args->d_panels || !args->d_site_stats || !args->d_rtab || !args->d_work ||
+      !args->d_windows || !args->d_out)
+    return fail(PGLD_EINVAL, "null device pointer in window args");
+  hipStream_t s = as_stream(stream);
+  if (desc->mode == PGLD_MODE_HAP) return launch_windows<HapTraits<1, true>>(L, args, s);
+  return L.pop_size[0] <= 1023 ? launch_windows<GenoTraits<1, 3, true>>(L, args, s)
+                               : launch_windows<GenoTraits<1, 5, true>>(L, args, s);
+}
+

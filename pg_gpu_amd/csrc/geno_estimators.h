@@ -203,3 +203,26 @@ __device__ __forceinline__ double geno_pi2_alldiff(const GenoPop& a, const GenoP
                                                    const GenoPop& c, const GenoPop& d) {
   return 0.25 * (g_p(a) * g_r(b) + g_r(a) * g_p(b)) * (g_q(c) * g_s(d) + g_s(c) * g_q(d));
 }
+
+// ---- r^2 of dosages (pairwise-complete Pearson) ----------------------------
+// Over the individuals valid at both loci, x and y the 0/1/2 dosages:
+// r = (n Sxy - Sx Sy) / sqrt((n Sxx - Sx^2)(n Syy - Sy^2)).  Without missing
+// calls this is Rogers-Huff r (pg_gpu/ld_statistics.py:492-521).  Moments
+// order g[0..8] = n22 n21 n20 n12 n11 n10 n02 n01 n00 (first digit: locus i).
+// x[0] = r^2 (0 when a locus has no dosage variance), x[1] = 1 when defined.
+__device__ __forceinline__ void geno_r2(const GenoPop& p, double* x) {
+  const double* g = p.g;
+  const double X2 = g[0] + g[1] + g[2], X1 = g[3] + g[4] + g[5];
+  const double Y2 = g[0] + g[3] + g[6], Y1 = g[1] + g[4] + g[7];
+  const double sx = 2.0 * X2 + X1, sxx = 4.0 * X2 + X1;
+  const double sy = 2.0 * Y2 + Y1, syy = 4.0 * Y2 + Y1;
+  const double sxy = 4.0 * g[0] + 2.0 * (g[1] + g[3]) + g[4];
+  const double n = p.n;
+  const double num = n * sxy - sx * sy;            // exact integers below 2^53
+  const double vx = n * sxx - sx * sx, vy = n * syy - sy * sy;
+  const float nf = (float)num;
+  const float r2 = nf * nf * __builtin_amdgcn_rcpf((float)vx * (float)vy);
+  const bool ok = vx > 0.0 && vy > 0.0;
+  x[0] = ok ? (double)r2 : 0.0;
+  x[1] = ok ? 1.0 : 0.0;
+}

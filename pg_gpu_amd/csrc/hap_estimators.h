@@ -134,3 +134,19 @@ __device__ __forceinline__ double hap_pi2_alldiff(const HapPop& a, const HapPop&
   return (a.pA * b.qA + a.qA * b.pA) * (c.pB * d.qB + c.qB * d.pB) * 0.25 *
          (a.r1 * b.r1 * c.r1 * d.r1);
 }
+
+// ---- r^2 (pg_gpu/haplotype_kernels.py:35-46, ld_statistics.r_squared) ------
+// r^2 = D^2 / (pA qA pB qB) with frequencies; in counts
+// (c1 c4 - c2 c3)^2 / ((c1+c2)(c3+c4)(c1+c3)(c2+c4)).  Undefined (NaN in the
+// reference) when a locus is monomorphic among the valid samples, i.e. when
+// the denominator is 0.  x[0] = r^2 (0 when undefined), x[1] = 1 when defined.
+// The ratio is formed in FP32 (the numerator and denominator are exact).
+__device__ __forceinline__ void hap_r2(const HapPop& a, double* x) {
+  const double num = a.c1 * a.c4 - a.c2 * a.c3;
+  const double den = (a.pA * a.qA) * (a.pB * a.qB);
+  const float nf = (float)num;
+  const float r2 = nf * nf * __builtin_amdgcn_rcpf((float)den);
+  const bool ok = den > 0.0;
+  x[0] = ok ? (double)r2 : 0.0;
+  x[1] = ok ? 1.0 : 0.0;
+}

@@ -56,6 +56,7 @@ template <int P> struct PlainSet<PGLD_MODE_HAP, P> {
     if constexpr (P == 3) hap_stats_P3(p, o);
     if constexpr (P == 4) hap_stats_P4(p, o);
   }
+  __device__ static void r2(const Pop& p, double* o) { hap_r2(p, o); }
 };
 template <int P> struct PlainSet<PGLD_MODE_GENO, P> {
   static constexpr int NS = P == 1 ? GENO_NSTATS_P1 : P == 2 ? GENO_NSTATS_P2 : P == 3 ? GENO_NSTATS_P3
@@ -80,6 +81,7 @@ template <int P> struct PlainSet<PGLD_MODE_GENO, P> {
     if constexpr (P == 3) geno_stats_P3(p, o);
     if constexpr (P == 4) geno_stats_P4(p, o);
   }
+  __device__ static void r2(const Pop& p, double* o) { geno_r2(p, o); }
 };
 
 template <int NS>
@@ -96,10 +98,10 @@ __device__ __forceinline__ void plain_flush(unsigned long long* slimb, int bin, 
   atomicAdd(dst + 3 * NS, (unsigned long long)count);
 }
 
-template <int MODE, int P>
+template <int MODE, int P, bool R2>
 __global__ void __launch_bounds__(256) plain_kernel(PlainParams a) {
   using S = PlainSet<MODE, P>;
-  constexpr int NS = S::NS;
+  constexpr int NS = S::NS + (R2 ? 2 : 0);  // output row width
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   double* sedges = reinterpret_cast<double*>(smem);
   unsigned long long* slimb = reinterpret_cast<unsigned long long*>(sedges + PGLD_MAX_BINS + 8);
@@ -137,6 +139,7 @@ __global__ void __launch_bounds__(256) plain_kernel(PlainParams a) {
     for (int p = 0; p < P; ++p) pp[p] = S::pop(a, srows, p, i, j);
     double x[NS];
     S::eval(pp, x);
+    if constexpr (R2) S::r2(pp[0], x + S::NS);
     if (b != cur) {
       if (cur >= 0) plain_flush<NS>(slimb, cur, acc, count);
       cur = b;
@@ -162,9 +165,9 @@ __global__ void __launch_bounds__(256) plain_kernel(PlainParams a) {
   }
 }
 
-template <int MODE, int P>
+template <int MODE, int P, bool R2 = false>
 static int launch_plain(const PanelLayout& L, const pgld_binned_args* g, hipStream_t s) {
-  constexpr int NS = PlainSet<MODE, P>::NS;
+  constexpr int NS = PlainSet<MODE, P>::NS + (R2 ? 2 : 0);
   PlainParams a;
   a.mat = g->d_mat;
   a.ld = g->ld;
@@ -186,12 +189,80 @@ static int launch_plain(const PanelLayout& L, const pgld_binned_args* g, hipStre
                      (size_t)g->n_bins * (NS + 1) * 3 * sizeof(unsigned long long) +
                      (size_t)a.row_offset[P] * sizeof(int32_t);
   if (lds > 160 * 1024) return fail(PGLD_EINVAL, "plain kernel: too many sample rows for LDS");
-  PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(plain_kernel<MODE, P>),
+  PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(plain_kernel<MODE, P, R2>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (L.n_sites < 2) return PGLD_OK;
-  hipLaunchKernelGGL((plain_kernel<MODE, P>), dim3((unsigned)(L.n_sites - 1)), dim3(256), lds, s, a);
+  hipLaunchKernelGGL((plain_kernel<MODE, P, R2>), dim3((unsigned)(L.n_sites - 1)), dim3(256), lds, s, a);
   PGLD_LAUNCH_CHECK();
   return PGLD_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Ragged windows, plain form: one workgroup per anchor site i, one thread per
+// partner j in i's window, per-pair sample loops; workgroup tree reduction.
+// ---------------------------------------------------------------------------
+struct PlainWinParams {
+  PlainParams p;
+  const int64_t* win;  // [n_windows][2], sorted, non-overlapping
+  int64_t n_windows;
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(256) plain_window_kernel(PlainWinParams w) {
+  using S = PlainSet<MODE, 1>;
+  constexpr int NS = S::NS + 2;
+  __shared__ double red[256];
+  __shared__ int64_t s_win;
+  const PlainParams& a = w.p;
+  const int64_t i = blockIdx.x;
+  const int t = threadIdx.x;
+  if (t == 0) {  // last window with lo <= i
+    int64_t lo = 0, hi = w.n_windows;
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (w.win[2 * mid] <= i) lo = mid; else hi = mid;
+    }
+    s_win = (w.n_windows > 0 && w.win[2 * lo] <= i && i < w.win[2 * lo + 1]) ? lo : -1;
+  }
+  __syncthreads();
+  const int64_t win = s_win;
+  if (win < 0) return;  // uniform across the workgroup
+  const int64_t hi = w.win[2 * win + 1];
+  double acc[NS];
+  for (int q = 0; q < NS; ++q) acc[q] = 0.0;
+  int count = 0;
+  for (int64_t j = i + 1 + t; j < hi; j += blockDim.x) {
+    typename S::Pop pp[1];
+    pp[0] = S::pop(a, a.rows, 0, i, j);
+    double x[NS];
+    S::eval(pp, x);
+    S::r2(pp[0], x + S::NS);
+    for (int q = 0; q < NS; ++q) acc[q] += x[q];
+    ++count;
+  }
+  unsigned long long* dst = a.out + (int64_t)win * (NS + 1) * 3;
+  for (int q = 0; q <= NS; ++q) {
+    red[t] = q < NS ? acc[q] : (double)count;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (t < h) red[t] += red[t + h];
+      __syncthreads();
+    }
+    if (t == 0) {
+      const double v = red[0];
+      if (q < NS) {
+        long long x2, x1, x0;
+        pgld_to_limbs(v, x2, x1, x0);
+        if (x0) atomicAdd(dst + 3 * q + 0, (unsigned long long)x0);
+        if (x1) atomicAdd(dst + 3 * q + 1, (unsigned long long)x1);
+        if (x2) atomicAdd(dst + 3 * q + 2, (unsigned long long)x2);
+      } else if (v > 0.0) {
+        atomicAdd(dst + 3 * NS, (unsigned long long)v);
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -291,6 +362,10 @@ extern "C" int pgld_binned_plain(const pgld_panel_desc* desc, const pgld_binned_
   if (rc) return rc;
   if (!args->d_mat || !args->d_rows) return fail(PGLD_EINVAL, "plain kernel needs d_mat and d_rows");
   hipStream_t s = as_stream(stream);
+  if (args->with_r2) {  // one population (checked by validate_binned)
+    return desc->mode == PGLD_MODE_HAP ? launch_plain<PGLD_MODE_HAP, 1, true>(L, args, s)
+                                       : launch_plain<PGLD_MODE_GENO, 1, true>(L, args, s);
+  }
   if (desc->mode == PGLD_MODE_HAP) {
     switch (desc->n_pops) {
       case 1: return launch_plain<PGLD_MODE_HAP, 1>(L, args, s);
@@ -330,3 +405,36 @@ extern "C" int pgld_count_block(const pgld_panel_desc* desc, const int8_t* d_pan
   PGLD_LAUNCH_CHECK();
   return PGLD_OK;
 }
+
+extern "C" int pgld_windows_plain(const pgld_panel_desc* desc, const pgld_window_args* args,
+                                  void* stream) {
+  PanelLayout L;
+  int rc = make_layout(desc, &L);
+  if (rc) return rc;
+  if (!args) return fail(PGLD_EINVAL, "null window args");
+  if (desc->n_pops != 1) return fail(PGLD_EUNSUPPORTED, "window sums are built for one population");
+  if (!args->d_mat || !args->d_rows || !args->d_rtab || !args->d_windows || !args->d_out)
+    return fail(PGLD_EINVAL, "null device pointer in window args");
+  if (args->n_windows <= 0 || L.n_sites == 0) return PGLD_OK;
+  PlainWinParams w{};
+  w.p.mat = args->d_mat;
+  w.p.ld = args->ld;
+  w.p.rows = args->d_rows;
+  w.p.row_offset[0] = 0;
+  w.p.row_offset[1] = L.pop_size[0];
+  w.p.rtab = args->d_rtab;
+  w.p.out = reinterpret_cast<unsigned long long*>(args->d_out);
+  w.p.V = L.n_sites;
+  w.win = args->d_windows;
+  w.n_windows = args->n_windows;
+  hipStream_t s = as_stream(stream);
+  if (desc->mode == PGLD_MODE_HAP)
+    hipLaunchKernelGGL(plain_window_kernel<PGLD_MODE_HAP>, dim3((unsigned)L.n_sites), dim3(256), 0,
+                       s, w);
+  else
+    hipLaunchKernelGGL(plain_window_kernel<PGLD_MODE_GENO>, dim3((unsigned)L.n_sites), dim3(256),
+                       0, s, w);
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}
+

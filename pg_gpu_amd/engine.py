@@ -79,14 +79,20 @@ def decode_limbs(raw, n_stats):
 class PendingResult:
     """A launched binning job; ``finish()`` waits for it and decodes the limbs."""
 
-    def __init__(self, out, n_stats, n_pairs, n_tiles):
+    def __init__(self, out, n_stats, n_pairs, n_tiles, r2=False):
         self.out, self.n_stats, self.n_pairs, self.n_tiles = out, n_stats, n_pairs, n_tiles
+        self.r2 = r2
         self._res = None
 
     def finish(self):
         if self._res is None:
-            sums, counts = decode_limbs(self.out.cpu().numpy(), self.n_stats)
-            self._res = BinnedResult(sums, counts, self.n_pairs, self.n_tiles)
+            width = self.n_stats + (2 if self.r2 else 0)
+            sums, counts = decode_limbs(self.out.cpu().numpy(), width)
+            res = BinnedResult(sums[:, :self.n_stats].copy(), counts, self.n_pairs, self.n_tiles)
+            if self.r2:
+                res.r2_sum = sums[:, self.n_stats].copy()
+                res.r2_count = np.rint(sums[:, self.n_stats + 1]).astype(np.int64)
+            self._res = res
         return self._res
 
 
@@ -140,7 +146,7 @@ class BinnedPlan:
         if self.ns < 0:
             raise NotImplementedError("statistic set not available for this mode / n_pops")
 
-    def args(self, out, grid=0):
+    def args(self, out, grid=0, r2=False):
         pan = self.pan
         a = _lib.BinnedArgs()
         a.d_panels = _lib.ptr(pan.panels).value
@@ -163,16 +169,22 @@ class BinnedPlan:
         a.grid = int(grid)
         for p in range(pan.P):
             a.n_lo[p] = int(pan.n_lo[p])
+        a.with_r2 = int(bool(r2))
         return a
 
-    def run(self, kernel="fused", grid=0):
+    def run(self, kernel="fused", grid=0, r2=False):
+        """Launch one binning pass.  With ``r2`` (one population) the bins also
+        receive the sum of r^2 and the number of pairs where r^2 is defined."""
         torch = _torch()
         pan = self.pan
-        out = torch.zeros((self.nb, self.ns + 1, 3), dtype=torch.int64, device=pan.device)
-        a = self.args(out, grid)
+        if r2 and pan.P != 1:
+            raise NotImplementedError("r2 sums are built for one population")
+        width = self.ns + (2 if r2 else 0)
+        out = torch.zeros((self.nb, width + 1, 3), dtype=torch.int64, device=pan.device)
+        a = self.args(out, grid, r2)
         fn = _lib.lib().pgld_binned_fused if kernel == "fused" else _lib.lib().pgld_binned_plain
         _lib.check(fn(ctypes.byref(pan.desc), ctypes.byref(a), _lib.stream_handle(pan.stream)))
-        return PendingResult(out, self.ns, self.n_pairs, len(self.tiles))
+        return PendingResult(out, self.ns, self.n_pairs, len(self.tiles), r2)
 
 
 @dataclass
@@ -181,6 +193,16 @@ class BinnedResult:
     counts: np.ndarray      # [n_bins] pairs per bin
     n_pairs: int            # enumerated pairs (within the cutoff)
     n_tiles: int
+    r2_sum: np.ndarray = None    # [n_bins] sum of r^2 where defined (with r2=True)
+    r2_count: np.ndarray = None  # [n_bins] pairs with r^2 defined
+
+    @property
+    def mean_r2(self):
+        """Mean r^2 per bin over pairs where it is defined (NaN if none)."""
+        if self.r2_sum is None:
+            return None
+        with np.errstate(invalid="ignore", divide="ignore"):
+            return np.where(self.r2_count > 0, self.r2_sum / np.maximum(self.r2_count, 1), np.nan)
 
 
 class LDPanels:
@@ -268,17 +290,18 @@ class LDPanels:
         return BinnedPlan(self, positions, bins, max_dist, coords, tiles)
 
     def binned(self, positions, bins, max_dist=None, coords=None, kernel="fused", grid=0,
-               tiles=None, sync=True):
+               tiles=None, sync=True, r2=False):
         """Bin sums of every LD statistic over pairs within ``max_dist``.
 
         positions: sorted site positions (the cutoff coordinate), numpy or tensor
         bins:      increasing bin edges; pair distance d goes to bin b when
                    bins[b] <= d < bins[b+1] (np.digitize semantics)
         coords:    optional binning coordinate (e.g. Morgans); default positions
+        r2:        one population only: also sum r^2 per bin (see BinnedResult.mean_r2)
         With ``sync=False`` the launch is left in flight and a PendingResult is
         returned; call ``.finish()`` on it to wait and decode.
         """
-        pend = self.plan(positions, bins, max_dist, coords, tiles).run(kernel, grid)
+        pend = self.plan(positions, bins, max_dist, coords, tiles).run(kernel, grid, r2)
         return pend.finish() if sync else pend
 
     def count_block(self, pop, i0, ni, j0, nj):

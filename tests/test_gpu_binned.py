@@ -147,3 +147,44 @@ def test_four_pops_no_missing_ragged(cuda):
     rng = np.random.default_rng(32)
     hap, pos, pops = small_pops(rng, 0, 150, 20_000, 0.0, sizes=[3, 33, 64, 97])
     run_case(cuda, hap, pos, pops, np.array([0.0, 300.0, 2_000.0, 8_000.0]), 8_000.0)
+
+
+def check_r2(res, ref, n_ld, base):
+    """r^2 columns: the defined-pair count is exact; the r^2 sum is formed from
+    FP32 ratios (about 2^-22 relative per term).  The LD sums are bit-identical
+    to the same launch without r^2 (checked against the oracle elsewhere)."""
+    np.testing.assert_array_equal(res.r2_count, np.rint(ref[:, n_ld + 1]).astype(np.int64))
+    np.testing.assert_allclose(res.r2_sum, ref[:, n_ld], rtol=2e-6, atol=1e-9)
+    np.testing.assert_array_equal(res.sums, base.sums)
+    np.testing.assert_array_equal(res.counts, base.counts)
+
+
+@pytest.mark.parametrize("kernel", ["fused", "plain"])
+def test_single_pop_mean_r2(cuda, kernel):
+    """Workload c0 shape: DD, Dz, pi2 and mean r^2 per bin."""
+    import torch
+
+    w = synth.make("c0", scale=0.1)
+    hap = w.mat.copy()
+    hap[:, 7] = 0                                     # a monomorphic site: r^2 undefined
+    pan = LDPanels(torch.as_tensor(hap, device=cuda), w.pop_rows, "hap")
+    ref, cnt, _ = o.binned_sums_hap(hap, w.positions, w.pop_rows, w.bins, max_dist=w.max_dist,
+                                    r2=True)
+    res = pan.binned(w.positions, w.bins, w.max_dist, kernel=kernel, r2=True)
+    base = pan.binned(w.positions, w.bins, w.max_dist, kernel=kernel)
+    np.testing.assert_array_equal(res.counts, cnt)
+    check_r2(res, ref, 3, base)
+    assert np.all(res.r2_count <= res.counts) and res.r2_count.sum() < res.counts.sum()
+
+
+def test_single_pop_mean_r2_missing(cuda):
+    import torch
+
+    rng = np.random.default_rng(33)
+    hap, pos, pops = small_pops(rng, 0, 300, 30_000, 0.06, sizes=[90])
+    pan = LDPanels(torch.as_tensor(hap, device=cuda), pops, "hap")
+    bins = synth.log_bins(100, 10_000, 5)
+    ref, cnt, _ = o.binned_sums_hap(hap, pos, pops, bins, max_dist=10_000.0, r2=True)
+    res = pan.binned(pos, bins, 10_000.0, r2=True)
+    np.testing.assert_array_equal(res.counts, cnt)
+    check_r2(res, ref, 3, pan.binned(pos, bins, 10_000.0))

@@ -153,3 +153,19 @@ def test_geno_three_and_four_pops(cuda, sizes, missing):
     rng = np.random.default_rng(21)
     geno, pos, pops = make_geno(rng, sizes, 200, 20_000, missing)
     run_geno(cuda, geno, pos, pops, synth.log_bins(100, 8_000, 4), 8_000.0)
+
+
+@pytest.mark.parametrize("kernel", ["fused", "plain"])
+def test_geno_mean_r2_c2_scaled(cuda, kernel):
+    """Workload c2 shape: genotype DD, Dz, pi2 and mean dosage r^2 per bin."""
+    import torch
+
+    from test_gpu_binned import check_r2
+
+    w = synth.make("c2", scale=0.004)
+    pan = LDPanels(torch.as_tensor(w.mat, device=cuda), w.pop_rows, "geno")
+    ref, cnt, _ = o.binned_sums_geno(w.mat, w.positions, w.pop_rows, w.bins,
+                                     max_dist=w.max_dist, r2=True)
+    res = pan.binned(w.positions, w.bins, w.max_dist, kernel=kernel, r2=True)
+    np.testing.assert_array_equal(res.counts, cnt)
+    check_r2(res, ref, 3, pan.binned(w.positions, w.bins, w.max_dist, kernel=kernel))

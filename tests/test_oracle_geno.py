@@ -109,3 +109,44 @@ def test_moments_order_mapping():
     c[0, 0] = 2      # n00 -> g9
     g = go.to_moments(c)
     assert g[0, 0] == 5 and g[0, 8] == 2
+
+
+def test_dosage_r2_matches_brute_force_pearson():
+    """r2_dosage from the nine counts equals numpy's Pearson r^2 over the
+    individuals valid at both loci (and NaN exactly where a locus is constant)."""
+    from oracle import ld_oracle as o
+
+    rng = np.random.default_rng(7)
+    g = rng.integers(0, 3, (40, 30)).astype(np.int8)
+    g[rng.random(g.shape) < 0.1] = -1
+    g[:, 5] = np.where(g[:, 5] >= 0, 1, -1)           # monomorphic among the valid
+    ii, jj = np.triu_indices(30, 1)
+    c, _ = o.geno_counts(g, ii, jj)
+    got = go.r2_dosage(c)
+    for k, (i, j) in enumerate(zip(ii, jj)):
+        ok = (g[:, i] >= 0) & (g[:, j] >= 0)
+        x, y = g[ok, i].astype(float), g[ok, j].astype(float)
+        if x.std() == 0 or y.std() == 0:
+            assert np.isnan(got[k])
+        else:
+            assert abs(got[k] - np.corrcoef(x, y)[0, 1] ** 2) < 1e-12
+
+
+def test_haplotype_r2_matches_brute_force_pearson():
+    """ld_statistics.r_squared's count formula is the Pearson r^2 of the 0/1
+    alleles over samples valid at both loci."""
+    from oracle import ld_oracle as o
+
+    rng = np.random.default_rng(8)
+    h = (rng.random((50, 25)) < 0.4).astype(np.int8)
+    h[rng.random(h.shape) < 0.08] = -1
+    ii, jj = np.triu_indices(25, 1)
+    c, nv = o.hap_counts(h, ii, jj)
+    got = o.r_family(c, nv)[1]
+    for k, (i, j) in enumerate(zip(ii, jj)):
+        ok = (h[:, i] >= 0) & (h[:, j] >= 0)
+        x, y = h[ok, i].astype(float), h[ok, j].astype(float)
+        if x.std() == 0 or y.std() == 0:
+            assert np.isnan(got[k])
+        else:
+            assert abs(got[k] - np.corrcoef(x, y)[0, 1] ** 2) < 1e-12
