@@ -35,13 +35,33 @@ PEAK_I8_TOPS = 5000.0      # MI355X dense int8 MFMA, TOP/s
 PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
+R2_WORKLOADS = ("c0", "c2")   # mean r^2 per bin is part of these workloads
+
 WORKLOADS = {
-    "c0": "c0: 1 pop x 200 haplotypes, 5k SNPs / 1 Mb, pairs <= 100 kb, 10 log bins, 3 stats",
+    "c0": "c0: 1 pop x 200 haplotypes, 5k SNPs / 1 Mb, pairs <= 100 kb, 10 log bins, "
+          "DD/Dz/pi2 + mean r^2",
     "c1": "c1: 2 pops x 500 haplotypes, 200k SNPs / 20 Mb, 5% missing, pairs <= 200 kb, "
           "20 log bins 100 bp-200 kb, 15 stats",
     "c2": "c2: 1 pop x 1000 diploid genotypes, 100k SNPs / 10 Mb, 2% missing, pairs <= 100 kb, "
-          "15 log bins, 3 stats",
+          "15 log bins, DD/Dz/pi2 + mean dosage r^2",
+    "c3": "c3: 300 haplotypes, 5,000 x 50 kb windows, SNPs per window clip(Gamma(0.8, 500), "
+          "20, 4000), all pairs per window, DD/Dz/pi2 + mean r^2 per window",
+    "c4": "c4: 4 pops x 40/120/300/800 haplotypes, 500k SNPs / 50 Mb, 3% missing, 30% rare "
+          "sites, pairs <= 500 kb, 25 log bins, 105 stats",
 }
+
+
+def traffic_from_profile(name):
+    """HBM bytes per launch recorded by the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes in profiles/<name>_hbm_traffic.md (None if not profiled)."""
+    path = os.path.join(ROOT, "profiles", f"{name}_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if "traffic_bytes_per_launch:" in line:
+                return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
 
 
 def log(*a):
@@ -60,41 +80,71 @@ def _cpu_worker(block):
     t0 = time.perf_counter()
     sums, counts, n, absr = o.binned_sums(w.mat, w.positions, w.pop_rows, w.bins, w.max_dist,
                                           chunk=100_000, with_abs=True, anchors=block,
-                                          mode=w.mode)
+                                          mode=w.mode, r2=w.name in R2_WORKLOADS)
     return sums, counts, n, absr, time.perf_counter() - t0
 
 
+def _cpu_worker_windows(block):
+    from oracle import ld_oracle as o
+
+    w = _W
+    t0 = time.perf_counter()
+    sums, counts = o.window_sums(w.mat, w.windows[block[0]:block[1]], mode=w.mode)
+    return sums, counts, int(counts.sum()), None, time.perf_counter() - t0
+
+
 def cpu_baseline(w, target_s=15.0, max_workers=16):
-    """Oracle (NumPy restatement of the reference estimators) on an anchor
-    prefix, one process per core over contiguous anchor blocks."""
+    """Oracle (NumPy restatement of the reference estimators) on a prefix of
+    the work (anchor sites, or windows), one process per core over contiguous
+    blocks."""
     global _W
     from oracle import ld_oracle as o
 
     _W = w
     cores = max(1, min(os.cpu_count() or 1, max_workers))
-    # calibrate on one small block in this process
-    t0 = time.perf_counter()
-    _, _, n_cal, _ = o.binned_sums(w.mat, w.positions, w.pop_rows, w.bins, w.max_dist,
-                                   chunk=100_000, with_abs=True, anchors=(0, 64), mode=w.mode)
-    rate = n_cal / max(time.perf_counter() - t0, 1e-6)
-    jmax = np.searchsorted(w.positions, w.positions + w.max_dist, side="right")
-    per = np.maximum(0, jmax - np.arange(len(w.positions)) - 1)
-    cum = np.cumsum(per)
-    want = rate * cores * target_s
-    X = int(np.searchsorted(cum, want)) + 1
-    X = max(64 * cores, min(len(w.positions), (X // (64 * cores)) * 64 * cores))
-    edges = np.linspace(0, X, cores + 1).astype(int)
-    blocks = [(int(edges[k]), int(edges[k + 1])) for k in range(cores)]
-    log(f"cpu baseline: {cores} workers, anchors [0, {X}) ~ {int(cum[X - 1]):,} pairs")
+    if w.windows is not None:
+        k = w.windows[:, 1] - w.windows[:, 0]
+        cum = np.cumsum(k * (k - 1) // 2)
+        t0 = time.perf_counter()
+        n_cal = int(o.window_sums(w.mat, w.windows[:4], mode=w.mode)[1].sum())
+        rate = n_cal / max(time.perf_counter() - t0, 1e-6)
+        X = int(np.searchsorted(cum, rate * cores * target_s)) + 1
+        X = max(cores, min(len(k), X))
+        edges = np.linspace(0, X, cores + 1).astype(int)
+        blocks = [(int(edges[q]), int(edges[q + 1])) for q in range(cores)]
+        log(f"cpu baseline: {cores} workers, windows [0, {X}) ~ {int(cum[X - 1]):,} pairs")
+        worker = _cpu_worker_windows
+    else:
+        t0 = time.perf_counter()
+        _, _, n_cal, _ = o.binned_sums(w.mat, w.positions, w.pop_rows, w.bins, w.max_dist,
+                                       chunk=100_000, with_abs=True, anchors=(0, 64),
+                                       mode=w.mode, r2=w.name in R2_WORKLOADS)
+        rate = n_cal / max(time.perf_counter() - t0, 1e-6)
+        jmax = np.searchsorted(w.positions, w.positions + w.max_dist, side="right")
+        per = np.maximum(0, jmax - np.arange(len(w.positions)) - 1)
+        cum = np.cumsum(per)
+        want = rate * cores * target_s
+        X = int(np.searchsorted(cum, want)) + 1
+        X = max(64 * cores, min(len(w.positions), (X // (64 * cores)) * 64 * cores))
+        edges = np.linspace(0, X, cores + 1).astype(int)
+        blocks = [(int(edges[q]), int(edges[q + 1])) for q in range(cores)]
+        log(f"cpu baseline: {cores} workers, anchors [0, {X}) ~ {int(cum[X - 1]):,} pairs")
+        worker = _cpu_worker
     t0 = time.perf_counter()
     with mp.get_context("fork").Pool(cores) as pool:
-        parts = pool.map(_cpu_worker, blocks)
+        parts = pool.map(worker, blocks)
     wall = time.perf_counter() - t0
-    nb, ns = w.bins.size - 1, parts[0][0].shape[1]
-    sums = np.array([[math.fsum(p[0][b, s] for p in parts) for s in range(ns)] for b in range(nb)])
-    counts = sum(p[1] for p in parts)
     n_pairs = sum(p[2] for p in parts)
-    absr = sum(p[3] for p in parts)
+    if w.windows is not None:
+        sums = np.concatenate([p[0] for p in parts])
+        counts = np.concatenate([p[1] for p in parts])
+        absr = None
+    else:
+        nb, ns = w.bins.size - 1, parts[0][0].shape[1]
+        sums = np.array([[math.fsum(p[0][b, s] for p in parts) for s in range(ns)]
+                         for b in range(nb)])
+        counts = sum(p[1] for p in parts)
+        absr = sum(p[3] for p in parts)
     return dict(value=n_pairs / wall, cores=cores, X=X, n_pairs=n_pairs, wall=wall, sums=sums,
                 counts=counts, abs=absr)
 
@@ -147,21 +197,31 @@ def main():
     import torch
 
     from pg_gpu_amd.engine import TILE, LDPanels, band_tiles
+    from pg_gpu_amd.windows import WindowPlan
 
     dev = torch.device("cuda:0")
     mat = torch.as_tensor(w.mat, device=dev)
     pan = LDPanels(mat, w.pop_rows, w.mode)
-    tiles = band_tiles(w.positions, w.max_dist)
     ns = pan.n_stats
     stream = torch.cuda.current_stream()
-
-    plan = pan.plan(w.positions, w.bins, w.max_dist, tiles=tiles)
+    windows = w.windows is not None
+    r2 = w.name in R2_WORKLOADS
+    if windows:
+        plan = WindowPlan(pan, w.windows)
+        tiles = None
+        run = plan.run
+        n_bins = None
+    else:
+        tiles = band_tiles(w.positions, w.max_dist)
+        plan = pan.plan(w.positions, w.bins, w.max_dist, tiles=tiles)
+        run = lambda kernel="fused": plan.run(kernel, r2=r2)  # noqa: E731
+        n_bins = int(len(w.bins) - 1)
 
     def step(kernel="fused", ev=None):
         pan.repack()
         if ev is not None:
             ev[0].record(stream)
-        r = plan.run(kernel)
+        r = run(kernel)
         if ev is not None:
             ev[1].record(stream)
         return r
@@ -181,13 +241,16 @@ def main():
     total_ms = start.elapsed_time(stop)
     kern_ms = [a.elapsed_time(b) for a, b in evs]
     res = results[-1].finish()
-    n_pairs = res.n_pairs
+    n_pairs = int(res.n_pairs.sum()) if windows else res.n_pairs
     ms_step = total_ms / args.steps
     log(f"fused: {ms_step:.3f} ms/step, kernel {np.mean(kern_ms):.3f} ms, {n_pairs:,} pairs")
 
+    def main_sums(r):
+        return np.stack([r.dd, r.dz, r.pi2], axis=1) if windows else r.sums
+
     # all steps must agree bit for bit (integer accumulation)
     for r in results[:-1]:
-        assert np.array_equal(r.finish().sums, res.sums)
+        assert np.array_equal(main_sums(r.finish()), main_sums(res))
 
     plain = None
     if not args.no_plain:
@@ -196,23 +259,49 @@ def main():
         torch.cuda.synchronize()
         p_ms = ev[0].elapsed_time(ev[1])
         rp = rp.finish()
-        rel = float(np.max(np.abs(rp.sums - res.sums) / np.maximum(np.abs(res.sums), 1e-300)))
+        a, b = main_sums(rp), main_sums(res)
+        rel = float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+        cnt_eq = (np.array_equal(rp.n_pairs, res.n_pairs) if windows
+                  else np.array_equal(rp.counts, res.counts))
         plain = dict(ms=p_ms, value=n_pairs / (p_ms * 1e-3), rel_vs_fused=rel,
-                     counts_equal=bool(np.array_equal(rp.counts, res.counts)))
+                     counts_equal=bool(cnt_eq))
         log(f"plain: {p_ms:.1f} ms, {plain['value']:.3e} pairs/s, rel diff {rel:.2e}")
 
     prec = None
     if cpu:
         X = cpu["X"]
-        sub = tiles[tiles[:, 0] < X // TILE]
-        rg = pan.binned(w.positions, w.bins, w.max_dist, kernel="fused", tiles=sub)
-        scale = np.maximum(np.abs(cpu["sums"]), 1e-3 * cpu["abs"])
-        err = np.abs(rg.sums - cpu["sums"]) / np.maximum(scale, 1e-300)
-        prec = dict(max_rel_err=float(err.max()),
-                    max_rel_err_vs_sum=float(np.max(np.abs(rg.sums - cpu["sums"]) /
-                                                    np.maximum(np.abs(cpu["sums"]), 1e-300))),
-                    counts_equal=bool(np.array_equal(rg.counts, cpu["counts"])),
-                    sample_pairs=int(cpu["n_pairs"]))
+        if windows:
+            rg = WindowPlan(pan, w.windows[:X]).run().finish()
+            got = np.stack([rg.dd, rg.dz, rg.pi2], axis=1)
+            ref = cpu["sums"][:, :3]
+            scale = np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max(axis=0))
+            err = np.abs(got - ref) / np.maximum(scale, 1e-300)
+            prec = dict(max_rel_err=float(err.max()),
+                        max_rel_err_vs_sum=float(np.max(np.abs(got - ref) /
+                                                        np.maximum(np.abs(ref), 1e-300))),
+                        r2_sum_max_rel_err=float(np.max(np.abs(rg.r2_sum - cpu["sums"][:, 3]) /
+                                                        np.maximum(cpu["sums"][:, 3], 1e-300))),
+                        counts_equal=bool(np.array_equal(rg.n_pairs, cpu["counts"]) and
+                                          np.array_equal(rg.r2_count,
+                                                         np.rint(cpu["sums"][:, 4]).astype(int))),
+                        sample_pairs=int(cpu["n_pairs"]))
+        else:
+            sub = tiles[tiles[:, 0] < X // TILE]
+            rg = pan.binned(w.positions, w.bins, w.max_dist, kernel="fused", tiles=sub, r2=r2)
+            ref = cpu["sums"][:, :ns]
+            scale = np.maximum(np.abs(ref), 1e-3 * cpu["abs"][:, :ns])
+            err = np.abs(rg.sums - ref) / np.maximum(scale, 1e-300)
+            prec = dict(max_rel_err=float(err.max()),
+                        max_rel_err_vs_sum=float(np.max(np.abs(rg.sums - ref) /
+                                                        np.maximum(np.abs(ref), 1e-300))),
+                        counts_equal=bool(np.array_equal(rg.counts, cpu["counts"])),
+                        sample_pairs=int(cpu["n_pairs"]))
+            if r2:
+                prec["r2_sum_max_rel_err"] = float(np.max(
+                    np.abs(rg.r2_sum - cpu["sums"][:, ns]) /
+                    np.maximum(np.abs(cpu["sums"][:, ns]), 1e-300)))
+                prec["counts_equal"] = prec["counts_equal"] and bool(np.array_equal(
+                    rg.r2_count, np.rint(cpu["sums"][:, ns + 1]).astype(np.int64)))
         log(f"precision vs oracle on {cpu['n_pairs']:,} pairs: {prec}")
 
     k_ms = float(np.mean(kern_ms))
@@ -223,6 +312,10 @@ def main():
     nprod = [((9 if m else 4) if geno else (4 if m else 1)) for m in pan.has_missing]
     ops_per_pair = 2 * sum(q * k for q, k in zip(nprod, pan.pop_sizes))
     achieved = ops_per_pair * n_pairs / (k_ms * 1e-3) / 1e12
+    kname = (f"window_kernel<{'Geno' if geno else 'Hap'}Traits<1, r2>>" if windows else
+             f"fused_kernel<{'Geno' if geno else 'Hap'}Traits<{pan.P}{', r2' if r2 else ''}>>")
+    prof = f"{'windows' if windows else 'fused'}_{w.mode}{pan.P}_{w.name}"
+    traffic = traffic_from_profile(prof) if args.scale == 1.0 else None
     line = {
         "metric": "site-pairs/sec binned DD/Dz/pi2, I8-MFMA fused vs plain FP64; max rel err",
         "value": n_pairs / (ms_step * 1e-3),
@@ -234,24 +327,26 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "i8 counts -> i32 accum, f64 estimators",
+        "dtype": "i8 counts -> i32 accum, f64 estimators" + (", f32 r^2 ratios" if
+                                                            (r2 or windows) else ""),
         "data": "synthetic (founder-copying generator, seeded)",
         "config": {"workload": WORKLOADS.get(w.name, w.name) if args.scale == 1.0 else
                    f"{w.name} at scale {args.scale}",
-                   "n_sites": V, "pairs": n_pairs, "n_bins": int(len(w.bins) - 1),
-                   "n_stats": ns, "scale": args.scale, "params": w.params},
+                   "n_sites": V, "pairs": n_pairs, "n_bins": n_bins,
+                   "n_windows": None if not windows else int(len(w.windows)),
+                   "n_stats": ns, "r2": bool(r2 or windows), "scale": args.scale,
+                   "params": w.params},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_I8_TOPS,
-                     "unit": "TOP/s (int8)", "frac": achieved / PEAK_I8_TOPS, "traffic": None,
-                     "kernel": f"fused_kernel<{'Geno' if geno else 'Hap'}Traits<{pan.P}>>",
-                     "ops_per_pair": ops_per_pair,
-                     "kernel_ms": k_ms},
+                     "unit": "TOP/s (int8)", "frac": achieved / PEAK_I8_TOPS, "traffic": traffic,
+                     "traffic_profile": f"profiles/{prof}_hbm_traffic.md" if traffic else None,
+                     "kernel": kname, "ops_per_pair": ops_per_pair, "kernel_ms": k_ms},
         "plain_fp64": plain,
         "speedup_vs_plain": (plain["ms"] / k_ms) if plain else None,
         "precision": prec,
         "cpu_baseline": None if cpu is None else {
             "value": cpu["value"], "unit": "site-pairs/s", "cores": cpu["cores"], "kind": "port",
-            "sample": f"oracle (NumPy restatement) on anchors [0, {cpu['X']}): "
-                      f"{cpu['n_pairs']:,} pairs, {cpu['wall']:.1f}s wall"},
+            "sample": (f"oracle (NumPy restatement) on {'windows' if windows else 'anchors'} "
+                       f"[0, {cpu['X']}): {cpu['n_pairs']:,} pairs, {cpu['wall']:.1f}s wall")},
     }
     print(json.dumps(line))
 
