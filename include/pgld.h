@@ -224,6 +224,30 @@ int pgld_hap_calls(const double* d_counts, int64_t stride, int32_t n_pops,
                    const double* d_nvalid, int64_t n_pairs, const int32_t* calls,
                    int32_t n_calls, double* d_out, void* stream);
 
+/* Raw genotype estimator calls per pair.  d_counts: float64 [n][stride],
+ * population p in columns 9p..9p+8 as the nine counts n00, n01, ..., n22
+ * (first digit the genotype at locus i, as compute_genotype_counts_for_pairs
+ * returns them); d_nvalid, calls and d_out as in pgld_hap_calls.  Replaces
+ * compute_all_dd_geno / compute_all_dz_geno / compute_all_pi2_geno
+ * (pg_gpu/genotype_kernels.py:561-878) and the per-case functions of
+ * pg_gpu/ld_statistics_genotype.py:37-1505.  Where a population has too few
+ * valid individuals for an estimator's falling factorial the value is 0. */
+int pgld_geno_calls(const double* d_counts, int64_t stride, int32_t n_pops,
+                    const double* d_nvalid, int64_t n_pairs, const int32_t* calls,
+                    int32_t n_calls, double* d_out, void* stream);
+
+/* The full genotype statistic set per pair (pgld_n_stats(PGLD_MODE_GENO,
+ * n_pops) columns, ld_names order).  Replaces
+ * compute_multi_pop_statistics_batch_geno (pg_gpu/genotype_kernels.py:881-936). */
+int pgld_geno_stat_set(const double* d_counts, int64_t stride, int32_t n_pops,
+                       const double* d_nvalid, int64_t n_pairs, double* d_out, void* stream);
+
+/* sigma_D^2 = DD / pi2 per pair from one population's nine counts; NaN where
+ * fewer than 4 individuals are valid or pi2 <= 0.  Replaces sigma_d2_geno
+ * (pg_gpu/ld_statistics_genotype.py:1518-1598, genotype_kernels.py:91-169). */
+int pgld_sigma_d2_geno(const double* d_counts, int64_t stride, int64_t n_pairs, double* d_out,
+                       void* stream);
+
 /* The full moments statistic set (pgld_n_stats(PGLD_MODE_HAP, n_pops)
  * columns, ld_names order) per pair.  Replaces
  * compute_single_pop_statistics_batch / compute_two_pop_statistics_batch

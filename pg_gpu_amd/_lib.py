@@ -97,6 +97,11 @@ _SIGS = {
                                  POINTER(c_int32), c_int32, c_void_p, c_void_p]),
     "pgld_hap_stat_set": (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_int64, c_void_p,
                                     c_void_p]),
+    "pgld_geno_calls": (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_int64,
+                                  POINTER(c_int32), c_int32, c_void_p, c_void_p]),
+    "pgld_geno_stat_set": (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_int64, c_void_p,
+                                     c_void_p]),
+    "pgld_sigma_d2_geno": (c_int32, [c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
     "pgld_r_family": (c_int32, [c_void_p, c_int64, c_void_p, c_int64, c_int32, c_void_p,
                                 c_void_p]),
     "pgld_heterozygosity": (c_int32, [POINTER(PanelDesc), c_void_p, c_void_p, c_void_p,

@@ -9,6 +9,7 @@
 // reference does (numerator / denominator with a den > 0 guard).
 #include "pgld_internal.h"
 #include "stats_hap.inc"
+#include "stats_geno.inc"
 
 namespace pgld {
 
@@ -247,6 +248,112 @@ __global__ void __launch_bounds__(256) hap_set_kernel(const double* counts, int6
   for (int q = 0; q < NS; ++q) out[t * NS + q] = x[q];
 }
 
+// ---------------------------------------------------------------------------
+// Per-pair genotype estimators on FP64 nine-way counts.
+// ---------------------------------------------------------------------------
+// c: the nine counts in the reference's order n00, n01, ..., n22 (first digit
+// the genotype at locus i); n: valid individuals.
+__device__ __forceinline__ GenoPop geno_pop_div(const double* c, double n) {
+  GenoPop p;
+#pragma unroll
+  for (int m = 0; m < 9; ++m) p.g[m] = c[8 - m];  // moments order n22 .. n00
+  p.n = n;
+  const double f1 = n, f2 = f1 * (n - 1.0), f3 = f2 * (n - 2.0), f4 = f3 * (n - 3.0);
+  p.r1 = f1 > 0.0 ? 1.0 / f1 : 0.0;
+  p.r2 = f2 > 0.0 ? 1.0 / f2 : 0.0;
+  p.r3 = f3 > 0.0 ? 1.0 / f3 : 0.0;
+  p.r4 = f4 > 0.0 ? 1.0 / f4 : 0.0;
+  return p;
+}
+
+// Case dispatch of compute_all_{dd,dz,pi2}_geno (pg_gpu/genotype_kernels.py:
+// 561-878), populations passed in call order as the reference does.
+__device__ double geno_raw_call(const GenoPop* P, int kind, int i, int j, int k, int l) {
+  if (kind == 0) return i == j ? geno_dd_same(P[i]) : geno_dd_pair(P[i], P[j]);
+  if (kind == 1) {
+    if (i == j && j == k) return geno_dz_same(P[i]);
+    if (i == j) return geno_dz_iik(P[i], P[k]);
+    if (i == k) return geno_dz_iji(P[i], P[j]);
+    if (j == k) return geno_dz_ijj(P[i], P[j]);
+    return geno_dz_ijk(P[i], P[j], P[k]);
+  }
+  int cnt[PGLD_MAX_POPS] = {0, 0, 0, 0};
+  cnt[i]++; cnt[j]++; cnt[k]++; cnt[l]++;
+  int nu = 0, mc = 0, t3 = -1, s1 = -1;
+  for (int q = 0; q < PGLD_MAX_POPS; ++q) {
+    nu += cnt[q] > 0;
+    mc = cnt[q] > mc ? cnt[q] : mc;
+    if (cnt[q] == 3) t3 = q;
+    if (cnt[q] == 1) s1 = q;
+  }
+  if (nu == 1) return geno_pi2_same(P[i]);
+  if (mc == 3)
+    return (i == t3 && j == t3) ? geno_pi2_iiij(P[t3], P[s1]) : geno_pi2_ijii(P[t3], P[s1]);
+  if (i == j && k == l) return geno_pi2_iikk(P[i], P[k]);
+  if (i == j) return geno_pi2_iikl(P[i], P[k], P[l]);
+  if (k == l) return geno_pi2_ijkk(P[i], P[j], P[k]);
+  if ((i == k && j == l) || (i == l && j == k)) return geno_pi2_ijij(P[i], P[j]);
+  if (nu == 3) {
+    if (i == k) return geno_pi2_shared(P[i], P[j], P[l]);
+    if (i == l) return geno_pi2_shared(P[i], P[j], P[k]);
+    if (j == k) return geno_pi2_shared(P[j], P[i], P[l]);
+    return geno_pi2_shared(P[j], P[i], P[k]);
+  }
+  return geno_pi2_alldiff(P[i], P[j], P[k], P[l]);
+}
+
+__device__ __forceinline__ void load_geno_pops(const double* counts, int64_t stride,
+                                               const double* nv, int P, int64_t t, GenoPop* pp) {
+  for (int p = 0; p < P; ++p) {
+    const double* c = counts + t * stride + 9 * p;
+    const double given = nv ? nv[t * P + p] : -1.0;  // < 0: use the sum of the counts
+    double n = 0.0;
+    for (int m = 0; m < 9; ++m) n += c[m];
+    pp[p] = geno_pop_div(c, given >= 0.0 ? given : n);
+  }
+}
+
+__global__ void __launch_bounds__(256) geno_calls_kernel(HapCallArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.N) return;
+  GenoPop pp[PGLD_MAX_POPS];
+  load_geno_pops(a.counts, a.stride, a.nvalid, a.P, t, pp);
+  for (int c = 0; c < a.n_calls; ++c)
+    a.out[t * a.n_calls + c] =
+        geno_raw_call(pp, a.calls[c][0], a.calls[c][1], a.calls[c][2], a.calls[c][3], a.calls[c][4]);
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) geno_set_kernel(const double* counts, int64_t stride,
+                                                       const double* nvalid, int64_t N,
+                                                       double* out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  GenoPop pp[P];
+  load_geno_pops(counts, stride, nvalid, P, t, pp);
+  double x[P == 1 ? GENO_NSTATS_P1 : P == 2 ? GENO_NSTATS_P2 : P == 3 ? GENO_NSTATS_P3
+                                                                     : GENO_NSTATS_P4];
+  if constexpr (P == 1) geno_stats_P1(pp, x);
+  if constexpr (P == 2) geno_stats_P2(pp, x);
+  if constexpr (P == 3) geno_stats_P3(pp, x);
+  if constexpr (P == 4) geno_stats_P4(pp, x);
+  constexpr int NS = sizeof(x) / sizeof(double);
+  for (int q = 0; q < NS; ++q) out[t * NS + q] = x[q];
+}
+
+// sigma_D^2 = DD / pi2 of one population (ld_statistics_genotype.py:1518-1598,
+// genotype_kernels.py:91-169): NaN where n < 4 or pi2 <= 0.
+__global__ void __launch_bounds__(256) sigma_d2_geno_kernel(const double* counts, int64_t stride,
+                                                            int64_t N, double* out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  GenoPop pp[1];
+  load_geno_pops(counts, stride, nullptr, 1, t, pp);
+  pp[0].r4 = 1.0;  // the common 1/(n)_4 cancels: take the ratio of the numerators
+  const double dd = geno_dd_same(pp[0]), pi2 = geno_pi2_same(pp[0]);
+  out[t] = (pp[0].n >= 4.0 && pi2 > 0.0) ? dd / pi2 : __builtin_nan("");
+}
+
 // r, r^2, D' (haplotype_kernels.py:20-62 arithmetic)
 __global__ void r_family_kernel(const double* counts, int64_t stride, const double* nvalid,
                                 int64_t N, int which, double* out) {
@@ -391,3 +498,69 @@ extern "C" int pgld_r_family(const double* d_counts, int64_t stride, const doubl
   PGLD_LAUNCH_CHECK();
   return PGLD_OK;
 }
+
+static int check_calls(const int32_t* calls, int32_t n_calls, int32_t n_pops, HapCallArgs* a) {
+  if (n_calls < 1 || n_calls > 64) return fail(PGLD_EINVAL, "1..64 calls per launch");
+  for (int c = 0; c < n_calls; ++c) {
+    const int kind = calls[5 * c];
+    if (kind < 0 || kind > 2) return fail(PGLD_EINVAL, "call kind must be 0 (dd), 1 (dz), 2 (pi2)");
+    const int arity = kind + 2;
+    for (int q = 0; q < 5; ++q) a->calls[c][q] = calls[5 * c + q];
+    for (int q = 1; q <= arity; ++q)
+      if (a->calls[c][q] < 0 || a->calls[c][q] >= n_pops)
+        return fail(PGLD_EINVAL, "population index out of range");
+    for (int q = arity + 1; q < 5; ++q) a->calls[c][q] = 0;
+  }
+  a->n_calls = n_calls;
+  return PGLD_OK;
+}
+
+extern "C" int pgld_geno_calls(const double* d_counts, int64_t stride, int32_t n_pops,
+                               const double* d_nvalid, int64_t n_pairs, const int32_t* calls,
+                               int32_t n_calls, double* d_out, void* stream) {
+  if (n_pops < 1 || n_pops > PGLD_MAX_POPS) return fail(PGLD_EINVAL, "1-4 populations supported");
+  if (stride < 9 * n_pops) return fail(PGLD_EINVAL, "count rows narrower than 9 * n_pops");
+  HapCallArgs a;
+  int rc = check_calls(calls, n_calls, n_pops, &a);
+  if (rc) return rc;
+  a.counts = d_counts;
+  a.stride = stride;
+  a.nvalid = d_nvalid;
+  a.N = n_pairs;
+  a.P = n_pops;
+  a.out = d_out;
+  if (n_pairs == 0) return PGLD_OK;
+  hipLaunchKernelGGL(geno_calls_kernel, dim3(blocks_for(n_pairs)), dim3(256), 0, as_stream(stream),
+                     a);
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}
+
+extern "C" int pgld_geno_stat_set(const double* d_counts, int64_t stride, int32_t n_pops,
+                                  const double* d_nvalid, int64_t n_pairs, double* d_out,
+                                  void* stream) {
+  if (n_pops < 1 || n_pops > PGLD_MAX_POPS) return fail(PGLD_EINVAL, "1-4 populations supported");
+  if (stride < 9 * n_pops) return fail(PGLD_EINVAL, "count rows narrower than 9 * n_pops");
+  if (n_pairs == 0) return PGLD_OK;
+  const dim3 g(blocks_for(n_pairs)), b(256);
+  hipStream_t s = as_stream(stream);
+  switch (n_pops) {
+    case 1: hipLaunchKernelGGL(geno_set_kernel<1>, g, b, 0, s, d_counts, stride, d_nvalid, n_pairs, d_out); break;
+    case 2: hipLaunchKernelGGL(geno_set_kernel<2>, g, b, 0, s, d_counts, stride, d_nvalid, n_pairs, d_out); break;
+    case 3: hipLaunchKernelGGL(geno_set_kernel<3>, g, b, 0, s, d_counts, stride, d_nvalid, n_pairs, d_out); break;
+    default: hipLaunchKernelGGL(geno_set_kernel<4>, g, b, 0, s, d_counts, stride, d_nvalid, n_pairs, d_out); break;
+  }
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}
+
+extern "C" int pgld_sigma_d2_geno(const double* d_counts, int64_t stride, int64_t n_pairs,
+                                  double* d_out, void* stream) {
+  if (stride < 9) return fail(PGLD_EINVAL, "count rows narrower than 9");
+  if (n_pairs == 0) return PGLD_OK;
+  hipLaunchKernelGGL(sigma_d2_geno_kernel, dim3(blocks_for(n_pairs)), dim3(256), 0,
+                     as_stream(stream), d_counts, stride, n_pairs, d_out);
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}
+

@@ -163,3 +163,92 @@ def test_haplotype_matrix_two_pops_raw_and_mean(cuda):
         assert np.allclose(ratio, ratio[0], rtol=1e-12) and ratio[0] > 1
     single = hm.compute_ld_statistics_gpu_single_pop(fx["bp_bins"], raw=True)
     assert len(single) == 3 and all(len(v) == 3 for v in single.values())
+
+
+def _rand_geno_counts(rng, N, n_ind, missing):
+    g = rng.integers(0, 3, (n_ind, 2 * N)).astype(np.int8)
+    if missing:
+        g[rng.random(g.shape) < missing] = -1
+    c, nv = o.geno_counts(g, np.arange(N), np.arange(N, 2 * N))
+    return c, nv
+
+
+@pytest.mark.parametrize("P", [1, 2, 3])
+def test_genotype_batch_and_raw_calls(cuda, P):
+    """compute_all_{dd,dz,pi2}_geno and compute_multi_pop_statistics_batch_geno
+    against the oracle's kernel form (genotype_kernels.py:561-936)."""
+    from oracle import geno_oracle as go
+    from pg_gpu_amd import genotype_kernels as gk
+    from pg_gpu_amd.ld_pipeline import generate_stat_specs
+
+    rng = np.random.default_rng(50 + P)
+    sizes = [3, 9, 25][:P] if P == 3 else [14, 30][:P]
+    cl, nl = zip(*[_rand_geno_counts(rng, 400, k, 0.06) for k in sizes])
+    ref_pops = [go.GenoPop(c, nv) for c, nv in zip(cl, nl)]
+    specs = generate_stat_specs(P)
+    def same(got, ref):
+        # where the reference divides by a non-positive falling factorial
+        # (too few valid individuals) it yields inf/NaN; the build gives 0
+        fin = np.isfinite(ref)
+        np.testing.assert_allclose(got[fin], ref[fin], rtol=1e-10, atol=1e-13)
+        assert np.all(got[~fin] == 0.0)
+
+    got = gk.compute_multi_pop_statistics_batch_geno(list(cl), list(nl), None, specs)
+    with np.errstate(invalid="ignore"):
+        ref = go.multi_pop_stats_geno(list(cl), list(nl))
+    same(got.cpu().numpy(), ref)
+    # a custom spec list goes through the raw-call path
+    sub = specs[::3]
+    got2 = gk.compute_multi_pop_statistics_batch_geno(list(cl), list(nl), None, sub).cpu().numpy()
+    same(got2, ref[:, ::3])
+    pops = [gk.PopDataGeno(c, nv) for c, nv in zip(cl, nl)]
+    for kind, fn in (("dd", gk.compute_all_dd_geno), ("dz", gk.compute_all_dz_geno),
+                     ("pi2", gk.compute_all_pi2_geno)):
+        calls = sorted({pp for _, cs in specs for _, st, pp in cs if st == kind})
+        for pp, v in zip(calls, fn(pops, calls)):
+            same(v.cpu().numpy(), go.kernel_call(ref_pops, kind, pp))
+
+
+def test_haplotype_batch_custom_specs(cuda):
+    from pg_gpu_amd import haplotype_kernels as hk
+    from pg_gpu_amd.ld_pipeline import generate_stat_specs
+
+    rng = np.random.default_rng(60)
+    h = (rng.random((60, 400)) < 0.4).astype(np.int8)
+    h[rng.random(h.shape) < 0.05] = -1
+    ii, jj = np.arange(200), np.arange(200, 400)
+    rows = [range(0, 12), range(12, 35), range(35, 60)]
+    cl, nl = zip(*[o.hap_counts(h, ii, jj, list(r)) for r in rows])
+    specs = generate_stat_specs(3)
+    ref = o.multi_pop_stats_hap(list(cl), list(nl))
+    got = hk.compute_multi_pop_statistics_batch_hap(list(cl), list(nl), None, specs)
+    np.testing.assert_allclose(got.cpu().numpy(), ref, rtol=1e-10, atol=1e-13)
+    sub = specs[1::4]
+    got2 = hk.compute_multi_pop_statistics_batch_hap(list(cl), list(nl), None, sub)
+    np.testing.assert_allclose(got2.cpu().numpy(), ref[:, 1::4], rtol=1e-10, atol=1e-13)
+
+
+def test_sigma_d2_geno(cuda):
+    """DD/pi2 of one population's genotype counts (ld_statistics_genotype.py:1518-1598)."""
+    from oracle import geno_oracle as go
+    from pg_gpu_amd.genotype_matrix import GenotypeMatrix
+    from pg_gpu_amd.ld_statistics_genotype import sigma_d2_geno
+
+    rng = np.random.default_rng(61)
+    g = rng.integers(0, 3, (30, 40)).astype(np.int8)
+    g[:, 3] = 1                                   # monomorphic dosage: pi2 <= 0 -> NaN
+    gm = GenotypeMatrix(g, np.arange(40) * 10)
+    got = sigma_d2_geno(gm).cpu().numpy()
+    ii, jj = np.triu_indices(40, 1)
+    c, _ = o.geno_counts(g, ii, jj)
+    pop = go.GenoPop(c)
+    dd, pi2 = go.kernel_call([pop], "dd", (0, 0)), go.kernel_call([pop], "pi2", (0, 0, 0, 0))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ref = np.where(pi2 > 0, dd / pi2, np.nan)
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(ref))
+    ok = ~np.isnan(ref)
+    np.testing.assert_allclose(got[ok], ref[ok], rtol=1e-10)
+    g2 = g.copy()
+    g2[0, 0] = -1
+    with pytest.raises(ValueError):
+        sigma_d2_geno(GenotypeMatrix(g2, np.arange(40) * 10))
