@@ -39,6 +39,7 @@ enum { PGLD_MODE_HAP = 0, PGLD_MODE_GENO = 1 };
 
 #define PGLD_MAX_POPS 4
 #define PGLD_MAX_BINS 64
+#define PGLD_SCHED_WORDS 256   /* words of the tile-scheduler buffer */
 
 /* Library version (major*10000 + minor*100 + patch). */
 int pgld_version(void);
@@ -107,6 +108,10 @@ typedef struct {
   int32_t with_r2;             /* 1 (one population only): two more sums per bin, the
                                   sum of r^2 over pairs where it is defined and the
                                   number of such pairs (mean r^2 = ratio) */
+  uint32_t* d_sched;           /* PGLD_SCHED_WORDS zeroed words of tile-scheduler state
+                                  for this launch (fused kernel), or NULL for a static
+                                  tile schedule; launches sharing one buffer must not
+                                  overlap.  The result does not depend on it. */
 } pgld_binned_args;
 
 /* Number of statistics produced per bin for (mode, n_pops). */
@@ -144,6 +149,7 @@ typedef struct {
   long long* d_out;            /* [n_windows][n_stats + 3][3] limbs, zeroed */
   int32_t grid;                /* workgroups (0 = auto) */
   int32_t n_lo[PGLD_MAX_POPS]; /* as in pgld_binned_args */
+  uint32_t* d_sched;           /* as in pgld_binned_args (fused kernel) */
 } pgld_window_args;
 
 /* Per-window sums over every i < j pair of sites inside each window (no

@@ -51,6 +51,7 @@ class BinnedArgs(ctypes.Structure):
         ("grid", c_int32),
         ("n_lo", c_int32 * MAX_POPS),
         ("with_r2", c_int32),
+        ("d_sched", c_void_p),
     ]
 
 
@@ -69,8 +70,11 @@ class WindowArgs(ctypes.Structure):
         ("d_out", c_void_p),
         ("grid", c_int32),
         ("n_lo", c_int32 * MAX_POPS),
+        ("d_sched", c_void_p),
     ]
 
+
+SCHED_WORDS = 256  # PGLD_SCHED_WORDS
 
 _SIGS = {
     "pgld_version": (c_int32, []),

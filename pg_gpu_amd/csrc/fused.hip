@@ -51,6 +51,7 @@ struct FusedParams {
   int32_t rt_lo[PGLD_MAX_POPS];   // smallest n kept in the LDS table of pop p
   int32_t rt_off[PGLD_MAX_POPS];  // first entry of pop p in the LDS table
   int32_t rt_len;                 // total LDS table entries
+  uint32_t* sched;                // per-XCD tile counters (null: static schedule)
 };
 
 // 16 packed count words per lane (one per MFMA output slot).  Kept as a
@@ -279,10 +280,11 @@ template <int P, bool R2 = false>
 struct HapTraits {
   static constexpr int NP = P;
   static constexpr bool WITH_R2 = R2;
-  // Workgroups per CU: two waves per SIMD while the live state fits the 256
-  // registers of a wave at that occupancy; three or four populations keep
-  // more counts and run one wave per SIMD with the full register file.
-  static constexpr int MINB = P <= 2 ? 2 : 1;
+  // Workgroups per CU (waves per SIMD): as many as the live state allows
+  // without spilling in the tile loop -- three for one population (168
+  // registers), two for two (256), one for three or four populations, which
+  // keep more counts and use the full register file.
+  static constexpr int MINB = P == 1 ? 3 : P == 2 ? 2 : 1;
   static constexpr int NS = P == 1 ? HAP_NSTATS_P1 : P == 2 ? HAP_NSTATS_P2 : P == 3 ? HAP_NSTATS_P3
                                                                                      : HAP_NSTATS_P4;
   static constexpr int NG = P == 1 ? HAP_NGROUPS_P1 : P == 2 ? HAP_NGROUPS_P2 : P == 3 ? HAP_NGROUPS_P3
@@ -576,14 +578,21 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
   // XCD-aware split: workgroups are dealt round-robin to the 8 XCDs, so each
   // XCD gets a contiguous stretch of the tile list (ordered in L2-sized
   // squares of tiles); the waves of one XCD then share operand panels in
-  // that XCD's L2.
+  // that XCD's L2.  With a scheduler buffer, after its first tile a wave
+  // takes the next unclaimed tile of its XCD's stretch from an atomic
+  // counter, so the waves of an XCD stay within one square of tiles
+  // (a static stride lets fast and slow waves drift apart and the panels in
+  // flight outgrow the L2).  The bin sums do not depend on the assignment.
   const int nx = 8;
   const int xcd = blockIdx.x % nx;
   const int waves_per_xcd = (gridDim.x / nx) * 4;
   const int64_t seg_lo = a.n_tiles * xcd / nx, seg_hi = a.n_tiles * (xcd + 1) / nx;
+  uint32_t* ctr = a.sched ? a.sched + xcd * 32 : nullptr;
 
-  for (int64_t tile = seg_lo + (blockIdx.x / nx) * 4 + wave; tile < seg_hi;
-       tile += waves_per_xcd) {
+  for (int64_t local = (blockIdx.x / nx) * 4 + wave; seg_lo + local < seg_hi;) {
+    const int64_t tile = seg_lo + local;
+    uint32_t claim = 0;
+    if (ctr && lane == 0) claim = atomicAdd(ctr, 1u);  // the next tile, claimed early
     const int64_t rb = a.tiles[2 * tile], cb = a.tiles[2 * tile + 1];
     typename T::Counts cts;
     T::counts(a, rb, cb, lane, cts);
@@ -600,6 +609,8 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
     t.fast = (cb > rb) && (j1 == j0 + 31) && (a.pos[j1] <= a.pos[i0] + a.max_dist) &&
              t.bmin == t.bmax && t.bmin >= 0 && t.bmin < nb;
     tile_groups<T, 0>(a, t, cts);
+    local = ctr ? waves_per_xcd + (int64_t)__builtin_amdgcn_readfirstlane(__shfl(claim, 0, 64))
+                : local + waves_per_xcd;
   }
   // nothing may stay staged
   stage_drain<NS, GS>(t.stg, t.scw, slimb, lane, 0, nb - 1, 0);
@@ -637,6 +648,7 @@ static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStre
   a.tiles = g->d_tiles;
   a.n_tiles = g->n_tiles;
   a.out = reinterpret_cast<unsigned long long*>(g->d_out);
+  a.sched = g->d_sched;
   a.V = L.n_sites;
   a.block_bytes = L.block_bytes;
   int off = 0;
@@ -759,7 +771,11 @@ __global__ void __launch_bounds__(256, T::MINB) window_kernel(WindowParams w) {
   const int xcd = blockIdx.x % nx;
   const int waves_per_xcd = (gridDim.x / nx) * 4;
   const int64_t seg_lo = w.n_work * xcd / nx, seg_hi = w.n_work * (xcd + 1) / nx;
-  for (int64_t k = seg_lo + (blockIdx.x / nx) * 4 + wave; k < seg_hi; k += waves_per_xcd) {
+  uint32_t* ctr = a.sched ? a.sched + xcd * 32 : nullptr;
+  for (int64_t local = (blockIdx.x / nx) * 4 + wave; seg_lo + local < seg_hi;) {
+    const int64_t k = seg_lo + local;
+    uint32_t claim = 0;
+    if (ctr && lane == 0) claim = atomicAdd(ctr, 1u);
     const int64_t win = w.work[3 * k], rb = w.work[3 * k + 1], cb = w.work[3 * k + 2];
     const int64_t lo = w.win[2 * win], hi = w.win[2 * win + 1];
     typename T::Counts cts;
@@ -768,6 +784,8 @@ __global__ void __launch_bounds__(256, T::MINB) window_kernel(WindowParams w) {
     const bool full = cb > rb && i0 >= lo && j0 + 32 <= hi;
     window_groups<T, 0>(w, srt, cts, lane, full, i0, j0, lo, hi,
                         w.out + (int64_t)win * (NS + 1) * 3);
+    local = ctr ? waves_per_xcd + (int64_t)__builtin_amdgcn_readfirstlane(__shfl(claim, 0, 64))
+                : local + waves_per_xcd;
   }
 }
 
@@ -790,6 +808,7 @@ static int launch_windows(const PanelLayout& L, const pgld_window_args* g, hipSt
   a.rt_lo[0] = lo;
   a.rt_off[0] = 0;
   a.rt_len = L.pop_size[0] - lo + 1;
+  a.sched = g->d_sched;
   w.work = g->d_work;
   w.n_work = g->n_work;
   w.win = g->d_windows;

@@ -172,9 +172,11 @@ class BinnedPlan:
         a.with_r2 = int(bool(r2))
         return a
 
-    def run(self, kernel="fused", grid=0, r2=False):
+    def run(self, kernel="fused", grid=0, r2=False, dynamic=True):
         """Launch one binning pass.  With ``r2`` (one population) the bins also
-        receive the sum of r^2 and the number of pairs where r^2 is defined."""
+        receive the sum of r^2 and the number of pairs where r^2 is defined.
+        ``dynamic`` selects the atomic-counter tile schedule of the fused
+        kernel (False: static stride); the result is the same either way."""
         torch = _torch()
         pan = self.pan
         if r2 and pan.P != 1:
@@ -182,9 +184,15 @@ class BinnedPlan:
         width = self.ns + (2 if r2 else 0)
         out = torch.zeros((self.nb, width + 1, 3), dtype=torch.int64, device=pan.device)
         a = self.args(out, grid, r2)
+        sched = None
+        if dynamic:
+            sched = torch.zeros(_lib.SCHED_WORDS, dtype=torch.int32, device=pan.device)
+            a.d_sched = _lib.ptr(sched).value
         fn = _lib.lib().pgld_binned_fused if kernel == "fused" else _lib.lib().pgld_binned_plain
         _lib.check(fn(ctypes.byref(pan.desc), ctypes.byref(a), _lib.stream_handle(pan.stream)))
-        return PendingResult(out, self.ns, self.n_pairs, len(self.tiles), r2)
+        res = PendingResult(out, self.ns, self.n_pairs, len(self.tiles), r2)
+        res._sched = sched  # kept alive until the launch has been waited for
+        return res
 
 
 @dataclass

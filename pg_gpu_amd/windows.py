@@ -104,9 +104,16 @@ class WindowPlan:
         a.d_out = _lib.ptr(out).value
         a.grid = int(grid)
         a.n_lo[0] = int(pan.n_lo[0])
+        # Window items are short (one population, often no missing calls): a
+        # shared atomic counter per XCD would be claimed tens of millions of
+        # times per second, so the window kernel keeps the static schedule,
+        # under which the items of one window already run side by side.
+        sched = None
         fn = _lib.lib().pgld_windows_fused if kernel == "fused" else _lib.lib().pgld_windows_plain
         _lib.check(fn(ctypes.byref(pan.desc), ctypes.byref(a), _lib.stream_handle(pan.stream)))
-        return _PendingWindows(out, self.ns, nw)
+        res = _PendingWindows(out, self.ns, nw)
+        res._sched = sched
+        return res
 
 
 class _PendingWindows:

@@ -188,3 +188,18 @@ def test_single_pop_mean_r2_missing(cuda):
     res = pan.binned(pos, bins, 10_000.0, r2=True)
     np.testing.assert_array_equal(res.counts, cnt)
     check_r2(res, ref, 3, pan.binned(pos, bins, 10_000.0))
+
+
+def test_static_and_dynamic_tile_schedules_agree(cuda):
+    """The atomic-counter schedule and the static stride give the same limbs."""
+    import torch
+
+    w = synth.make("c1", scale=0.004)
+    pan = LDPanels(torch.as_tensor(w.mat, device=cuda), w.pop_rows, "hap")
+    plan = pan.plan(w.positions, w.bins, w.max_dist)
+    a = plan.run(dynamic=True).finish()
+    b = plan.run(dynamic=False).finish()
+    c = plan.run(dynamic=True, grid=16).finish()
+    np.testing.assert_array_equal(a.sums, b.sums)
+    np.testing.assert_array_equal(a.sums, c.sums)
+    np.testing.assert_array_equal(a.counts, b.counts)
