@@ -3,8 +3,9 @@
 // Every wave owns whole 32 x 32 site tiles of the distance band (row block rb,
 // column block cb >= rb) and processes each one end to end, with no
 // workgroup barrier inside the tile loop:
-//   1. counting -- v_mfma_i32_32x32x32_i8 multiplies the packed int8
-//      indicator panels (exact int32 accumulation): H.H, H.V, V.H, V.V give
+//   1. counting -- v_mfma_scale_f32_32x32x64_f8f6f4 multiplies the packed
+//      4-bit (E2M1) indicator panels, 64 samples per step, with FP32
+//      accumulation that is exact for counts below 2^24: H.H, H.V, V.H, V.V give
 //      n11, n10, n01, n00 of the tile's 1024 pairs (only H.H when the
 //      population has no missing calls; the marginals come from site
 //      tallies).  The counts are packed to 16-bit pairs in registers.
@@ -59,13 +60,6 @@ struct FusedParams {
 // register (index mode) instead of spilling the tile's counts to scratch.
 typedef uint32_t v16u __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ v16i zero16() {
-  v16i z;
-#pragma unroll
-  for (int q = 0; q < 16; ++q) z[q] = 0;
-  return z;
-}
-
 // Row (within a 32x32 MFMA output block) of accumulator register r of `lane`;
 // the column is lane & 31.
 __device__ __forceinline__ int mfma_row(int lane, int r) {
@@ -82,10 +76,10 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
   if (a.missing[p]) {
     const v4i* Av = A + ks * 64;
     const v4i* Bv = B + ks * 64;
-    v16i hh = zero16(), hv = zero16(), vh = zero16(), vv = zero16();
+    v16f hh = zero16f(), hv = zero16f(), vh = zero16f(), vv = zero16f();
     // Operand loads run PF k-steps ahead of the MFMAs (a register ring), so
     // the L2 latency of one step overlaps the multiplies of the previous ones.
-    constexpr int PF = 4;
+    constexpr int PF = 2;
     v4i ah[PF], av[PF], bh[PF], bv[PF];
 #pragma unroll
     for (int d = 0; d < PF; ++d) {
@@ -98,10 +92,10 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
       for (int d = 0; d < PF; ++d) {
         const int s = s0 + d;
         if (s < ks) {
-          hh = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah[d], bh[d], hh, 0, 0, 0);
-          hv = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah[d], bv[d], hv, 0, 0, 0);
-          vh = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[d], bh[d], vh, 0, 0, 0);
-          vv = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[d], bv[d], vv, 0, 0, 0);
+          hh = pgld_mfma_fp4(ah[d], bh[d], hh);
+          hv = pgld_mfma_fp4(ah[d], bv[d], hv);
+          vh = pgld_mfma_fp4(av[d], bh[d], vh);
+          vv = pgld_mfma_fp4(av[d], bv[d], vv);
           if (s + PF < ks) {
             const int n = (s + PF) * 64;
             ah[d] = A[n]; av[d] = Av[n]; bh[d] = B[n]; bv[d] = Bv[n];
@@ -111,13 +105,13 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const uint32_t n11 = hh[r];
-      w0[r] = n11 | ((uint32_t)(hv[r] - hh[r]) << 16);
-      w1[r] = (uint32_t)(vh[r] - hh[r]) | ((uint32_t)(vv[r] - hv[r] - vh[r] + hh[r]) << 16);
+      const int n11 = (int)hh[r], h_v = (int)hv[r], v_h = (int)vh[r], v_v = (int)vv[r];
+      w0[r] = (uint32_t)n11 | ((uint32_t)(h_v - n11) << 16);
+      w1[r] = (uint32_t)(v_h - n11) | ((uint32_t)(v_v - h_v - v_h + n11) << 16);
     }
   } else {
-    v16i hh = zero16();
-    constexpr int PF = 4;
+    v16f hh = zero16f();
+    constexpr int PF = 2;
     v4i ah[PF], bh[PF];
 #pragma unroll
     for (int d = 0; d < PF; ++d)
@@ -127,7 +121,7 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
       for (int d = 0; d < PF; ++d) {
         const int s = s0 + d;
         if (s < ks) {
-          hh = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah[d], bh[d], hh, 0, 0, 0);
+          hh = pgld_mfma_fp4(ah[d], bh[d], hh);
           if (s + PF < ks) { ah[d] = A[(s + PF) * 64]; bh[d] = B[(s + PF) * 64]; }
         }
       }
@@ -140,7 +134,7 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
     for (int r = 0; r < 16; ++r) {
       const int64_t i = rb * 32 + mfma_row(lane, r);
       const int ki = i < a.V ? cnt1[i] : 0;
-      const int n11 = hh[r];
+      const int n11 = (int)hh[r];
       w0[r] = (uint32_t)n11 | ((uint32_t)(ki - n11) << 16);
       w1[r] = (uint32_t)(kj - n11) | ((uint32_t)(K - ki - kj + n11) << 16);
     }
@@ -163,10 +157,10 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
   const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[p]) + lane;
   const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[p]) + lane;
   const int fs = ks * 64;  // feature stride in v4i
-  v16i bb = zero16(), bc = zero16(), cbm = zero16(), cc = zero16();
+  v16f bb = zero16f(), bc = zero16f(), cbm = zero16f(), cc = zero16f();
   int cnt[16][9];
   if (a.missing[p]) {
-    v16i bv = zero16(), cv = zero16(), vb = zero16(), vc = zero16(), vv = zero16();
+    v16f bv = zero16f(), cv = zero16f(), vb = zero16f(), vc = zero16f(), vv = zero16f();
     constexpr int PF = 2;
     v4i fa[PF][3], fb[PF][3];
 #pragma unroll
@@ -179,15 +173,15 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
       for (int d = 0; d < PF; ++d) {
         const int s = s0 + d;
         if (s < ks) {
-          bb = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][0], fb[d][0], bb, 0, 0, 0);
-          bc = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][0], fb[d][1], bc, 0, 0, 0);
-          bv = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][0], fb[d][2], bv, 0, 0, 0);
-          cbm = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][1], fb[d][0], cbm, 0, 0, 0);
-          cc = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][1], fb[d][1], cc, 0, 0, 0);
-          cv = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][1], fb[d][2], cv, 0, 0, 0);
-          vb = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][2], fb[d][0], vb, 0, 0, 0);
-          vc = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][2], fb[d][1], vc, 0, 0, 0);
-          vv = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][2], fb[d][2], vv, 0, 0, 0);
+          bb = pgld_mfma_fp4(fa[d][0], fb[d][0], bb);
+          bc = pgld_mfma_fp4(fa[d][0], fb[d][1], bc);
+          bv = pgld_mfma_fp4(fa[d][0], fb[d][2], bv);
+          cbm = pgld_mfma_fp4(fa[d][1], fb[d][0], cbm);
+          cc = pgld_mfma_fp4(fa[d][1], fb[d][1], cc);
+          cv = pgld_mfma_fp4(fa[d][1], fb[d][2], cv);
+          vb = pgld_mfma_fp4(fa[d][2], fb[d][0], vb);
+          vc = pgld_mfma_fp4(fa[d][2], fb[d][1], vc);
+          vv = pgld_mfma_fp4(fa[d][2], fb[d][2], vv);
           if (s + PF < ks) {
             const int n = (s + PF) * 64;
 #pragma unroll
@@ -198,15 +192,16 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int n11 = bb[r], n12 = bc[r], n21 = cbm[r], n22 = cc[r];
-      const int n10 = bv[r] - n11 - n12, n20 = cv[r] - n21 - n22;
-      const int n01 = vb[r] - n11 - n21, n02 = vc[r] - n12 - n22;
-      const int n00 = vv[r] - bv[r] - cv[r] - n01 - n02;
+      const int n11 = (int)bb[r], n12 = (int)bc[r], n21 = (int)cbm[r], n22 = (int)cc[r];
+      const int b_v = (int)bv[r], c_v = (int)cv[r];
+      const int n10 = b_v - n11 - n12, n20 = c_v - n21 - n22;
+      const int n01 = (int)vb[r] - n11 - n21, n02 = (int)vc[r] - n12 - n22;
+      const int n00 = (int)vv[r] - b_v - c_v - n01 - n02;
       cnt[r][0] = n22; cnt[r][1] = n21; cnt[r][2] = n20; cnt[r][3] = n12; cnt[r][4] = n11;
       cnt[r][5] = n10; cnt[r][6] = n02; cnt[r][7] = n01; cnt[r][8] = n00;
     }
   } else {
-    constexpr int PF = 4;
+    constexpr int PF = 2;
     v4i fa[PF][2], fb[PF][2];
 #pragma unroll
     for (int d = 0; d < PF; ++d)
@@ -218,10 +213,10 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
       for (int d = 0; d < PF; ++d) {
         const int s = s0 + d;
         if (s < ks) {
-          bb = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][0], fb[d][0], bb, 0, 0, 0);
-          bc = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][0], fb[d][1], bc, 0, 0, 0);
-          cbm = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][1], fb[d][0], cbm, 0, 0, 0);
-          cc = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d][1], fb[d][1], cc, 0, 0, 0);
+          bb = pgld_mfma_fp4(fa[d][0], fb[d][0], bb);
+          bc = pgld_mfma_fp4(fa[d][0], fb[d][1], bc);
+          cbm = pgld_mfma_fp4(fa[d][1], fb[d][0], cbm);
+          cc = pgld_mfma_fp4(fa[d][1], fb[d][1], cc);
           if (s + PF < ks) {
             const int n = (s + PF) * 64;
 #pragma unroll
@@ -238,7 +233,7 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
     for (int r = 0; r < 16; ++r) {
       const int64_t i = rb * 32 + mfma_row(lane, r);
       const int bvi = i < a.V ? st[i] : 0, cvi = i < a.V ? st[a.V + i] : 0;
-      const int n11 = bb[r], n12 = bc[r], n21 = cbm[r], n22 = cc[r];
+      const int n11 = (int)bb[r], n12 = (int)bc[r], n21 = (int)cbm[r], n22 = (int)cc[r];
       const int n10 = bvi - n11 - n12, n20 = cvi - n21 - n22;
       const int n01 = vbj - n11 - n21, n02 = vcj - n12 - n22;
       const int n00 = K - bvi - cvi - n01 - n02;

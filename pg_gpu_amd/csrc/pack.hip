@@ -1,13 +1,15 @@
 // Site tallies and packing of sample-major int8 matrices into MFMA panels.
 //
-// Panel format: for 32-site block b, population p, feature f and 32-sample
-// k-step s there is one 1 KiB fragment at
+// Panel format: 0/1 indicator features as 4-bit E2M1 values (1.0 = 0x2, 0).
+// For 32-site block b, population p, feature f and 64-sample k-step s there
+// is one 1 KiB fragment at
 //   b * block_bytes + pop_off[p] + (f * ksteps[p] + s) * 1024
-// holding, for lane l (0..63) and byte e (0..15),
-//   site  = 32 b + (l & 31),   sample = 32 s + 16 (l >> 5) + e
-// which is the operand layout a wave feeds to v_mfma_i32_32x32x32_i8 both as
-// the A (row sites) and the B (column sites) matrix.  Samples past the
-// population size and sites past n_sites are zero in every feature.
+// holding, for lane l (0..63), byte e (0..15) and nibble h (0 low, 1 high),
+//   site  = 32 b + (l & 31),   sample = 64 s + 32 (l >> 5) + 2 e + h
+// which is the operand layout a wave feeds to v_mfma_scale_f32_32x32x64_f8f6f4
+// (FP4 operands) both as the A (row sites) and the B (column sites) matrix.
+// Samples past the population size and sites past n_sites are zero in every
+// feature.
 #include <string>
 #include <thread>
 #include "pgld_internal.h"
@@ -78,60 +80,59 @@ __global__ void __launch_bounds__(256) scan_sites_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Pack: one workgroup per (32-site block, pop, 64-sample chunk).
-// Phase 1 stages 64 rows x 32 sites through LDS; phase 2 writes the 2 k-steps
-// x n_feat fragments, 16 bytes per lane, fully coalesced.
+// Pack: one workgroup per (4 site blocks = 128 sites, pop, 64-sample k-step).
+// Phase 1 stages 64 sample rows x 128 sites through LDS (4 threads read one
+// row's 128 contiguous bytes); phase 2 writes 4 x n_feat fragments, 16 bytes
+// per lane, fully coalesced.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) pack_kernel(
     const int8_t* __restrict__ mat, int64_t n_sites, int64_t ld,
     const int32_t* __restrict__ rows, int32_t row_begin, int32_t pop_size, int32_t mode,
-    int32_t n_feat, int32_t ksteps, int64_t pop_off, int64_t block_bytes,
+    int32_t n_feat, int32_t ksteps, int64_t pop_off, int64_t block_bytes, int64_t n_blocks,
     int8_t* __restrict__ panels) {
-  __shared__ int8_t tile[64][33];  // [sample][site], +1 pad
-  const int64_t blk = blockIdx.x;
-  const int chunk = blockIdx.y;    // 64-sample chunk
+  __shared__ int8_t tile[64][128 + 4];  // [sample][site]
+  const int64_t blk0 = (int64_t)blockIdx.x * 4;
+  const int ks = blockIdx.y;
   const int t = threadIdx.x;
-  // Phase 1: thread t loads 8 sites of sample row (t >> 2).
-  {
+  {  // Phase 1: thread t loads 32 sites of sample row (t >> 2).
     const int s = t >> 2;
-    const int k = chunk * 64 + s;
-    const int sx = (t & 3) * 8;
-    for (int e = 0; e < 8; ++e) {
-      const int64_t site = blk * 32 + sx + e;
-      int8_t v = 0;
-      bool have = (k < pop_size) && (site < n_sites);
-      if (have) v = mat[(int64_t)rows[row_begin + k] * ld + site];
-      tile[s][sx + e] = have ? v : (int8_t)-2;  // -2: padding (all features 0)
+    const int k = ks * 64 + s;
+    const int sx = (t & 3) * 32;
+    const bool row_ok = k < pop_size;
+    const int8_t* src = row_ok ? mat + (int64_t)rows[row_begin + k] * ld : mat;
+#pragma unroll 8
+    for (int e = 0; e < 32; ++e) {
+      const int64_t site = blk0 * 32 + sx + e;
+      const bool have = row_ok && site < n_sites;
+      tile[s][sx + e] = have ? src[site] : (int8_t)-2;  // -2: padding (every feature 0)
     }
   }
   __syncthreads();
-  // Phase 2: fragments (f, kstep in chunk) -> 64 lanes x 16 bytes.
-  const int n_frag = n_feat * 2;
-  for (int w = t; w < n_frag * 64; w += 256) {
-    const int frag = w >> 6, lane = w & 63;
-    const int f = frag >> 1, kk = frag & 1;
-    const int s_local = chunk * 2 + kk;
-    if (s_local >= ksteps) continue;
-    const int site = lane & 31;
-    const int s0 = kk * 32 + (lane >> 5) * 16;
+  // Phase 2: (block, feature, lane) -> 16 bytes = 32 samples of one site.
+  for (int w = t; w < 4 * n_feat * 64; w += 256) {
+    const int lane = w & 63, f = (w >> 6) % n_feat, bl = (w >> 6) / n_feat;
+    if (blk0 + bl >= n_blocks) continue;
+    const int site = bl * 32 + (lane & 31);
+    const int s0 = (lane >> 5) * 32;
     // feature code: hap {0: H, 1: V}; geno {0: I1, 1: I2, 2: V}
-    const int fcode = f;
     uint32_t word[4];
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
       uint32_t acc = 0;
-      for (int e = 0; e < 4; ++e) {
-        const int v = tile[s0 + q * 4 + e][site];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int v = tile[s0 + q * 8 + e][site];
         int bit;
         if (mode == PGLD_MODE_HAP)
-          bit = fcode == 0 ? (v == 1) : (v >= 0);
+          bit = f == 0 ? (v == 1) : (v >= 0);
         else
-          bit = fcode == 0 ? (v == 1) : (fcode == 1 ? (v == 2) : (v >= 0));
-        acc |= (uint32_t)bit << (8 * e);
+          bit = f == 0 ? (v == 1) : (f == 1 ? (v == 2) : (v >= 0));
+        acc |= (bit ? PGLD_FP4_ONE : 0u) << (4 * e);
       }
       word[q] = acc;
     }
-    int8_t* dst = panels + blk * block_bytes + pop_off +
-                  ((int64_t)f * ksteps + s_local) * PGLD_FRAG_BYTES + lane * 16;
+    int8_t* dst = panels + (blk0 + bl) * block_bytes + pop_off +
+                  ((int64_t)f * ksteps + ks) * PGLD_FRAG_BYTES + lane * 16;
     *reinterpret_cast<uint4*>(dst) = make_uint4(word[0], word[1], word[2], word[3]);
   }
 }
@@ -193,10 +194,10 @@ extern "C" int pgld_pack(const int8_t* d_mat, int64_t n_rows, int64_t n_sites, i
   hipStream_t s = as_stream(stream);
   for (int p = 0; p < desc->n_pops; ++p) {
     if (L.ksteps[p] == 0) continue;
-    const unsigned chunks = (unsigned)((L.ksteps[p] + 1) / 2);
-    hipLaunchKernelGGL(pack_kernel, dim3((unsigned)L.n_blocks, chunks), dim3(256), 0, s, d_mat,
-                       n_sites, ld, d_rows, pop_offsets[p], L.pop_size[p], desc->mode,
-                       L.n_feat[p], L.ksteps[p], L.pop_off[p], L.block_bytes, d_panels);
+    hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((L.n_blocks + 3) / 4), (unsigned)L.ksteps[p]),
+                       dim3(256), 0, s, d_mat, n_sites, ld, d_rows, pop_offsets[p],
+                       L.pop_size[p], desc->mode, L.n_feat[p], L.ksteps[p], L.pop_off[p],
+                       L.block_bytes, L.n_blocks, d_panels);
     PGLD_LAUNCH_CHECK();
   }
   return PGLD_OK;

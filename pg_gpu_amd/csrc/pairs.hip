@@ -46,7 +46,7 @@ __global__ void emit_pairs_kernel(const int64_t* __restrict__ offs, int64_t v_lo
 }
 
 // ---------------------------------------------------------------------------
-// Counts for explicit pair lists, from the packed panels (dot4 on bytes).
+// Counts for explicit pair lists, from the packed panels (dot8 on 4-bit values).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ const uint4* site_half(const int8_t* panels, int64_t block_bytes,
                                                   int64_t pop_off, int ks, int f, int s,
@@ -56,11 +56,13 @@ __device__ __forceinline__ const uint4* site_half(const int8_t* panels, int64_t 
   return reinterpret_cast<const uint4*>(frag + ((site & 31) + 32 * half) * 16);
 }
 
-__device__ __forceinline__ int dot16(uint4 a, uint4 b, int acc) {
-  acc = __builtin_amdgcn_udot4(a.x, b.x, acc, false);
-  acc = __builtin_amdgcn_udot4(a.y, b.y, acc, false);
-  acc = __builtin_amdgcn_udot4(a.z, b.z, acc, false);
-  acc = __builtin_amdgcn_udot4(a.w, b.w, acc, false);
+// Dot product of 32 4-bit indicator values (E2M1 1.0 = 2, so each matching
+// pair contributes 4; the caller divides the total by 4).
+__device__ __forceinline__ int dot32(uint4 a, uint4 b, int acc) {
+  acc = __builtin_amdgcn_udot8(a.x, b.x, acc, false);
+  acc = __builtin_amdgcn_udot8(a.y, b.y, acc, false);
+  acc = __builtin_amdgcn_udot8(a.z, b.z, acc, false);
+  acc = __builtin_amdgcn_udot8(a.w, b.w, acc, false);
   return acc;
 }
 
@@ -85,9 +87,13 @@ __device__ __forceinline__ void feature_products(const int8_t* panels, int64_t b
 #pragma unroll
       for (int a = 0; a < NF; ++a)
 #pragma unroll
-        for (int b = 0; b < NF; ++b) prod[a][b] = dot16(fi[a], fj[b], prod[a][b]);
+        for (int b = 0; b < NF; ++b) prod[a][b] = dot32(fi[a], fj[b], prod[a][b]);
     }
   }
+#pragma unroll
+  for (int a = 0; a < NF; ++a)
+#pragma unroll
+    for (int b = 0; b < NF; ++b) prod[a][b] >>= 2;
 }
 
 struct CountArgs {

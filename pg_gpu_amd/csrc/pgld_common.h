@@ -6,11 +6,30 @@
 #define PGLD_MAX_POPS 4
 #define PGLD_MAX_BINS 64
 #define PGLD_SITE_BLOCK 32      // sites per packed site block (MFMA 32x32 M/N)
-#define PGLD_KSTEP 32           // samples per packed k-step (MFMA 32x32x32 K)
-#define PGLD_FRAG_BYTES 1024    // one 32 sites x 32 samples int8 fragment
+#define PGLD_KSTEP 64           // samples per packed k-step (MFMA 32x32x64 K)
+#define PGLD_FRAG_BYTES 1024    // one 32 sites x 64 samples fragment of 4-bit values
+#define PGLD_FP4_ONE 2u         // 1.0 in FP4 E2M1 (sign 0, exponent 01, mantissa 0)
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v8i __attribute__((ext_vector_type(8)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+// Counting product of two 0/1 indicator fragments: 4-bit E2M1 operands
+// (1.0 or 0), FP32 accumulation.  Every partial sum is an integer below 2^24,
+// so the accumulated counts are exact.  The E8M0 block scales are 127 (2^0).
+__device__ __forceinline__ v16f pgld_mfma_fp4(v4i a, v4i b, v16f c) {
+  const v8i a8 = {a[0], a[1], a[2], a[3], 0, 0, 0, 0};
+  const v8i b8 = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
+  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8, b8, c, 4, 4, 0, 127, 0, 127);
+}
+
+__device__ __forceinline__ v16f zero16f() {
+  v16f z;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) z[q] = 0.0f;
+  return z;
+}
 
 // Fixed-point bin accumulator: value = (L2 * 2^64 + L1 * 2^32 + L0) * 2^-72.
 // A double is split as sign * (h + m * 2^-32 + s * 2^-64) * 2^-8 with h, m, s

@@ -23,7 +23,7 @@ struct PanelLayout {
   int32_t n_pops;
   int32_t mode;
   int32_t n_feat[PGLD_MAX_POPS];      // features per pop (1..3)
-  int32_t ksteps[PGLD_MAX_POPS];      // 32-sample steps per pop
+  int32_t ksteps[PGLD_MAX_POPS];      // 64-sample steps per pop
   int32_t pop_size[PGLD_MAX_POPS];
   int32_t missing[PGLD_MAX_POPS];
   int64_t pop_off[PGLD_MAX_POPS];     // byte offset of the pop inside a site block

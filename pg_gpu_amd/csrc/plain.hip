@@ -277,16 +277,15 @@ __global__ void __launch_bounds__(64) count_block_hap_kernel(
   const int64_t rb = i0 / 32 + blockIdx.y, cb = j0 / 32 + blockIdx.x;
   const v4i* A = reinterpret_cast<const v4i*>(panels + rb * block_bytes + pop_off) + lane;
   const v4i* B = reinterpret_cast<const v4i*>(panels + cb * block_bytes + pop_off) + lane;
-  v16i hh, hv, vh, vv;
-  for (int q = 0; q < 16; ++q) hh[q] = hv[q] = vh[q] = vv[q] = 0;
+  v16f hh = zero16f(), hv = zero16f(), vh = zero16f(), vv = zero16f();
   for (int s = 0; s < ks; ++s) {
     const v4i ah = A[s * 64], bh = B[s * 64];
-    hh = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh, hh, 0, 0, 0);
+    hh = pgld_mfma_fp4(ah, bh, hh);
     if (missing) {
       const v4i av = A[(ks + s) * 64], bv = B[(ks + s) * 64];
-      hv = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bv, hv, 0, 0, 0);
-      vh = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bh, vh, 0, 0, 0);
-      vv = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, vv, 0, 0, 0);
+      hv = pgld_mfma_fp4(ah, bv, hv);
+      vh = pgld_mfma_fp4(av, bh, vh);
+      vv = pgld_mfma_fp4(av, bv, vv);
     }
   }
   const int64_t j = cb * 32 + (lane & 31);
@@ -295,11 +294,11 @@ __global__ void __launch_bounds__(64) count_block_hap_kernel(
     if (i >= i0 + ni || j >= j0 + nj || i >= V || j >= V) continue;
     int hvr, vhr, vvr;
     if (missing) {
-      hvr = hv[r]; vhr = vh[r]; vvr = vv[r];
+      hvr = (int)hv[r]; vhr = (int)vh[r]; vvr = (int)vv[r];
     } else {
       hvr = cnt1[i]; vhr = cnt1[j]; vvr = K;
     }
-    const int n11 = hh[r];
+    const int n11 = (int)hh[r];
     const int64_t o = (i - i0) * nj + (j - j0);
     const int64_t plane = ni * nj;
     out[o] = n11;
@@ -319,25 +318,25 @@ __global__ void __launch_bounds__(64) count_block_geno_kernel(
   const v4i* A = reinterpret_cast<const v4i*>(panels + rb * block_bytes + pop_off) + lane;
   const v4i* B = reinterpret_cast<const v4i*>(panels + cb * block_bytes + pop_off) + lane;
   const int nf = missing ? 3 : 2;
-  v16i pr[3][3];
+  v16f pr[3][3];
   for (int x = 0; x < 3; ++x)
-    for (int y = 0; y < 3; ++y)
-      for (int q = 0; q < 16; ++q) pr[x][y][q] = 0;
+    for (int y = 0; y < 3; ++y) pr[x][y] = zero16f();
   for (int s = 0; s < ks; ++s) {
     v4i fa[3], fb[3];
     for (int f = 0; f < nf; ++f) { fa[f] = A[(f * ks + s) * 64]; fb[f] = B[(f * ks + s) * 64]; }
     for (int x = 0; x < nf; ++x)
-      for (int y = 0; y < nf; ++y)
-        pr[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[x], fb[y], pr[x][y], 0, 0, 0);
+      for (int y = 0; y < nf; ++y) pr[x][y] = pgld_mfma_fp4(fa[x], fb[y], pr[x][y]);
   }
   const int64_t j = cb * 32 + (lane & 31);
   for (int r = 0; r < 16; ++r) {
     const int64_t i = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     if (i >= i0 + ni || j >= j0 + nj || i >= V || j >= V) continue;
-    const int n11 = pr[0][0][r], n12 = pr[0][1][r], n21 = pr[1][0][r], n22 = pr[1][1][r];
+    const int n11 = (int)pr[0][0][r], n12 = (int)pr[0][1][r];
+    const int n21 = (int)pr[1][0][r], n22 = (int)pr[1][1][r];
     int bv, cv, vb, vc, vv;
     if (missing) {
-      bv = pr[0][2][r]; cv = pr[1][2][r]; vb = pr[2][0][r]; vc = pr[2][1][r]; vv = pr[2][2][r];
+      bv = (int)pr[0][2][r]; cv = (int)pr[1][2][r];
+      vb = (int)pr[2][0][r]; vc = (int)pr[2][1][r]; vv = (int)pr[2][2][r];
     } else {
       bv = st[i]; cv = st[V + i]; vb = st[j]; vc = st[V + j]; vv = K;
     }
