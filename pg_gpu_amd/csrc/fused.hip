@@ -66,19 +66,44 @@ __device__ __forceinline__ int mfma_row(int lane, int r) {
   return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
 }
 
+// acc1 += R.C1 and acc2 += R.C2 over the ks k-steps of one row operand R and
+// two column operands; operand loads run PF steps ahead in a register ring,
+// so the L2 latency of one step overlaps the multiplies of the previous ones.
+__device__ __forceinline__ void mfma_row_pass(const v4i* R, const v4i* C1, const v4i* C2, int ks,
+                                              v16f& acc1, v16f& acc2) {
+  constexpr int PF = 2;
+  v4i r[PF], c1[PF], c2[PF];
+#pragma unroll
+  for (int d = 0; d < PF; ++d)
+    if (d < ks) { r[d] = R[d * 64]; c1[d] = C1[d * 64]; c2[d] = C2[d * 64]; }
+  for (int s0 = 0; s0 < ks; s0 += PF) {
+#pragma unroll
+    for (int d = 0; d < PF; ++d) {
+      const int s = s0 + d;
+      if (s < ks) {
+        acc1 = pgld_mfma_fp4(r[d], c1[d], acc1);
+        acc2 = pgld_mfma_fp4(r[d], c2[d], acc2);
+        if (s + PF < ks) {
+          const int n = (s + PF) * 64;
+          r[d] = R[n]; c1[d] = C1[n]; c2[d] = C2[n];
+        }
+      }
+    }
+  }
+}
+
 // Counts of population p for tile (rb, cb), packed per slot as
 // w0 = n11 | n10 << 16, w1 = n01 | n00 << 16.
+template <bool TWO_PASS>
 __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t rb, int64_t cb,
                                             int lane, v16u& w0, v16u& w1) {
   const int ks = a.ksteps[p];
   const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[p]) + lane;
   const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[p]) + lane;
-  if (a.missing[p]) {
+  if (a.missing[p] && !TWO_PASS) {
+    v16f hh = zero16f(), hv = zero16f(), vh = zero16f(), vv = zero16f();
     const v4i* Av = A + ks * 64;
     const v4i* Bv = B + ks * 64;
-    v16f hh = zero16f(), hv = zero16f(), vh = zero16f(), vv = zero16f();
-    // Operand loads run PF k-steps ahead of the MFMAs (a register ring), so
-    // the L2 latency of one step overlaps the multiplies of the previous ones.
     constexpr int PF = 2;
     v4i ah[PF], av[PF], bh[PF], bv[PF];
 #pragma unroll
@@ -107,6 +132,25 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
     for (int r = 0; r < 16; ++r) {
       const int n11 = (int)hh[r], h_v = (int)hv[r], v_h = (int)vh[r], v_v = (int)vv[r];
       w0[r] = (uint32_t)n11 | ((uint32_t)(h_v - n11) << 16);
+      w1[r] = (uint32_t)(v_h - n11) | ((uint32_t)(v_v - h_v - v_h + n11) << 16);
+    }
+  } else if (a.missing[p]) {
+    // Two passes over K with one row operand each (H, then V) keep two
+    // accumulators live instead of four (for three waves per SIMD); the
+    // column operands are re-read from L1/L2 in the second pass.
+    v16f hh = zero16f(), hv = zero16f();
+    mfma_row_pass(A, B, B + ks * 64, ks, hh, hv);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int n11 = (int)hh[r], h_v = (int)hv[r];
+      w0[r] = (uint32_t)n11 | ((uint32_t)(h_v - n11) << 16);
+    }
+    v16f vh = zero16f(), vv = zero16f();
+    mfma_row_pass(A + ks * 64, B, B + ks * 64, ks, vh, vv);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int n11 = (int)(w0[r] & 0xffffu), h_v = n11 + (int)(w0[r] >> 16);
+      const int v_h = (int)vh[r], v_v = (int)vv[r];
       w1[r] = (uint32_t)(v_h - n11) | ((uint32_t)(v_v - h_v - v_h + n11) << 16);
     }
   } else {
@@ -290,7 +334,7 @@ struct HapTraits {
   struct Counts { v16u w0[P], w1[P]; };
   __device__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane, Counts& c) {
 #pragma unroll
-    for (int p = 0; p < P; ++p) tile_counts(a, p, rb, cb, lane, c.w0[p], c.w1[p]);
+    for (int p = 0; p < P; ++p) tile_counts<(MINB >= 3)>(a, p, rb, cb, lane, c.w0[p], c.w1[p]);
   }
   // statistics G*GS .. G*GS+GS-1 of the pair in accumulator slot r
   template <int G>
@@ -453,7 +497,66 @@ struct TileCtx {
   int64_t i0, j0;
 };
 
-// One pass over the tile's 1024 pairs for statistic group G.
+// Wave totals of a group's per-lane sums for one bin, added to the workgroup
+// limbs (lane q < GS: statistic S0+q); with COUNT, `pairs` joins the bin's
+// pair counter.
+template <int NS, int GS, int S0, bool COUNT>
+__device__ __forceinline__ void reduce_to_limbs(const TileCtx& t, double (&acc)[GS], int bin,
+                                                unsigned long long pairs) {
+  const int lane = t.lane;
+#pragma unroll
+  for (int q = 0; q < GS; ++q) acc[q] = pgld_wave_sum(acc[q]);  // valid in lanes 0..31
+  double v = acc[0];
+#pragma unroll
+  for (int q = 1; q < GS; ++q) v = lane == q ? acc[q] : v;
+  if (lane < GS) {
+    unsigned long long* dst = t.slimb + ((int64_t)bin * (NS + 1) + S0 + lane) * 3;
+    long long x2, x1, x0;
+    pgld_to_limbs(v, x2, x1, x0);
+    atomicAdd(dst + 0, (unsigned long long)x0);
+    atomicAdd(dst + 1, (unsigned long long)x1);
+    atomicAdd(dst + 2, (unsigned long long)x2);
+  } else if (COUNT && lane == GS) {
+    atomicAdd(t.slimb + ((int64_t)bin * (NS + 1) + NS) * 3, pairs);
+  }
+}
+
+// Bins of the lane's 16 pairs of a tile that is not single-bin: 8 bits per
+// slot r in word r >> 2, 0xFF where the pair is not counted (i >= j, beyond
+// the cutoff, outside the bins).  All loads are issued before the epilogue.
+__device__ __forceinline__ void tile_bins(const FusedParams& a, const TileCtx& t,
+                                          uint32_t (&bw)[4]) {
+  const int nb = a.n_bins, lane = t.lane;
+  const int64_t j = t.j0 + (lane & 31);
+  const bool jok = j < a.V;
+  const double pos_j = jok ? a.pos[j] : 0.0;
+  const double co_j = (jok && !a.coord_is_pos) ? a.coord[j] : pos_j;
+  int guess = nb - 1;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t i = t.i0 + 8 * q + 4 * (lane >> 5) + k;  // mfma_row(lane, 4q + k)
+      uint32_t b8 = 0xFF;
+      if (jok && i < j) {
+        const double pi = a.pos[i];
+        const double d = a.coord_is_pos ? (pos_j - pi) : fabs(co_j - a.coord[i]);
+        const int b = pgld_bin_scan(d, t.sedges, nb, guess, nb - 1);
+        if (pos_j <= pi + a.max_dist && b >= 0 && b < nb) {
+          b8 = (uint32_t)b;
+          guess = b;
+        }
+      }
+      w |= b8 << (8 * k);
+    }
+    bw[q] = w;
+  }
+}
+
+// One pass over the tile's 1024 pairs for statistic group G.  Single-bin
+// tiles are reduced to limbs at the tile end, so FP64 rounding never spans
+// more than one tile (and the result does not depend on the schedule).
 template <class T, int G>
 __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& t,
                                            const typename T::Counts& cts) {
@@ -474,42 +577,17 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
 #pragma unroll
       for (int q = 0; q < GS; ++q) acc[q] += x[q];
     }
-    // wave totals (lanes 0..31), then lane q < GS adds statistic S0+q's limbs
-#pragma unroll
-    for (int q = 0; q < GS; ++q) acc[q] = pgld_wave_sum(acc[q]);
-    double v = acc[0];
-#pragma unroll
-    for (int q = 1; q < GS; ++q) v = lane == q ? acc[q] : v;
-    if (lane < GS) {
-      unsigned long long* dst = t.slimb + ((int64_t)t.bmin * (NS + 1) + S0 + lane) * 3;
-      long long x2, x1, x0;
-      pgld_to_limbs(v, x2, x1, x0);
-      atomicAdd(dst + 0, (unsigned long long)x0);
-      atomicAdd(dst + 1, (unsigned long long)x1);
-      atomicAdd(dst + 2, (unsigned long long)x2);
-    } else if (G == 0 && lane == GS) {
-      atomicAdd(t.slimb + ((int64_t)t.bmin * (NS + 1) + NS) * 3, 1024ull);
-    }
+    reduce_to_limbs<NS, GS, S0, G == 0>(t, acc, t.bmin, 1024ull);
   } else {
-    const int64_t j = t.j0 + (lane & 31);
-    const bool jok = j < a.V;
-    const double pos_j = jok ? a.pos[j] : 0.0;
-    const double co_j = jok ? a.coord[j] : 0.0;
-    int cur = -1, count = 0, guess = nb - 1;
+    uint32_t bw[4];
+    tile_bins(a, t, bw);
+    int cur = -1, count = 0;
 #pragma unroll 1
     for (int r = 0; r < 16; ++r) {
-      const int64_t i = t.i0 + mfma_row(lane, r);
-      bool ok = jok && (i < j);
-      int b = -1;
-      if (ok) {
-        const double pi = a.pos[i];
-        ok = pos_j <= pi + a.max_dist;
-        const double d = a.coord_is_pos ? (pos_j - pi) : fabs(co_j - a.coord[i]);
-        b = pgld_bin_scan(d, t.sedges, nb, guess, nb - 1);
-        ok = ok && b >= 0 && b < nb;
-        guess = b < 0 ? 0 : (b >= nb ? nb - 1 : b);
-      }
-      if (ok) {
+      const int rq = r >> 2;
+      const uint32_t w = rq == 0 ? bw[0] : rq == 1 ? bw[1] : rq == 2 ? bw[2] : bw[3];
+      const int b = (int)((w >> (8 * (r & 3))) & 0xFFu);
+      if (b != 0xFF) {
         double x[GS];
         T::template stats<G>(a, t.srt, cts, r, x);
         if (b != cur) {

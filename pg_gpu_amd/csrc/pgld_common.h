@@ -40,12 +40,15 @@ __device__ __forceinline__ v16f zero16f() {
 // sums do not depend on the order in which workgroups, waves or lanes add
 // their contributions.
 
-// Exact conversion of a count (< 2^31) to FP64 through the bit pattern of
-// 2^52 + v.  The result is opaque to the compiler's integer reasoning, so
-// FP64 products and sums of counts stay single FP64 instructions instead of
-// being rewritten as integer multiplies followed by conversions.
+// Exact conversion of a count to FP64 (one v_cvt_f64_u32).  Written as an
+// asm statement so that the result is opaque to the compiler's integer
+// reasoning: FP64 products and sums of counts then stay single FP64
+// instructions instead of being rewritten as integer multiplies followed by
+// conversions.
 __device__ __forceinline__ double pgld_count_f64(uint32_t v) {
-  return __hiloint2double(0x43300000, (int)v) - 4503599627370496.0;
+  double d;
+  asm("v_cvt_f64_u32 %0, %1" : "=v"(d) : "v"(v));
+  return d;
 }
 
 #define PGLD_LIMBS 3
