@@ -32,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_I8_TOPS = 5000.0      # MI355X dense int8 MFMA, TOP/s
+PEAK_FP4_TOPS = 10100.0    # MI355X dense FP4 MFMA (the counting stage's operands), TOP/s
 PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
@@ -336,8 +337,10 @@ def main():
                    "n_windows": None if not windows else int(len(w.windows)),
                    "n_stats": ns, "r2": bool(r2 or windows), "scale": args.scale,
                    "params": w.params},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_I8_TOPS,
-                     "unit": "TOP/s (int8)", "frac": achieved / PEAK_I8_TOPS, "traffic": traffic,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP4_TOPS,
+                     "unit": "TOP/s (FP4 MFMA on 0/1 indicators)",
+                     "frac": achieved / PEAK_FP4_TOPS, "frac_of_i8_peak": achieved / PEAK_I8_TOPS,
+                     "traffic": traffic,
                      "traffic_profile": f"profiles/{prof}_hbm_traffic.md" if traffic else None,
                      "kernel": kname, "ops_per_pair": ops_per_pair, "kernel_ms": k_ms},
         "plain_fp64": plain,
