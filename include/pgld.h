@@ -41,6 +41,11 @@ enum { PGLD_MODE_HAP = 0, PGLD_MODE_GENO = 1 };
 #define PGLD_MAX_BINS 64
 #define PGLD_SCHED_WORDS 256   /* words of the tile-scheduler buffer */
 
+/* Kinds of pgld_pair_block. */
+#define PGLD_PAIR_SIGMA_D2 0   /* haplotypes: unbiased D^2 / pi^2 ratio */
+#define PGLD_PAIR_R2_NAIVE 1   /* haplotypes: frequency r^2 (per-site frequencies) */
+#define PGLD_PAIR_R2_DOSAGE 2  /* genotypes: r^2 of mean-centred dosages */
+
 /* Library version (major*10000 + minor*100 + patch). */
 int pgld_version(void);
 /* Message for the last error on this thread ("" if none). */
@@ -180,6 +185,24 @@ int pgld_binned_plain(const pgld_panel_desc* desc, const pgld_binned_args* args,
 int pgld_count_block(const pgld_panel_desc* desc, const int8_t* d_panels,
                      const int32_t* d_site_stats, int32_t pop, int64_t i0, int64_t ni,
                      int64_t j0, int64_t nj, int32_t* d_counts, void* stream);
+
+/* Dense per-pair LD values for the site rectangle [i0, i0+ni) x [j0, j0+nj)
+ * of one population (i0, j0 multiples of 32), from the MFMA counts:
+ * d_out float64 [2][ni][nj], plane 0 the value, plane 1 a 0/1 flag.
+ * PGLD_PAIR_SIGMA_D2: D^2-numerator / pi2-numerator where n >= 4 and the
+ *   latter is not 0 (else 0), flag = (n >= 4); replaces _tile_sigma_d2 /
+ *   _build_sigma_d2_matrix (pg_gpu/ld_statistics.py:311-341, :734-758).
+ * PGLD_PAIR_R2_NAIVE: frequency r^2 with per-site frequencies over each
+ *   site's valid samples, 0 where undefined, flag = defined; replaces
+ *   _tile_r2_naive and HaplotypeMatrix.pairwise_r2 (ld_statistics.py:301-309,
+ *   haplotype_matrix.py:1092-1170).
+ * PGLD_PAIR_R2_DOSAGE: r^2 of mean-centred dosages, missing calls zero,
+ *   0 where a site has no variance; replaces _r2_matrix_diploid
+ *   (ld_statistics.py:973-1021).
+ * These feed zns / omega (ld_statistics.py:534-862). */
+int pgld_pair_block(const pgld_panel_desc* desc, const int8_t* d_panels,
+                    const int32_t* d_site_stats, int32_t pop, int32_t kind, int64_t i0,
+                    int64_t ni, int64_t j0, int64_t nj, double* d_out, void* stream);
 
 /* Heterozygosity statistics H_i_j (i <= j, het_names order) summed over
  * sites from the pgld_scan_sites tallies, as fixed-point limbs

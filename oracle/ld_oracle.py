@@ -683,6 +683,111 @@ def window_sums(mat, windows, rows=None, mode="hap", exact=True):
     return sums, counts
 
 
+# ---------------------------------------------------------------------------
+# Dense tile consumers (ld_statistics.py:249-1021, haplotype_matrix.py:1092-1170)
+# ---------------------------------------------------------------------------
+
+
+def segregating(h, missing_data="include"):
+    """Columns kept by _prepare_segregating (ld_statistics.py:249-278)."""
+    h = np.asarray(h)
+    if missing_data == "exclude":
+        h = h[:, (h < 0).sum(axis=0) == 0]
+    valid = (h >= 0).sum(axis=0)
+    dac = np.maximum(h, 0).astype(np.int64).sum(axis=0)
+    return h[:, (dac > 0) & (dac < valid)]
+
+
+def _dense_counts(h):
+    hc = np.where(h >= 0, h, 0).astype(np.float64)
+    vm = (h >= 0).astype(np.float64)
+    c1 = hc.T @ hc
+    s12 = hc.T @ vm
+    s13 = vm.T @ hc
+    n = vm.T @ vm
+    return c1, s12 - c1, s13 - c1, n - s12 - s13 + c1, n, hc, vm
+
+
+def sigma_d2_matrix(h):
+    """(ratio, n >= 4 flag) matrices of _tile_sigma_d2 (ld_statistics.py:311-341),
+    diagonal zero."""
+    c1, c2, c3, c4, n, _, _ = _dense_counts(h)
+    dd = c1 * (c1 - 1) * c4 * (c4 - 1) + c2 * (c2 - 1) * c3 * (c3 - 1) - 2 * c1 * c2 * c3 * c4
+    s12, s13, s24, s34 = c1 + c2, c1 + c3, c2 + c4, c3 + c4
+    pi2 = (s12 * s13 * s24 * s34 - c1 * c4 * (-1 + c1 + 3 * c2 + 3 * c3 + c4)
+           - c2 * c3 * (-1 + 3 * c1 + c2 + c3 + 3 * c4))
+    valid = n >= 4
+    with np.errstate(divide="ignore", invalid="ignore"):
+        val = np.where(valid & (pi2 != 0), dd / np.where(pi2 != 0, pi2, 1.0), 0.0)
+    np.fill_diagonal(val, 0.0)
+    flag = valid.astype(np.float64)
+    np.fill_diagonal(flag, 0.0)
+    return val, flag
+
+
+def r2_naive_matrix(h):
+    """HaplotypeMatrix.pairwise_r2 / _tile_r2_naive (haplotype_matrix.py:1092-1170)."""
+    c1, _, _, _, n, hc, vm = _dense_counts(h)
+    nv = vm.sum(axis=0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        p = np.where(nv > 0, hc.sum(axis=0) / np.where(nv > 0, nv, 1), 0.0)
+        pab = np.where(n > 0, c1 / np.where(n > 0, n, 1), 0.0)
+        D = pab - np.outer(p, p)
+        den = np.outer(p * (1 - p), p * (1 - p))
+        r2 = np.where(den > 0, D ** 2 / np.where(den > 0, den, 1), 0.0)
+    np.fill_diagonal(r2, 0.0)
+    return r2
+
+
+def r2_dosage_matrix(g):
+    """_r2_matrix_diploid (ld_statistics.py:973-1021): per-site means over
+    valid individuals, missing entries zeroed after centring."""
+    g = np.asarray(g)
+    vm = (g >= 0).astype(np.float64)
+    gc = np.where(g >= 0, g, 0).astype(np.float64)
+    nv = vm.sum(axis=0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        mean = np.where(nv > 0, gc.sum(axis=0) / np.where(nv > 0, nv, 1), 0.0)
+    gn = (gc - mean[None, :]) * vm
+    var = (gn ** 2).sum(axis=0)
+    cov = gn.T @ gn
+    den = np.sqrt(np.outer(var, var))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r2 = np.where(den > 0, (cov / np.where(den > 0, den, 1)) ** 2, 0.0)
+    np.fill_diagonal(r2, 0.0)
+    return r2
+
+
+def zns_value(r2, flag=None):
+    """ZnS from a full r^2 (or sigma_D^2) matrix (ld_statistics.py:534-731)."""
+    m = r2.shape[0]
+    if m < 2:
+        return 0.0
+    if flag is not None:
+        n = flag.sum()
+        return float(r2.sum() / n) if n > 0 else 0.0
+    return float((r2.sum() - np.trace(r2)) / (m * (m - 1)))
+
+
+def omega_value(r2):
+    """Omega by direct summation over split points l = 3..m-2, upper-triangle
+    pairs (ld_statistics.py:761-862 computes the same with prefix sums)."""
+    m = r2.shape[0]
+    if m < 5:
+        return 0.0
+    u = np.triu(r2, 1)
+    best = 0.0
+    for l in range(3, m - 1):
+        left = u[:l, :l].sum()
+        right = u[l:, l:].sum()
+        cross = u[:l, l:].sum()
+        nw = l * (l - 1) // 2 + (m - l) * (m - l - 1) // 2
+        nc = l * (m - l)
+        if nw > 0 and nc > 0 and cross > 0:
+            best = max(best, ((left + right) / nw) / (cross / nc))
+    return float(best)
+
+
 def heterozygosity_hap(hap, pop_rows):
     """H_i_j sums over sites (moments_ld.py:346-388, haplotype branch).
 

@@ -16,6 +16,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libpgld.so"
 
 PGLD_OK, PGLD_EINVAL, PGLD_EDEVICE, PGLD_EUNSUPPORTED = 0, 1, 2, 3
 MODE_HAP, MODE_GENO = 0, 1
+PAIR_SIGMA_D2, PAIR_R2_NAIVE, PAIR_R2_DOSAGE = 0, 1, 2
 MAX_POPS = 4
 MAX_BINS = 64
 
@@ -106,6 +107,8 @@ _SIGS = {
     "pgld_geno_stat_set": (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_int64, c_void_p,
                                      c_void_p]),
     "pgld_sigma_d2_geno": (c_int32, [c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
+    "pgld_pair_block": (c_int32, [POINTER(PanelDesc), c_void_p, c_void_p, c_int32, c_int32,
+                                  c_int64, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
     "pgld_r_family": (c_int32, [c_void_p, c_int64, c_void_p, c_int64, c_int32, c_void_p,
                                 c_void_p]),
     "pgld_heterozygosity": (c_int32, [POINTER(PanelDesc), c_void_p, c_void_p, c_void_p,

@@ -225,6 +225,12 @@ class HaplotypeMatrix:
             out.append(((float(bins[b]), float(bins[b + 1])), vals))
         return out
 
+    def pairwise_r2(self, estimator="r2"):
+        """Pairwise r^2 of all variant pairs (haplotype_matrix.py:1138-1170)."""
+        from .ld_tiles import pairwise_r2
+
+        return pairwise_r2(self, estimator)
+
     def compute_ld_statistics_gpu_single_pop(self, bp_bins, raw=False, ac_filter=True,
                                              chunk_size="auto"):
         """{(lo, hi): (DD, Dz, pi2)} over all haplotypes; sums if ``raw``,

@@ -170,3 +170,8 @@ def compute_ld_statistics(counts, statistics=("dd", "dz", "pi2"), populations=No
         else:
             raise ValueError(f"Unknown statistic: {s}")
     return out
+
+
+# Dense tile consumers (ld_statistics.py:534-1021), built on pgld_pair_block.
+from .ld_tiles import omega, r2_matrix_diploid, zns  # noqa: E402,F401
+

@@ -257,3 +257,54 @@ def test_names_and_spec_counts():
     for p in range(1, 5):
         for _, calls in o.stat_specs(p):
             assert abs(sum(w for w, _, _ in calls) - 1.0) < 1e-15
+
+
+# ---------------------------------------------------------------- tile consumers
+
+def test_sigma_d2_matrix_is_ratio_of_unbiased_estimators():
+    """sigma_D^2 of the tile consumers is DD / pi2 of the pinned single-pop
+    estimators (their common 1/(n)_4 cancels)."""
+    rng = np.random.default_rng(70)
+    h = (rng.random((30, 12)) < 0.45).astype(np.int8)
+    h[rng.random(h.shape) < 0.05] = -1
+    val, flag = o.sigma_d2_matrix(h)
+    ii, jj = np.triu_indices(12, 1)
+    c, nv = o.hap_counts(h, ii, jj)
+    P = [o.HapPop(c, nv)]
+    dd, pi2 = o.hap_dd(P, 0, 0), o.hap_pi2(P, 0, 0, 0, 0)
+    ok = (P[0].n >= 4) & (pi2 != 0)
+    np.testing.assert_allclose(val[ii, jj][ok], (dd / np.where(ok, pi2, 1))[ok], rtol=1e-12)
+    np.testing.assert_array_equal(flag[ii, jj], (P[0].n >= 4).astype(float))
+
+
+def test_r2_naive_matrix_brute_force():
+    rng = np.random.default_rng(71)
+    h = (rng.random((25, 9)) < 0.5).astype(np.int8)
+    h[rng.random(h.shape) < 0.1] = -1
+    got = o.r2_naive_matrix(h)
+    for i in range(9):
+        for j in range(9):
+            if i == j:
+                assert got[i, j] == 0
+                continue
+            vi, vj = h[:, i] >= 0, h[:, j] >= 0
+            pi, pj = h[vi, i].mean(), h[vj, j].mean()
+            both = vi & vj
+            pab = np.mean((h[both, i] == 1) & (h[both, j] == 1)) if both.any() else 0.0
+            den = pi * (1 - pi) * pj * (1 - pj)
+            ref = (pab - pi * pj) ** 2 / den if den > 0 else 0.0
+            assert abs(got[i, j] - ref) < 1e-12
+
+
+def test_omega_prefix_sums_match_direct_sums():
+    """The prefix-sum omega of the build (ld_tiles.omega) restates the same
+    formula as the oracle's direct per-split sums; check the oracle's
+    split-point bookkeeping on a matrix with a known block structure."""
+    m = 8
+    r2 = np.full((m, m), 0.1)
+    r2[:4, :4] = 0.9
+    r2[4:, 4:] = 0.9
+    np.fill_diagonal(r2, 0)
+    # at l = 4 within-mean 0.9, cross-mean 0.1 -> 9; other splits mix blocks
+    assert abs(o.omega_value(r2) - 9.0) < 1e-12
+    assert o.zns_value(r2) == pytest.approx((0.9 * 24 + 0.1 * 32) / 56)
