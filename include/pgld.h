@@ -45,6 +45,7 @@ enum { PGLD_MODE_HAP = 0, PGLD_MODE_GENO = 1 };
 #define PGLD_PAIR_SIGMA_D2 0   /* haplotypes: unbiased D^2 / pi^2 ratio */
 #define PGLD_PAIR_R2_NAIVE 1   /* haplotypes: frequency r^2 (per-site frequencies) */
 #define PGLD_PAIR_R2_DOSAGE 2  /* genotypes: r^2 of mean-centred dosages */
+#define PGLD_PAIR_R2 3         /* haplotypes: r^2 of the pair's counts (NaN if undefined) */
 
 /* Library version (major*10000 + minor*100 + patch). */
 int pgld_version(void);
@@ -199,6 +200,10 @@ int pgld_count_block(const pgld_panel_desc* desc, const int8_t* d_panels,
  * PGLD_PAIR_R2_DOSAGE: r^2 of mean-centred dosages, missing calls zero,
  *   0 where a site has no variance; replaces _r2_matrix_diploid
  *   (ld_statistics.py:973-1021).
+ * PGLD_PAIR_R2: r^2 from the pair's own counts (samples valid at both
+ *   sites), NaN where undefined, flag = defined; replaces
+ *   tally_gpu_haplotypes + ld_statistics.r_squared in windowed_r_squared
+ *   (haplotype_matrix.py:1235-1317).
  * These feed zns / omega (ld_statistics.py:534-862). */
 int pgld_pair_block(const pgld_panel_desc* desc, const int8_t* d_panels,
                     const int32_t* d_site_stats, int32_t pop, int32_t kind, int64_t i0,

@@ -838,3 +838,35 @@ class _FracPop:
 def exact_hap_call(counts_per_pop, stat_type, pops):
     """Rational value of one raw call; same formulas as hap_call, in Fractions."""
     return hap_call([_FracPop(c) for c in counts_per_pop], stat_type, pops)
+
+
+def windowed_r_squared(hap, positions, bp_bins, percentile=50, rows=None, estimator="r2"):
+    """Percentiles of r^2 per distance bin over all i < j pairs
+    (haplotype_matrix.py:1235-1317): 'r2' from each pair's counts,
+    'rogers_huff' from the dosages of adjacent haplotype pairs (NaN where a
+    site is monomorphic, ld_statistics.py:446-466)."""
+    h = np.asarray(hap)
+    if rows is not None:
+        h = h[np.asarray(rows)]
+    m = h.shape[1]
+    ii, jj = np.triu_indices(m, 1)
+    if estimator == "rogers_huff":
+        g = h[0::2].astype(np.int64) + h[1::2]
+        poly = g.var(axis=0) > 0
+        r2 = np.where(poly[ii] & poly[jj], r2_dosage_matrix(g)[ii, jj], np.nan)
+    else:
+        c, nv = hap_counts(h, ii, jj)
+        r2 = r_family(c, nv)[1]
+    pos = np.asarray(positions, np.float64)
+    b = np.digitize(pos[jj] - pos[ii], np.asarray(bp_bins, np.float64)) - 1
+    nb = len(bp_bins) - 1
+    ok = (b >= 0) & (b < nb) & ~np.isnan(r2)
+    pct = np.atleast_1d(percentile)
+    res = np.full((nb, len(pct)), np.nan)
+    cnt = np.zeros(nb, dtype=int)
+    for k in range(nb):
+        sel = r2[ok & (b == k)]
+        cnt[k] = len(sel)
+        if len(sel):
+            res[k] = np.percentile(sel, pct)
+    return (res[:, 0] if res.shape[1] == 1 else res), cnt

@@ -16,7 +16,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libpgld.so"
 
 PGLD_OK, PGLD_EINVAL, PGLD_EDEVICE, PGLD_EUNSUPPORTED = 0, 1, 2, 3
 MODE_HAP, MODE_GENO = 0, 1
-PAIR_SIGMA_D2, PAIR_R2_NAIVE, PAIR_R2_DOSAGE = 0, 1, 2
+PAIR_SIGMA_D2, PAIR_R2_NAIVE, PAIR_R2_DOSAGE, PAIR_R2 = 0, 1, 2, 3
 MAX_POPS = 4
 MAX_BINS = 64
 

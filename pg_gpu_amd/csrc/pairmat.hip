@@ -15,6 +15,11 @@
 //                       valid at both, 0 where a site is monomorphic
 //   PGLD_PAIR_R2_DOSAGE r^2 of mean-centred dosages with missing calls set to
 //                       zero, as _r2_matrix_diploid (ld_statistics.py:973-1021)
+//   PGLD_PAIR_R2        r^2 of haplotypes from the pair's own counts over the
+//                       samples valid at both sites, as ld_statistics.r_squared
+//                       on tally_gpu_haplotypes counts (ld_statistics.py:
+//                       190-217, haplotype_matrix.py:1235-1317): NaN where a
+//                       site is monomorphic among those samples
 #include "pgld_internal.h"
 
 namespace pgld {
@@ -71,6 +76,11 @@ __global__ void __launch_bounds__(64) pair_block_hap_kernel(PairBlockArgs a) {
       const bool valid = n >= 4.0;
       v = (valid && pi2 != 0.0) ? dd / pi2 : 0.0;
       ok = valid ? 1.0 : 0.0;
+    } else if (a.kind == PGLD_PAIR_R2) {
+      const double den = (c1 + c2) * (c3 + c4) * (c1 + c3) * (c2 + c4);
+      const double num = c1 * c4 - c2 * c3;
+      v = den > 0.0 ? num * num / den : __builtin_nan("");
+      ok = den > 0.0 ? 1.0 : 0.0;
     } else {  // PGLD_PAIR_R2_NAIVE
       const double pi = ni_v > 0 ? (double)ci / (double)ni_v : 0.0;
       const double pj = nj_v > 0 ? (double)cj / (double)nj_v : 0.0;
@@ -154,7 +164,8 @@ extern "C" int pgld_pair_block(const pgld_panel_desc* desc, const int8_t* d_pane
   if (i0 % 32 || j0 % 32 || i0 < 0 || j0 < 0 || ni < 0 || nj < 0)
     return fail(PGLD_EINVAL, "i0/j0 must be non-negative multiples of 32");
   if (i0 + ni > L.n_sites || j0 + nj > L.n_sites) return fail(PGLD_EINVAL, "block out of range");
-  const bool hap_kind = kind == PGLD_PAIR_SIGMA_D2 || kind == PGLD_PAIR_R2_NAIVE;
+  const bool hap_kind = kind == PGLD_PAIR_SIGMA_D2 || kind == PGLD_PAIR_R2_NAIVE ||
+                        kind == PGLD_PAIR_R2;
   if (!hap_kind && kind != PGLD_PAIR_R2_DOSAGE) return fail(PGLD_EINVAL, "unknown pair kind");
   if (hap_kind != (desc->mode == PGLD_MODE_HAP))
     return fail(PGLD_EINVAL, "pair kind does not match the panel mode");

@@ -231,6 +231,12 @@ class HaplotypeMatrix:
 
         return pairwise_r2(self, estimator)
 
+    def windowed_r_squared(self, bp_bins, percentile=50, pop=None, estimator="r2"):
+        """Percentiles of r^2 in distance bins (haplotype_matrix.py:1235-1317)."""
+        from .ld_tiles import windowed_r_squared
+
+        return windowed_r_squared(self, bp_bins, percentile, pop, estimator)
+
     def compute_ld_statistics_gpu_single_pop(self, bp_bins, raw=False, ac_filter=True,
                                              chunk_size="auto"):
         """{(lo, hi): (DD, Dz, pi2)} over all haplotypes; sums if ``raw``,

@@ -45,6 +45,60 @@ def _pair_matrix(data, kind, mode):
     return val, flag
 
 
+def windowed_r_squared(hap_matrix, bp_bins, percentile=50, pop=None, estimator="r2"):
+    """Percentiles of r^2 in genomic-distance bins over every variant pair
+    (haplotype_matrix.py:1235-1317).  'r2': r^2 from each pair's counts over
+    the samples valid at both sites (of ``pop`` if given); 'rogers_huff':
+    r^2 of the dosages of adjacent haplotype pairs.  Returns (result
+    [n_bins] or [n_bins, n_percentiles], pair counts per bin); pairs with
+    undefined r^2 are left out, as in the reference."""
+    import numpy as np
+
+    t = dv.torch()
+    h = _device_data(hap_matrix, True)
+    if estimator == "rogers_huff":
+        if pop is not None:
+            raise NotImplementedError(
+                "windowed_r_squared(estimator='rogers_huff', pop=...) is not yet implemented; "
+                "pass the full matrix or subset to the desired haplotypes first.")
+        if bool((h < 0).any()):
+            raise ValueError("rogers_huff_r does not support missing values (-1)")
+        if h.shape[0] % 2:
+            raise ValueError("Need even number of haplotypes for diploid conversion")
+        g = (h[0::2].to(t.int16) + h[1::2]).to(t.int8)
+        r2, flag = _pair_matrix(g, _lib.PAIR_R2_DOSAGE, "geno")
+        r2 = t.where(flag > 0, r2, t.full_like(r2, float("nan")))  # monomorphic: r undefined
+    elif estimator == "r2":
+        if pop is not None:
+            rows = hap_matrix.sample_sets[pop] if isinstance(pop, str) else pop
+            h = h[dv.as_tensor(list(rows), t.int64, h.device)]
+        r2, _ = _pair_matrix(h, _lib.PAIR_R2, "hap")
+    else:
+        raise ValueError(f"Unknown estimator: {estimator!r} (expected 'r2' or 'rogers_huff')")
+    m = r2.shape[0]
+    ii, jj = t.triu_indices(m, m, offset=1, device=r2.device)
+    vals = r2[ii, jj]
+    pos = dv.as_tensor(hap_matrix.positions, t.float64, r2.device)
+    dist = pos[jj] - pos[ii]
+    edges = t.as_tensor(np.asarray(bp_bins, np.float64), device=r2.device)
+    b = t.bucketize(dist, edges, right=True) - 1          # np.digitize(d, bins) - 1
+    nb = len(bp_bins) - 1
+    ok = (b >= 0) & (b < nb) & ~vals.isnan()
+    v_cpu = vals[ok].cpu().numpy()
+    b_cpu = b[ok].cpu().numpy()
+    pct = np.atleast_1d(percentile)
+    result = np.full((nb, len(pct)), np.nan)
+    counts = np.zeros(nb, dtype=int)
+    for k in range(nb):
+        sel = v_cpu[b_cpu == k]
+        counts[k] = len(sel)
+        if len(sel):
+            result[k] = np.percentile(sel, pct)
+    if result.shape[1] == 1:
+        result = result[:, 0]
+    return result, counts
+
+
 def _is_hap(obj):
     from .haplotype_matrix import HaplotypeMatrix
 

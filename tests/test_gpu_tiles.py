@@ -83,3 +83,36 @@ def test_pairwise_r2_method(cuda):
     rh = _hm(h).pairwise_r2("rogers_huff").cpu().numpy()
     g = (h[0::2].astype(np.int16) + h[1::2]).astype(np.int8)
     np.testing.assert_allclose(rh, o.r2_dosage_matrix(g), rtol=1e-10, atol=1e-14)
+
+
+@pytest.mark.parametrize("missing", [0.0, 0.05])
+def test_windowed_r_squared(cuda, missing):
+    from pg_gpu_amd.haplotype_matrix import HaplotypeMatrix
+
+    h = _haps(83, 50, 120, missing)
+    pos = np.sort(np.random.default_rng(5).choice(np.arange(1, 20_000), 120, replace=False))
+    hm = HaplotypeMatrix(h, pos)
+    hm.sample_sets = {"a": list(range(0, 30)), "b": list(range(30, 50))}
+    bins = [0, 500, 2_000, 8_000, 30_000]
+    got, cnt = hm.windowed_r_squared(bins, percentile=[10, 50, 90])
+    ref, rcnt = o.windowed_r_squared(h, pos, bins, [10, 50, 90])
+    np.testing.assert_array_equal(cnt, rcnt)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-15)
+    got_a, cnt_a = hm.windowed_r_squared(bins, pop="a")
+    ref_a, rcnt_a = o.windowed_r_squared(h, pos, bins, 50, rows=list(range(30)))
+    np.testing.assert_array_equal(cnt_a, rcnt_a)
+    np.testing.assert_allclose(got_a, ref_a, rtol=1e-12, atol=1e-15)
+
+
+def test_windowed_r_squared_rogers_huff(cuda):
+    from pg_gpu_amd.haplotype_matrix import HaplotypeMatrix
+
+    h = _haps(84, 40, 100, 0.0)
+    h[:, 7] = 0  # a monomorphic site: its pairs are left out
+    pos = np.arange(100) * 37 + 5
+    hm = HaplotypeMatrix(h, pos)
+    bins = [0, 300, 1_000, 4_000]
+    got, cnt = hm.windowed_r_squared(bins, percentile=[25, 75], estimator="rogers_huff")
+    ref, rcnt = o.windowed_r_squared(h, pos, bins, [25, 75], estimator="rogers_huff")
+    np.testing.assert_array_equal(cnt, rcnt)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-15)

@@ -308,3 +308,39 @@ def test_omega_prefix_sums_match_direct_sums():
     # at l = 4 within-mean 0.9, cross-mean 0.1 -> 9; other splits mix blocks
     assert abs(o.omega_value(r2) - 9.0) < 1e-12
     assert o.zns_value(r2) == pytest.approx((0.9 * 24 + 0.1 * 32) / 56)
+
+
+def test_windowed_r_squared_matches_pearson_brute_force():
+    """Count r^2 over the samples valid at both sites is the squared Pearson
+    correlation of the two 0/1 columns there (np.corrcoef, pair by pair)."""
+    rng = np.random.default_rng(11)
+    h = (rng.random((16, 14)) < 0.35).astype(np.int8)
+    h[rng.random(h.shape) < 0.1] = -1
+    h[:, 2] = 0
+    pos = np.sort(rng.choice(np.arange(1, 500), 14, replace=False))
+    bins = [0, 60, 200, 600]
+    vals = {k: [] for k in range(3)}
+    for i in range(14):
+        for j in range(i + 1, 14):
+            ok = (h[:, i] >= 0) & (h[:, j] >= 0)
+            x, y = h[ok, i].astype(float), h[ok, j].astype(float)
+            if x.std() == 0 or y.std() == 0:
+                continue
+            k = np.digitize(pos[j] - pos[i], bins) - 1
+            if 0 <= k < 3:
+                vals[k].append(np.corrcoef(x, y)[0, 1] ** 2)
+    got, cnt = o.windowed_r_squared(h, pos, bins, [20, 50, 100])
+    for k in range(3):
+        assert cnt[k] == len(vals[k])
+        if vals[k]:
+            np.testing.assert_allclose(got[k], np.percentile(vals[k], [20, 50, 100]), rtol=1e-12)
+    h2 = np.where(h < 0, 0, h)
+    got, cnt = o.windowed_r_squared(h2, pos, bins, 50, estimator="rogers_huff")
+    g = h2[0::2].astype(float) + h2[1::2]
+    for k in range(3):
+        v = [np.corrcoef(g[:, i], g[:, j])[0, 1] ** 2 for i in range(14) for j in range(i + 1, 14)
+             if g[:, i].std() > 0 and g[:, j].std() > 0
+             and np.digitize(pos[j] - pos[i], bins) - 1 == k]
+        assert cnt[k] == len(v)
+        if v:
+            np.testing.assert_allclose(got[k], np.percentile(v, 50), rtol=1e-12)
