@@ -17,7 +17,25 @@ struct HapPop {
   double D;                // c2*c3 - c1*c4 (reference sign)
   double pA, qA, pB, qB;   // allele counts at locus A (left) and B (right)
   double r1, r2, r3, r4;   // 1/(n)_1 .. 1/(n)_4, 0 when not positive
+  // shared products of the estimators below (integers)
+  double x, y;             // c1*c4, c2*c3
+  double s14, s23;         // c1 + c4, c2 + c3
+  double uA, uB;           // qA - pA, qB - pB
+  double PA, PB;           // pA*qA, pB*qB
+  double w;                // uA*uB + s23 - s14
 };
+
+// The shared products of a population's counts (c1..c4 and the r's set).
+__device__ __forceinline__ void hap_basis(HapPop& p) {
+  p.x = p.c1 * p.c4; p.y = p.c2 * p.c3;
+  p.D = p.y - p.x;
+  p.pA = p.c1 + p.c2; p.qA = p.c3 + p.c4;
+  p.pB = p.c1 + p.c3; p.qB = p.c2 + p.c4;
+  p.s14 = p.c1 + p.c4; p.s23 = p.c2 + p.c3;
+  p.uA = p.qA - p.pA; p.uB = p.qB - p.pB;
+  p.PA = p.pA * p.qA; p.PB = p.pB * p.qB;
+  p.w = fma(p.uA, p.uB, p.s23 - p.s14);
+}
 
 // rtab holds 4 doubles per n: 1/(n)_1, 1/(n)_2, 1/(n)_3, 1/(n)_4.
 __device__ __forceinline__ HapPop hap_pop(int n11, int n10, int n01, int n00, int n,
@@ -25,20 +43,22 @@ __device__ __forceinline__ HapPop hap_pop(int n11, int n10, int n01, int n00, in
   HapPop p;
   p.c1 = pgld_count_f64(n11); p.c2 = pgld_count_f64(n10);
   p.c3 = pgld_count_f64(n01); p.c4 = pgld_count_f64(n00);
-  p.D = p.c2 * p.c3 - p.c1 * p.c4;
-  p.pA = p.c1 + p.c2; p.qA = p.c3 + p.c4;
-  p.pB = p.c1 + p.c3; p.qB = p.c2 + p.c4;
+  hap_basis(p);
   const double2* t = reinterpret_cast<const double2*>(rtab + 4 * n);
   const double2 a = t[0], b = t[1];
   p.r1 = a.x; p.r2 = a.y; p.r3 = b.x; p.r4 = b.y;
   return p;
 }
 
+// The numerators below are the reference's polynomials regrouped over the
+// shared products (identities in the counts, e.g.
+// c1(c1-1)c4(c4-1) + c2(c2-1)c3(c3-1) - 2c1c2c3c4 = D^2 - x(s14-1) - y(s23-1));
+// every intermediate is an integer, so they equal the reference's numerators
+// exactly whatever the grouping.
+
 // ---- DD -------------------------------------------------------------------
 __device__ __forceinline__ double hap_dd_same(const HapPop& a) {
-  const double num = a.c1 * (a.c1 - 1.0) * a.c4 * (a.c4 - 1.0) +
-                     a.c2 * (a.c2 - 1.0) * a.c3 * (a.c3 - 1.0) -
-                     2.0 * a.c1 * a.c2 * a.c3 * a.c4;
+  const double num = fma(a.D, a.D, -fma(a.x, a.s14 - 1.0, a.y * (a.s23 - 1.0)));
   return num * a.r4;
 }
 __device__ __forceinline__ double hap_dd_pair(const HapPop& a, const HapPop& b) {
@@ -46,75 +66,74 @@ __device__ __forceinline__ double hap_dd_pair(const HapPop& a, const HapPop& b) 
 }
 
 // ---- Dz -------------------------------------------------------------------
+// (c1c4 - c2c3)((qA-pA)(qB-pB) + (c2+c3) - (c1+c4)) + 2(c2c3 + c1c4)
 __device__ __forceinline__ double hap_dz_same(const HapPop& a) {
-  const double x = a.c1 * a.c4 - a.c2 * a.c3;
-  const double num = x * (a.qA - a.pA) * (a.qB - a.pB) +
-                     x * ((a.c2 + a.c3) - (a.c1 + a.c4)) + 2.0 * (a.c2 * a.c3 + a.c1 * a.c4);
+  const double num = fma(-a.D, a.w, 2.0 * (a.x + a.y));
   return num * a.r4;
 }
 // Dz(i,i,k)
 __device__ __forceinline__ double hap_dz_iik(const HapPop& a, const HapPop& b) {
-  const double num = (a.qA - a.pA) * (-a.D) * (b.qB - b.pB);
+  const double num = (a.uA * (-a.D)) * b.uB;
   return num * (a.r3 * b.r1);
 }
 // Dz(i,j,i)
 __device__ __forceinline__ double hap_dz_iji(const HapPop& a, const HapPop& b) {
-  const double num = (a.qB - a.pB) * (-a.D) * (b.qA - b.pA);
+  const double num = (a.uB * (-a.D)) * b.uA;
   return num * (a.r3 * b.r1);
 }
 // Dz(i,j,j)
 __device__ __forceinline__ double hap_dz_ijj(const HapPop& a, const HapPop& b) {
-  const double t = (b.c2 + b.c3) - (b.c1 + b.c4);
-  const double num = (-a.D) * (t + (b.qB - b.pB) * (b.qA - b.pA));
-  return num * (a.r2 * b.r2);
+  return ((-a.D) * b.w) * (a.r2 * b.r2);
 }
 // Dz(i,j,k), all distinct
 __device__ __forceinline__ double hap_dz_ijk(const HapPop& a, const HapPop& b, const HapPop& c) {
-  const double num = -(a.D * (b.pA - b.qA) * (c.pB - c.qB));
+  const double num = -(a.D * b.uA * c.uB);
   return num * (a.r2 * b.r1 * c.r1);
 }
 
 // ---- pi2 ------------------------------------------------------------------
+// pA pB qA qB - c1c4 (c1 + 3c2 + 3c3 + c4 - 1) - c2c3 (3c1 + c2 + c3 + 3c4 - 1)
 __device__ __forceinline__ double hap_pi2_same(const HapPop& a) {
-  const double num = a.pA * a.pB * a.qB * a.qA -
-                     a.c1 * a.c4 * (-1.0 + a.c1 + 3.0 * a.c2 + 3.0 * a.c3 + a.c4) -
-                     a.c2 * a.c3 * (-1.0 + 3.0 * a.c1 + a.c2 + a.c3 + 3.0 * a.c4);
+  const double u = fma(3.0, a.s23, a.s14) - 1.0, v = fma(3.0, a.s14, a.s23) - 1.0;
+  const double num = fma(a.PA, a.PB, -fma(a.x, u, a.y * v));
   return num * a.r4;
 }
-// pop t three times, pop s once (all "triple" cases share this formula)
+// pop t three times, pop s once (all "triple" cases share this formula):
+// (s.pB (PA qB - m) + s.qB (PA pB + m - 2 PA)) / 2 with PA, qB, pB, m of t and
+// m = pA c4 + c2 qA
 __device__ __forceinline__ double hap_pi2_triple(const HapPop& t, const HapPop& s) {
-  const double tpA = t.pA, tqA = t.qA;
-  const double num = (s.pB * (tpA * (t.c2 + t.c4) * tqA - tpA * t.c4 - t.c2 * tqA) +
-                      s.qB * (tpA * (t.c1 + t.c3) * tqA + tpA * t.c4 + t.c2 * tqA) -
-                      2.0 * tpA * tqA * s.qB) * 0.5;
+  const double m = fma(t.pA, t.c4, t.c2 * t.qA);
+  const double u = fma(t.PA, t.qB, -m);
+  const double v = fma(-2.0, t.PA, fma(t.PA, t.pB, m));
+  const double num = fma(s.qB, v, s.pB * u) * 0.5;
   return num * (s.r1 * t.r3);
 }
 // pi2(i,i,k,k)
 __device__ __forceinline__ double hap_pi2_iikk(const HapPop& a, const HapPop& b) {
-  return (a.pA * a.qA) * (b.pB * b.qB) * (a.r2 * b.r2);
+  return (a.PA * b.PB) * (a.r2 * b.r2);
 }
 // pi2(i,i,k,l), k != l
 __device__ __forceinline__ double hap_pi2_iikl(const HapPop& a, const HapPop& b, const HapPop& c) {
-  return (a.pA * a.qA) * (b.pB * c.qB + b.qB * c.pB) * 0.5 * (a.r2 * b.r1 * c.r1);
+  return a.PA * fma(b.pB, c.qB, b.qB * c.pB) * 0.5 * (a.r2 * b.r1 * c.r1);
 }
 // pi2(i,j,k,k), i != j
 __device__ __forceinline__ double hap_pi2_ijkk(const HapPop& a, const HapPop& b, const HapPop& c) {
-  return (c.pB * c.qB) * (a.pA * b.qA + a.qA * b.pA) * 0.5 * (c.r2 * a.r1 * b.r1);
+  return c.PB * fma(a.pA, b.qA, a.qA * b.pA) * 0.5 * (c.r2 * a.r1 * b.r1);
 }
-// pi2(i,j,i,j) and pi2(i,j,j,i)
+// pi2(i,j,i,j) and pi2(i,j,j,i).  The reference's quartic and cubic
+// polynomials in the counts a1..a4 of A and b1..b4 of B regroup as
+//   quartic = (A.qA B.pA + A.pA B.qA) (A.qB B.pB + A.pB B.qB)
+//   cubic   = (b1 a4 + b2 a3 + b3 a2 + b4 a1)
+//           - (b1 A.qA A.qB + b2 A.pB A.qA + b3 A.pA A.qB + b4 A.pA A.pB)
+//           - (a1 B.qA B.qB + a2 B.pB B.qA + a3 B.pA B.qB + a4 B.pA B.pB)
 __device__ __forceinline__ double hap_pi2_ijij(const HapPop& A, const HapPop& B) {
-  const double a1 = A.c1, a2 = A.c2, a3 = A.c3, a4 = A.c4;
-  const double b1 = B.c1, b2 = B.c2, b3 = B.c3, b4 = B.c4;
-  const double quartic = A.qB * A.qA * B.pA * B.pB + A.pB * A.qA * B.pA * B.qB +
-                         A.pA * A.qB * B.pB * B.qA + A.pA * A.pB * B.qB * B.qA;
-  const double cubic =
-      b1 * (a4 - a2 * a3 - a2 * a4 - a3 * a4 - a4 * a4 - a4 * b1) +
-      b2 * (a3 - a1 * a3 - a3 * a3 - a1 * a4 - a3 * a4 - a3 * b1 - a4 * b1 - a3 * b2) +
-      b3 * (a2 - a1 * a2 - a2 * a2 - a1 * a4 - a2 * a4 - a2 * b1 - a4 * b1 - a1 * b2 -
-            a4 * b2 - a2 * b3) +
-      b4 * (a1 - a1 * a1 - a1 * a2 - a1 * a3 - a2 * a3 - a2 * b1 - a3 * b1 - a1 * b2 -
-            a3 * b2 - a1 * b3 - a2 * b3 - a1 * b4);
-  return (quartic + cubic) * 0.25 * (A.r2 * B.r2);
+  const double quartic = fma(A.qA, B.pA, A.pA * B.qA) * fma(A.qB, B.pB, A.pB * B.qB);
+  const double lin = fma(B.c1, A.c4, fma(B.c2, A.c3, fma(B.c3, A.c2, B.c4 * A.c1)));
+  const double ta = fma(B.c1, A.qA * A.qB, fma(B.c2, A.pB * A.qA,
+                    fma(B.c3, A.pA * A.qB, B.c4 * (A.pA * A.pB))));
+  const double tb = fma(A.c1, B.qA * B.qB, fma(A.c2, B.pB * B.qA,
+                    fma(A.c3, B.pA * B.qB, A.c4 * (B.pA * B.pB))));
+  return ((quartic + lin) - (ta + tb)) * 0.25 * (A.r2 * B.r2);
 }
 // one pop (S) shared between the two locus pairs, A/B the other two pops
 __device__ __forceinline__ double hap_pi2_shared(const HapPop& S, const HapPop& A, const HapPop& B) {
@@ -131,7 +150,7 @@ __device__ __forceinline__ double hap_pi2_shared(const HapPop& S, const HapPop& 
 // pi2(i,j,k,l), all distinct
 __device__ __forceinline__ double hap_pi2_alldiff(const HapPop& a, const HapPop& b,
                                                   const HapPop& c, const HapPop& d) {
-  return (a.pA * b.qA + a.qA * b.pA) * (c.pB * d.qB + c.qB * d.pB) * 0.25 *
+  return fma(a.pA, b.qA, a.qA * b.pA) * fma(c.pB, d.qB, c.qB * d.pB) * 0.25 *
          (a.r1 * b.r1 * c.r1 * d.r1);
 }
 
@@ -142,8 +161,8 @@ __device__ __forceinline__ double hap_pi2_alldiff(const HapPop& a, const HapPop&
 // the denominator is 0.  x[0] = r^2 (0 when undefined), x[1] = 1 when defined.
 // The ratio is formed in FP32 (the numerator and denominator are exact).
 __device__ __forceinline__ void hap_r2(const HapPop& a, double* x) {
-  const double num = a.c1 * a.c4 - a.c2 * a.c3;
-  const double den = (a.pA * a.qA) * (a.pB * a.qB);
+  const double num = -a.D;
+  const double den = a.PA * a.PB;
   const float nf = (float)num;
   const float r2 = nf * nf * __builtin_amdgcn_rcpf((float)den);
   const bool ok = den > 0.0;

@@ -162,8 +162,7 @@ __device__ __forceinline__ HapPop hap_pop_div(double c1, double c2, double c3, d
                                               double n) {
   HapPop p;
   p.c1 = c1; p.c2 = c2; p.c3 = c3; p.c4 = c4;
-  p.D = c2 * c3 - c1 * c4;
-  p.pA = c1 + c2; p.qA = c3 + c4; p.pB = c1 + c3; p.qB = c2 + c4;
+  hap_basis(p);
   const double f1 = n, f2 = f1 * (n - 1.0), f3 = f2 * (n - 2.0), f4 = f3 * (n - 3.0);
   p.r1 = f1 > 0.0 ? 1.0 / f1 : 0.0;
   p.r2 = f2 > 0.0 ? 1.0 / f2 : 0.0;
