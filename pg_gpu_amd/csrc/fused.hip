@@ -397,11 +397,9 @@ struct GenoTraits {
         }
         g[8] = c.w[p][4][r];
       }
-      int n = 0;
-#pragma unroll
-      for (int m = 0; m < 9; ++m) n += g[m];
-      pp[p] = geno_pop(g, 0, rt_entry(a, srt, p, n));
-      pp[p].n = n;
+      int y[9];
+      geno_centered(g, y);  // y[0] = valid individuals
+      pp[p] = geno_pop(g, y, 0, rt_entry(a, srt, p, y[0]));
     }
     if constexpr (G < NG) geno_stats_group<P, G>(pp, x);
     else geno_r2(pp[0], x);

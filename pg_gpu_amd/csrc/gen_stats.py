@@ -12,7 +12,6 @@ Run:  python pg_gpu_amd/csrc/gen_stats.py   (writes stats_hap.inc beside it)
 """
 from __future__ import annotations
 
-import json
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
@@ -209,53 +208,6 @@ def geno_key(kind, pp):
     return ("geno_pi2_alldiff", (min(i, j), max(i, j), min(k, l), max(k, l)))
 
 
-def nested_poly(table, var="g"):
-    """C expression for sum(coef * prod var[m]^e_m), nested by the lowest
-    variable index (Horner in several variables).  With integer coefficients
-    and integer variables every intermediate is an integer, so the FP64
-    evaluation is exact while the magnitudes stay below 2^53."""
-    monos = []
-    for exps, coef in table:
-        c = int(coef)
-        assert str(c) == coef, "sub-polynomials have integer coefficients"
-        idx = tuple(m for m, e in enumerate(exps) for _ in range(e))
-        monos.append((idx, c))
-
-    def emit(ms):
-        const = sum(c for idx, c in ms if not idx)
-        groups = {}
-        for idx, c in ms:
-            if idx:
-                groups.setdefault(idx[0], []).append((idx[1:], c))
-        terms = []
-        for m in sorted(groups):
-            inner = emit(groups[m])
-            terms.append(f"{var}[{m}] * ({inner})" if inner not in ("1.0",) else f"{var}[{m}]")
-        if const:
-            terms.append(f"{float(const)!r}")
-        if not terms:
-            return "0.0"
-        return " + ".join(terms)
-
-    return emit(monos)
-
-
-def emit_geno_polys():
-    data = json.loads((HERE / "geno_kernel_polys.json").read_text())
-    out = ["// Generated by gen_stats.py from geno_kernel_polys.json -- do not edit by hand.",
-           "// Integer sub-polynomials of the single-population genotype estimators",
-           "// (pg_gpu/genotype_kernels.py:27-71 poly, :235-272 quartic, :364-405 corr),",
-           "// in moments class order g[0] = n22 ... g[8] = n00.",
-           "#pragma once", ""]
-    for name, fn in (("dd_poly", "geno_dd_poly"), ("dz_quartic", "geno_dz_quartic"),
-                     ("pi2_corr", "geno_pi2_corr")):
-        out.append(f"__device__ __forceinline__ double {fn}(const double* g) {{")
-        out.append(f"  return {nested_poly(data[name])};")
-        out.append("}")
-        out.append("")
-    (HERE / "geno_polys.inc").write_text("\n".join(out))
-
-
 def main():
     parts = ["// Generated by gen_stats.py -- do not edit by hand.",
              "#pragma once",
@@ -267,7 +219,6 @@ def main():
         parts.append(emit_set("hap", "HapPop", P, hap_key))
         parts.append("")
     (HERE / "stats_hap.inc").write_text("\n".join(parts))
-    emit_geno_polys()
     parts = ["// Generated by gen_stats.py -- do not edit by hand.",
              "#pragma once",
              '#include "geno_estimators.h"', "",

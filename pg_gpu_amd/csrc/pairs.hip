@@ -262,6 +262,9 @@ __device__ __forceinline__ GenoPop geno_pop_div(const double* c, double n) {
   GenoPop p;
 #pragma unroll
   for (int m = 0; m < 9; ++m) p.g[m] = c[8 - m];  // moments order n22 .. n00
+  double y[9];
+  geno_centered(p.g, y);
+  geno_set_centered(p, y);
   p.n = n;
   const double f1 = n, f2 = f1 * (n - 1.0), f3 = f2 * (n - 2.0), f4 = f3 * (n - 3.0);
   p.r1 = f1 > 0.0 ? 1.0 / f1 : 0.0;

@@ -51,6 +51,13 @@ __device__ __forceinline__ double pgld_count_f64(uint32_t v) {
   return d;
 }
 
+// Signed twin of pgld_count_f64 (one v_cvt_f64_i32).
+__device__ __forceinline__ double pgld_int_f64(int32_t v) {
+  double d;
+  asm("v_cvt_f64_i32 %0, %1" : "=v"(d) : "v"(v));
+  return d;
+}
+
 #define PGLD_LIMBS 3
 
 __device__ __forceinline__ void pgld_to_limbs(double v, long long& x2, long long& x1,

@@ -71,9 +71,9 @@ template <int P> struct PlainSet<PGLD_MODE_GENO, P> {
       const int x = row[i], y = row[j];
       if (x >= 0 && y >= 0) c[(2 - x) * 3 + (2 - y)] += 1;
     }
-    int n = 0;
-    for (int m = 0; m < 9; ++m) n += c[m];
-    return geno_pop(c, n, a.rtab);
+    int y[9];
+    geno_centered(c, y);  // y[0] = valid individuals
+    return geno_pop(c, y, y[0], a.rtab);
   }
   __device__ static void eval(const Pop* p, double* o) {
     if constexpr (P == 1) geno_stats_P1(p, o);

@@ -150,3 +150,56 @@ def test_haplotype_r2_matches_brute_force_pearson():
             assert np.isnan(got[k])
         else:
             assert abs(got[k] - np.corrcoef(x, y)[0, 1] ** 2) < 1e-12
+
+
+def test_centered_moment_forms_equal_reference_polynomials():
+    """The device code evaluates the reference's three single-population
+    sub-polynomials (genotype_kernels.py:27-71, 235-272, 364-405; the
+    coefficient tables in pg_gpu_amd/csrc/geno_kernel_polys.json) in centered
+    dosage moments.  Substituting the moments' definitions as linear forms of
+    the nine class counts, each form expands to the reference polynomial
+    exactly; so do the features D, pA, qA, pB, qB, A, B the other estimators
+    use."""
+    import json
+    from pathlib import Path
+
+    path = Path(__file__).resolve().parents[1] / "pg_gpu_amd" / "csrc" / "geno_kernel_polys.json"
+    data = json.loads(path.read_text())
+    g = sp.symbols("g0:9")
+    X = [2, 2, 2, 1, 1, 1, 0, 0, 0]   # left dosage of class m (moments order)
+    Y = [2, 1, 0, 2, 1, 0, 2, 1, 0]   # right dosage
+
+    def mom(k, l):
+        return sum(sp.Integer((X[m] - 1) ** k * (Y[m] - 1) ** l) * g[m] for m in range(9))
+
+    n, a1, a2, b1, b2 = mom(0, 0), mom(1, 0), mom(2, 0), mom(0, 1), mom(0, 2)
+    ab11, ab12, ab21, ab22 = mom(1, 1), mom(1, 2), mom(2, 1), mom(2, 2)
+    u = a2 * b2 - a1 ** 2 * b2 - a2 * b1 ** 2
+    t4, t56, s11 = a1 * b1 * ab11, a1 * ab12 + ab21 * b1, ab11 ** 2
+    forms = {
+        "dd_poly": 2 * (t4 + (n - 1) * t56) + u - s11 * (3 * n - 2) - ab22 * n * (n - 1),
+        "dz_quartic": t4 - u - (n + 1) * t56 - s11 * (n - 1) + 2 * ab22 * n,
+        "pi2_corr": (u + 4 * (t56 - t4) + (a1 ** 2 + b1 ** 2) * (5 * n - 6)
+                     + (a2 + b2) * (n - 2) * (n - 3) + 2 * s11 - 6 * ab22
+                     - n * (6 * n ** 2 - 11 * n + 6)),
+    }
+    for name, form in forms.items():
+        ref = sum(sp.Integer(int(c)) * sp.Mul(*[g[m] ** e for m, e in enumerate(exps)])
+                  for exps, c in data[name])
+        assert sp.expand(form - ref) == 0, name
+    half, quarter = sp.Rational(1, 2), sp.Rational(1, 4)
+    gxAB = g[0] + half * g[1] + half * g[3] + quarter * g[4]
+    gxAb = half * g[1] + g[2] + quarter * g[4] + half * g[5]
+    gxaB = half * g[3] + quarter * g[4] + g[6] + half * g[7]
+    gxab = quarter * g[4] + half * g[5] + half * g[7] + g[8]
+    feats = {  # reference feature (genotype_kernels.py:412-492) vs centered form
+        "D": (gxAB * gxab - gxAb * gxaB, (n * ab11 - a1 * b1) / 4),
+        "pA": (g[0] + g[1] + g[2] + half * (g[3] + g[4] + g[5]), (n + a1) / 2),
+        "qA": (half * (g[3] + g[4] + g[5]) + g[6] + g[7] + g[8], (n - a1) / 2),
+        "pB": (g[0] + half * g[1] + g[3] + half * g[4] + g[6] + half * g[7], (n + b1) / 2),
+        "qB": (half * g[1] + g[2] + half * g[4] + g[5] + half * g[7] + g[8], (n - b1) / 2),
+        "A": (-g[0] + g[2] - g[3] + g[5] - g[6] + g[8], -b1),
+        "B": (-g[0] - g[1] - g[2] + g[6] + g[7] + g[8], -a1),
+    }
+    for name, (ref, form) in feats.items():
+        assert sp.expand(ref - form) == 0, name
