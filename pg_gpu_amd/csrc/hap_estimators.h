@@ -23,6 +23,7 @@ struct HapPop {
   double uA, uB;           // qA - pA, qB - pB
   double PA, PB;           // pA*qA, pB*qB
   double w;                // uA*uB + s23 - s14
+  double n;                // c1 + c2 + c3 + c4
 };
 
 // The shared products of a population's counts (c1..c4 and the r's set).
@@ -35,6 +36,7 @@ __device__ __forceinline__ void hap_basis(HapPop& p) {
   p.uA = p.qA - p.pA; p.uB = p.qB - p.pB;
   p.PA = p.pA * p.qA; p.PB = p.pB * p.qB;
   p.w = fma(p.uA, p.uB, p.s23 - p.s14);
+  p.n = p.pA + p.qA;
 }
 
 // rtab holds 4 doubles per n: 1/(n)_1, 1/(n)_2, 1/(n)_3, 1/(n)_4.
@@ -120,32 +122,23 @@ __device__ __forceinline__ double hap_pi2_iikl(const HapPop& a, const HapPop& b,
 __device__ __forceinline__ double hap_pi2_ijkk(const HapPop& a, const HapPop& b, const HapPop& c) {
   return c.PB * fma(a.pA, b.qA, a.qA * b.pA) * 0.5 * (c.r2 * a.r1 * b.r1);
 }
-// pi2(i,j,i,j) and pi2(i,j,j,i).  The reference's quartic and cubic
-// polynomials in the counts a1..a4 of A and b1..b4 of B regroup as
-//   quartic = (A.qA B.pA + A.pA B.qA) (A.qB B.pB + A.pB B.qB)
-//   cubic   = (b1 a4 + b2 a3 + b3 a2 + b4 a1)
-//           - (b1 A.qA A.qB + b2 A.pB A.qA + b3 A.pA A.qB + b4 A.pA A.pB)
-//           - (a1 B.qA B.qB + a2 B.pB B.qA + a3 B.pA B.qB + a4 B.pA B.pB)
+// pi2(i,j,i,j) and pi2(i,j,j,i).  The reference's quartic + cubic
+// polynomial in the counts of A and B equals, with n = c1+c2+c3+c4 of each,
+//   (w_A w_B - (n_A - 1)(n_B - 1)(uA_A uA_B + uB_A uB_B - n_A n_B)) / 4
+// (uA = qA - pA, uB = qB - pB, w = uA uB + c2 + c3 - c1 - c4 are the centered
+// allele and haplotype sums up to sign), an identity in the eight counts.
 __device__ __forceinline__ double hap_pi2_ijij(const HapPop& A, const HapPop& B) {
-  const double quartic = fma(A.qA, B.pA, A.pA * B.qA) * fma(A.qB, B.pB, A.pB * B.qB);
-  const double lin = fma(B.c1, A.c4, fma(B.c2, A.c3, fma(B.c3, A.c2, B.c4 * A.c1)));
-  const double ta = fma(B.c1, A.qA * A.qB, fma(B.c2, A.pB * A.qA,
-                    fma(B.c3, A.pA * A.qB, B.c4 * (A.pA * A.pB))));
-  const double tb = fma(A.c1, B.qA * B.qB, fma(A.c2, B.pB * B.qA,
-                    fma(A.c3, B.pA * B.qB, A.c4 * (B.pA * B.pB))));
-  return ((quartic + lin) - (ta + tb)) * 0.25 * (A.r2 * B.r2);
+  const double nA = A.n, nB = B.n;
+  const double m = (nA - 1.0) * (nB - 1.0);
+  const double x = fma(-nA, nB, fma(A.uB, B.uB, A.uA * B.uA));
+  return (fma(-m, x, A.w * B.w) * 0.0625) * (A.r2 * B.r2);
 }
-// one pop (S) shared between the two locus pairs, A/B the other two pops
+// One pop (S) shared between the two locus pairs, A/B the other two pops.
+// The reference's polynomial equals, in the same centered sums,
+//   (uA_A uB_B w_S - (n_S - 1)(uA_S uA_A n_B + uB_S uB_B n_A - n_S n_A n_B)) / 4.
 __device__ __forceinline__ double hap_pi2_shared(const HapPop& S, const HapPop& A, const HapPop& B) {
-  const double s1 = S.c1, s2 = S.c2, s3 = S.c3, s4 = S.c4;
-  const double inner = (s3 - 1.0) * B.c1 + s3 * B.c2 - B.c3 + s3 * B.c3 + s3 * B.c4 + s1 * B.qB;
-  const double num =
-      s4 * s4 * A.pA * B.pB + s2 * s2 * A.qA * B.pB +
-      (s1 + s3 - 1.0) * (s3 * A.pA + s1 * A.qA) * B.qB +
-      s4 * (s1 * A.qA * B.pB + (A.c1 + A.c2) * inner) +
-      s2 * (s4 * (A.pA + A.qA) * B.pB + s3 * ((A.c1 + A.c2) * B.pB + A.qA * B.qB) +
-            A.qA * ((s1 - 1.0) * B.c1 - B.c3 + s1 * (B.c2 + B.c3 + B.c4)));
-  return num * 0.25 * (S.r2 * A.r1 * B.r1);
+  const double x = fma(-S.n, A.n * B.n, fma(S.uB * B.uB, A.n, (S.uA * A.uA) * B.n));
+  return (fma(-(S.n - 1.0), x, (A.uA * B.uB) * S.w) * 0.0625) * (S.r2 * A.r1 * B.r1);
 }
 // pi2(i,j,k,l), all distinct
 __device__ __forceinline__ double hap_pi2_alldiff(const HapPop& a, const HapPop& b,
