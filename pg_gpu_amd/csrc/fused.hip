@@ -332,13 +332,14 @@ struct HapTraits {
   static constexpr int NSO = NS + (R2 ? 2 : 0);
   static constexpr int NGT = NG + (R2 ? 1 : 0);
   struct Counts { v16u w0[P], w1[P]; };
-  __device__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane, Counts& c) {
+  __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane,
+                                                Counts& c) {
 #pragma unroll
     for (int p = 0; p < P; ++p) tile_counts<(MINB >= 3)>(a, p, rb, cb, lane, c.w0[p], c.w1[p]);
   }
   // statistics G*GS .. G*GS+GS-1 of the pair in accumulator slot r
   template <int G>
-  __device__ static void stats(const FusedParams& a, const double* srt, const Counts& c, int r,
+  __device__ __forceinline__ static void stats(const FusedParams& a, const double* srt, const Counts& c, int r,
                                double* x) {
     HapPop pp[P];
 #pragma unroll
@@ -369,12 +370,13 @@ struct GenoTraits {
   static constexpr int NSO = NS + (R2 ? 2 : 0);
   static constexpr int NGT = NG + (R2 ? 1 : 0);
   struct Counts { v16u w[P][W]; };
-  __device__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane, Counts& c) {
+  __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane,
+                                                Counts& c) {
 #pragma unroll
     for (int p = 0; p < P; ++p) geno_tile_counts<W>(a, p, rb, cb, lane, c.w[p]);
   }
   template <int G>
-  __device__ static void stats(const FusedParams& a, const double* srt, const Counts& c, int r,
+  __device__ __forceinline__ static void stats(const FusedParams& a, const double* srt, const Counts& c, int r,
                                double* x) {
     GenoPop pp[P];
 #pragma unroll
