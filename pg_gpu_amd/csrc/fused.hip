@@ -493,7 +493,9 @@ struct TileCtx {
   int* scw;      // this wave's pair counters [nb]
   unsigned long long* slimb;
   int lane, bmin, bmax;
-  bool fast;
+  bool fast;   // every pair valid and in bin bmin
+  bool dual;   // otherwise: every valid pair in bin blo or blo + 1
+  int blo;
   int64_t i0, j0;
 };
 
@@ -578,6 +580,39 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
       for (int q = 0; q < GS; ++q) acc[q] += x[q];
     }
     reduce_to_limbs<NS, GS, S0, G == 0>(t, acc, t.bmin, 1024ull);
+  } else if (t.dual) {
+    // Two bins: each pair's statistics go to both accumulators with 0/1
+    // weights (the estimators are finite for every slot: counts of padded
+    // sites are zero and the reciprocal table covers every n), so the wave
+    // does the fast path's work plus one FMA per statistic, and the per-bin
+    // pair counts are ballots.
+    uint32_t bw[4];
+    tile_bins(a, t, bw);
+    double acc1[GS];
+#pragma unroll
+    for (int q = 0; q < GS; ++q) acc1[q] = 0.0;
+    unsigned long long cnt0 = 0, cnt1 = 0;
+#pragma unroll 1
+    for (int r = 0; r < 16; ++r) {
+      const int rq = r >> 2;
+      const uint32_t w = rq == 0 ? bw[0] : rq == 1 ? bw[1] : rq == 2 ? bw[2] : bw[3];
+      const int b = (int)((w >> (8 * (r & 3))) & 0xFFu);
+      const bool in0 = b == t.blo, in1 = b == t.blo + 1;
+      if constexpr (G == 0) {
+        cnt0 += (unsigned long long)__popcll(__ballot(in0));
+        cnt1 += (unsigned long long)__popcll(__ballot(in1));
+      }
+      const double w0 = in0 ? 1.0 : 0.0, w1 = in1 ? 1.0 : 0.0;
+      double x[GS];
+      T::template stats<G>(a, t.srt, cts, r, x);
+#pragma unroll
+      for (int q = 0; q < GS; ++q) {
+        acc[q] = fma(w0, x[q], acc[q]);
+        acc1[q] = fma(w1, x[q], acc1[q]);
+      }
+    }
+    reduce_to_limbs<NS, GS, S0, G == 0>(t, acc, t.blo, cnt0);
+    if (t.blo + 1 < nb) reduce_to_limbs<NS, GS, S0, G == 0>(t, acc1, t.blo + 1, cnt1);
   } else {
     uint32_t bw[4];
     tile_bins(a, t, bw);
@@ -681,6 +716,8 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
     t.bmax = pgld_bin_scan(coord_diff(a, i0, j1), sedges, nb, t.bmin < 0 ? 0 : t.bmin, nb - 1);
     t.fast = (cb > rb) && (j1 == j0 + 31) && (a.pos[j1] <= a.pos[i0] + a.max_dist) &&
              t.bmin == t.bmax && t.bmin >= 0 && t.bmin < nb;
+    t.blo = t.bmin < 0 ? 0 : t.bmin;
+    t.dual = !t.fast && t.blo < nb && (t.bmax >= nb ? nb - 1 : t.bmax) - t.blo <= 1;
     tile_groups<T, 0>(a, t, cts);
     local = ctr ? waves_per_xcd + (int64_t)__builtin_amdgcn_readfirstlane(__shfl(claim, 0, 64))
                 : local + waves_per_xcd;
