@@ -65,6 +65,22 @@ def traffic_from_profile(name):
     return None
 
 
+def utilisation_from_profile(name):
+    """MFMA / VALU utilisation of the fused kernel recorded by the rocprofv3
+    counter pass in profiles/<name>_utilisation.md (None if not profiled)."""
+    path = os.path.join(ROOT, "profiles", f"{name}_utilisation.md")
+    out = {}
+    try:
+        for line in open(path):
+            for key in ("mfma_util_percent", "valu_busy_percent", "mfma_ops_per_launch",
+                        "valu_instructions_per_launch"):
+                if line.startswith(f"- {key}:"):
+                    out[key] = float(line.split(":")[1])
+    except OSError:
+        return None
+    return out or None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -218,8 +234,12 @@ def main():
         run = lambda kernel="fused": plan.run(kernel, r2=r2)  # noqa: E731
         n_bins = int(len(w.bins) - 1)
 
-    def step(kernel="fused", ev=None):
+    def step(kernel="fused", ev=None, pev=None):
+        if pev is not None:
+            pev[0].record(stream)
         pan.repack()
+        if pev is not None:
+            pev[1].record(stream)
         if ev is not None:
             ev[0].record(stream)
         r = run(kernel)
@@ -233,14 +253,19 @@ def main():
     log("warmup done")
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(args.steps)]
+    pevs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            for _ in range(args.steps)]
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     start.record(stream)
-    results = [step(ev=evs[k]) for k in range(args.steps)]
+    results = [step(ev=evs[k], pev=pevs[k]) for k in range(args.steps)]
     stop.record(stream)
     torch.cuda.synchronize()
     total_ms = start.elapsed_time(stop)
     kern_ms = [a.elapsed_time(b) for a, b in evs]
+    pack_ms = float(np.mean([a.elapsed_time(b) for a, b in pevs]))
+    # pack: every (row, site) of each population read once, the panels written once
+    pack_bytes = int(sum(pan.pop_sizes)) * V + int(pan.panels.numel())
     res = results[-1].finish()
     n_pairs = int(res.n_pairs.sum()) if windows else res.n_pairs
     ms_step = total_ms / args.steps
@@ -317,6 +342,7 @@ def main():
              f"fused_kernel<{'Geno' if geno else 'Hap'}Traits<{pan.P}{', r2' if r2 else ''}>>")
     prof = f"{'windows' if windows else 'fused'}_{w.mode}{pan.P}_{w.name}"
     traffic = traffic_from_profile(prof) if args.scale == 1.0 else None
+    util = utilisation_from_profile(prof) if args.scale == 1.0 else None
     line = {
         "metric": "site-pairs/sec binned DD/Dz/pi2, I8-MFMA fused vs plain FP64; max rel err",
         "value": n_pairs / (ms_step * 1e-3),
@@ -342,7 +368,12 @@ def main():
                      "frac": achieved / PEAK_FP4_TOPS, "frac_of_i8_peak": achieved / PEAK_I8_TOPS,
                      "traffic": traffic,
                      "traffic_profile": f"profiles/{prof}_hbm_traffic.md" if traffic else None,
-                     "kernel": kname, "ops_per_pair": ops_per_pair, "kernel_ms": k_ms},
+                     "kernel": kname, "ops_per_pair": ops_per_pair, "kernel_ms": k_ms,
+                     "counters": util,
+                     "counters_profile": f"profiles/{prof}_utilisation.md" if util else None},
+        "pack": {"ms": pack_ms, "bytes": pack_bytes,
+                 "GB/s": pack_bytes / (pack_ms * 1e-3) / 1e9,
+                 "frac_of_hbm_peak": pack_bytes / (pack_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
         "plain_fp64": plain,
         "speedup_vs_plain": (plain["ms"] / k_ms) if plain else None,
         "precision": prec,

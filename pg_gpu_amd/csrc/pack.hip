@@ -80,56 +80,88 @@ __global__ void __launch_bounds__(256) scan_sites_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Pack: one workgroup per (4 site blocks = 128 sites, pop, 64-sample k-step).
-// Phase 1 stages 64 sample rows x 128 sites through LDS (4 threads read one
-// row's 128 contiguous bytes); phase 2 writes 4 x n_feat fragments, 16 bytes
-// per lane, fully coalesced.
+// Pack: one workgroup per (PACK_SB site blocks, pop, 64-sample k-step).
+// Phase 1 reads 64 sample rows x 32 PACK_SB sites, 16 bytes per load (long
+// contiguous row segments: the rows of a population lie far apart), and stores them
+// transposed ([site][sample]) in LDS; phase 2 gives each lane the 32 samples
+// of one site as two 16-byte LDS reads, turns them into 4-bit indicators with
+// bytewise (SWAR) tests, and writes 16 bytes per lane, fully coalesced.
 // ---------------------------------------------------------------------------
+// Bit 0 of byte b set where byte b of x equals c (exact per byte: the 7-bit
+// sums cannot carry across bytes).
+__device__ __forceinline__ uint32_t bytes_eq(uint32_t x, uint32_t c) {
+  const uint32_t u = x ^ c;
+  return (~(((u & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | u | 0x7F7F7F7Fu)) >> 7;
+}
+// The four byte flags (bits 0, 8, 16, 24) moved to nibbles 0..3 (bits 0, 4, 8, 12).
+__device__ __forceinline__ uint32_t flags_to_nibbles(uint32_t m) {
+  const uint32_t y = m | (m >> 4);
+  return (y & 0x11u) | ((y >> 8) & 0x1100u);
+}
+
+constexpr int PACK_SB = 4;  // site blocks per workgroup
+
 __global__ void __launch_bounds__(256) pack_kernel(
     const int8_t* __restrict__ mat, int64_t n_sites, int64_t ld,
     const int32_t* __restrict__ rows, int32_t row_begin, int32_t pop_size, int32_t mode,
     int32_t n_feat, int32_t ksteps, int64_t pop_off, int64_t block_bytes, int64_t n_blocks,
-    int8_t* __restrict__ panels) {
-  __shared__ int8_t tile[64][128 + 4];  // [sample][site]
-  const int64_t blk0 = (int64_t)blockIdx.x * 4;
+    int32_t vec_ok, int8_t* __restrict__ panels) {
+  constexpr int PITCH = 64 + 16;                              // bytes per site row
+  constexpr int SITES = 32 * PACK_SB, GROUPS = SITES / 16;
+  __shared__ __attribute__((aligned(16))) int8_t tile[SITES * PITCH];  // [site][sample]
+  const int64_t blk0 = (int64_t)blockIdx.x * PACK_SB;
   const int ks = blockIdx.y;
   const int t = threadIdx.x;
-  {  // Phase 1: thread t loads 32 sites of sample row (t >> 2).
-    const int s = t >> 2;
+  // Phase 1: item (sample row s, 16-site group g); consecutive threads take
+  // consecutive groups of one row.
+  for (int it = t; it < 64 * GROUPS; it += 256) {
+    const int s = it / GROUPS, g = it % GROUPS;
     const int k = ks * 64 + s;
-    const int sx = (t & 3) * 32;
-    const bool row_ok = k < pop_size;
-    const int8_t* src = row_ok ? mat + (int64_t)rows[row_begin + k] * ld : mat;
-#pragma unroll 8
-    for (int e = 0; e < 32; ++e) {
-      const int64_t site = blk0 * 32 + sx + e;
-      const bool have = row_ok && site < n_sites;
-      tile[s][sx + e] = have ? src[site] : (int8_t)-2;  // -2: padding (every feature 0)
+    const int64_t site0 = blk0 * 32 + g * 16;
+    uint32_t w[4] = {0xFEFEFEFEu, 0xFEFEFEFEu, 0xFEFEFEFEu, 0xFEFEFEFEu};  // -2: padding
+    if (k < pop_size) {
+      const int8_t* src = mat + (int64_t)rows[row_begin + k] * ld + site0;
+      if (vec_ok && site0 + 16 <= n_sites) {
+        const uint4 q = *reinterpret_cast<const uint4*>(src);
+        w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const uint32_t v = site0 + e < n_sites ? (uint8_t)src[e] : 0xFEu;
+          w[e >> 2] = (w[e >> 2] & ~(0xFFu << (8 * (e & 3)))) | (v << (8 * (e & 3)));
+        }
+      }
     }
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+      tile[(g * 16 + e) * PITCH + s] = (int8_t)(w[e >> 2] >> (8 * (e & 3)));
   }
   __syncthreads();
   // Phase 2: (block, feature, lane) -> 16 bytes = 32 samples of one site.
-  for (int w = t; w < 4 * n_feat * 64; w += 256) {
+  for (int w = t; w < PACK_SB * n_feat * 64; w += 256) {
     const int lane = w & 63, f = (w >> 6) % n_feat, bl = (w >> 6) / n_feat;
     if (blk0 + bl >= n_blocks) continue;
     const int site = bl * 32 + (lane & 31);
     const int s0 = (lane >> 5) * 32;
-    // feature code: hap {0: H, 1: V}; geno {0: I1, 1: I2, 2: V}
+    const uint4* src = reinterpret_cast<const uint4*>(tile + site * PITCH + s0);
+    const uint4 lo = src[0], hi = src[1];
+    const uint32_t x[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};  // samples 4j..4j+3
+    // feature code: hap {0: [v==1], 1: [v>=0]}; geno {0: [v==1], 1: [v==2], 2: [v>=0]}
+    const bool valid_f = f == n_feat - 1 && f > 0 && (mode == PGLD_MODE_HAP ? f == 1 : f == 2);
+    const uint32_t c = (mode == PGLD_MODE_GENO && f == 1) ? 0x02020202u : 0x01010101u;
     uint32_t word[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      uint32_t acc = 0;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int v = tile[s0 + q * 8 + e][site];
-        int bit;
-        if (mode == PGLD_MODE_HAP)
-          bit = f == 0 ? (v == 1) : (v >= 0);
-        else
-          bit = f == 0 ? (v == 1) : (f == 1 ? (v == 2) : (v >= 0));
-        acc |= (bit ? PGLD_FP4_ONE : 0u) << (4 * e);
+      uint32_t m0, m1;
+      if (valid_f) {
+        m0 = (~x[2 * q] & 0x80808080u) >> 7;
+        m1 = (~x[2 * q + 1] & 0x80808080u) >> 7;
+      } else {
+        m0 = bytes_eq(x[2 * q], c);
+        m1 = bytes_eq(x[2 * q + 1], c);
       }
-      word[q] = acc;
+      // nibble i of word q <-> sample s0 + 8q + i; 1.0 in E2M1 is 0x2
+      word[q] = (flags_to_nibbles(m0) | (flags_to_nibbles(m1) << 16)) * PGLD_FP4_ONE;
     }
     int8_t* dst = panels + (blk0 + bl) * block_bytes + pop_off +
                   ((int64_t)f * ksteps + ks) * PGLD_FRAG_BYTES + lane * 16;
@@ -192,12 +224,15 @@ extern "C" int pgld_pack(const int8_t* d_mat, int64_t n_rows, int64_t n_sites, i
     return fail(PGLD_EINVAL, "matrix shape disagrees with descriptor");
   if (L.n_blocks == 0) return PGLD_OK;
   hipStream_t s = as_stream(stream);
+  // 16-byte row loads need 16-byte aligned rows
+  const int32_t vec_ok = (reinterpret_cast<uintptr_t>(d_mat) % 16 == 0) && (ld % 16 == 0);
   for (int p = 0; p < desc->n_pops; ++p) {
     if (L.ksteps[p] == 0) continue;
-    hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((L.n_blocks + 3) / 4), (unsigned)L.ksteps[p]),
+    hipLaunchKernelGGL(pack_kernel,
+                       dim3((unsigned)((L.n_blocks + PACK_SB - 1) / PACK_SB), (unsigned)L.ksteps[p]),
                        dim3(256), 0, s, d_mat, n_sites, ld, d_rows, pop_offsets[p],
                        L.pop_size[p], desc->mode, L.n_feat[p], L.ksteps[p], L.pop_off[p],
-                       L.block_bytes, L.n_blocks, d_panels);
+                       L.block_bytes, L.n_blocks, vec_ok, d_panels);
     PGLD_LAUNCH_CHECK();
   }
   return PGLD_OK;
