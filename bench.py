@@ -115,32 +115,34 @@ def cpu_baseline(w, target_s=15.0, max_workers=16):
     the work (anchor sites, or windows), one process per core over contiguous
     blocks."""
     global _W
-    from oracle import ld_oracle as o
-
     _W = w
     cores = max(1, min(os.cpu_count() or 1, max_workers))
+    ctx = mp.get_context("fork")
+
+    def pool_rate(worker, blocks):
+        # aggregate pairs/s of all workers on small blocks (the rate of one
+        # process alone overstates what `cores` processes reach together)
+        t0 = time.perf_counter()
+        with ctx.Pool(cores) as pool:
+            parts = pool.map(worker, blocks)
+        return sum(p[2] for p in parts) / max(time.perf_counter() - t0, 1e-6)
+
     if w.windows is not None:
         k = w.windows[:, 1] - w.windows[:, 0]
         cum = np.cumsum(k * (k - 1) // 2)
-        t0 = time.perf_counter()
-        n_cal = int(o.window_sums(w.mat, w.windows[:4], mode=w.mode)[1].sum())
-        rate = n_cal / max(time.perf_counter() - t0, 1e-6)
-        X = int(np.searchsorted(cum, rate * cores * target_s)) + 1
+        rate = pool_rate(_cpu_worker_windows, [(2 * q, 2 * q + 2) for q in range(cores)])
+        X = int(np.searchsorted(cum, rate * target_s)) + 1
         X = max(cores, min(len(k), X))
         edges = np.linspace(0, X, cores + 1).astype(int)
         blocks = [(int(edges[q]), int(edges[q + 1])) for q in range(cores)]
         log(f"cpu baseline: {cores} workers, windows [0, {X}) ~ {int(cum[X - 1]):,} pairs")
         worker = _cpu_worker_windows
     else:
-        t0 = time.perf_counter()
-        _, _, n_cal, _ = o.binned_sums(w.mat, w.positions, w.pop_rows, w.bins, w.max_dist,
-                                       chunk=100_000, with_abs=True, anchors=(0, 64),
-                                       mode=w.mode, r2=w.name in R2_WORKLOADS)
-        rate = n_cal / max(time.perf_counter() - t0, 1e-6)
+        rate = pool_rate(_cpu_worker, [(64 * q, 64 * q + 64) for q in range(cores)])
         jmax = np.searchsorted(w.positions, w.positions + w.max_dist, side="right")
         per = np.maximum(0, jmax - np.arange(len(w.positions)) - 1)
         cum = np.cumsum(per)
-        want = rate * cores * target_s
+        want = rate * target_s
         X = int(np.searchsorted(cum, want)) + 1
         X = max(64 * cores, min(len(w.positions), (X // (64 * cores)) * 64 * cores))
         edges = np.linspace(0, X, cores + 1).astype(int)
@@ -148,7 +150,7 @@ def cpu_baseline(w, target_s=15.0, max_workers=16):
         log(f"cpu baseline: {cores} workers, anchors [0, {X}) ~ {int(cum[X - 1]):,} pairs")
         worker = _cpu_worker
     t0 = time.perf_counter()
-    with mp.get_context("fork").Pool(cores) as pool:
+    with ctx.Pool(cores) as pool:
         parts = pool.map(worker, blocks)
     wall = time.perf_counter() - t0
     n_pairs = sum(p[2] for p in parts)
@@ -331,7 +333,7 @@ def main():
         log(f"precision vs oracle on {cpu['n_pairs']:,} pairs: {prec}")
 
     k_ms = float(np.mean(kern_ms))
-    # int8 MFMA products per pair and population: haplotypes H.H (+ H.V, V.H,
+    # indicator products per pair and population: haplotypes H.H (+ H.V, V.H,
     # V.V with missing calls); genotypes the 2x2 (or, with missing, 3x3)
     # feature products.  2 ops (multiply + add) per sample and product.
     geno = w.mode == "geno"

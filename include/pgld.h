@@ -54,8 +54,9 @@ const char* pgld_last_error(void);
 
 /* Description of the packed sample panels shared by the counting kernels.
  * Sites are grouped in blocks of 32; per block, per population, per feature,
- * per 32-sample k-step there is one 1 KiB fragment in the operand layout of
- * the gfx950 32x32x32 int8 MFMA.  Features per population:
+ * per 64-sample k-step there is one 1 KiB fragment of 4-bit E2M1 indicators
+ * (1.0 or 0) in the operand layout of the gfx950 32x32x64 FP4 MFMA
+ * (v_mfma_scale_f32_32x32x64_f8f6f4).  Features per population:
  *   haplotype: H = [x==1] and, if the population has missing calls, V = [x>=0]
  *   genotype:  I1 = [x==1], I2 = [x==2] and, with missing calls, V = [x>=0]. */
 typedef struct {
@@ -105,7 +106,7 @@ typedef struct {
   const double* d_edges;       /* n_bins + 1 increasing bin edges */
   int32_t n_bins;
   const double* d_rtab;        /* 4*(max pop size + 1) reciprocal falling factorials */
-  const int32_t* d_tiles;      /* (row tile, col tile) pairs in 64-site units */
+  const int32_t* d_tiles;      /* (row block, col block) pairs of 32-site blocks, row <= col */
   int64_t n_tiles;
   long long* d_out;            /* [n_bins][n_stats + 1][3] fixed-point limbs, zeroed */
   int32_t grid;                /* workgroups for the persistent fused kernel (0 = auto) */
@@ -123,7 +124,8 @@ typedef struct {
 /* Number of statistics produced per bin for (mode, n_pops). */
 int pgld_n_stats(int32_t mode, int32_t n_pops);
 
-/* Fused binned LD: banded int8 MFMA counting + estimator epilogue + bin
+/* Fused binned LD: banded MFMA counting (FP4 indicators, exact FP32 sums) +
+ * estimator epilogue + bin
  * reduction in one kernel.  Replaces the chunk loop of
  * pg_gpu/moments_ld.py:_compute_ld_sums (:248-343) -- pair enumeration
  * (ld_pipeline.py:57-122), counting (:130-263), the estimator batch
