@@ -351,6 +351,26 @@ struct HapTraits {
     if constexpr (G < NG) hap_stats_group<P, G>(pp, x);
     else hap_r2(pp[0], x);
   }
+  // acc[q] += wt * statistic G*GS+q (wt 0 or 1)
+  template <int G>
+  __device__ __forceinline__ static void accum(const FusedParams& a, const double* srt,
+                                               const Counts& c, int r, double* acc, double wt) {
+    if constexpr (G < NG) {
+      HapPop pp[P];
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const int n11 = c.w0[p][r] & 0xffff, n10 = c.w0[p][r] >> 16;
+        const int n01 = c.w1[p][r] & 0xffff, n00 = c.w1[p][r] >> 16;
+        pp[p] = hap_pop(n11, n10, n01, n00, 0, rt_entry(a, srt, p, n11 + n10 + n01 + n00));
+      }
+      hap_acc_group<P, G>(pp, acc, wt);
+    } else {
+      double x[2];
+      stats<G>(a, srt, c, r, x);
+      acc[0] = fma(x[0], wt, acc[0]);
+      acc[1] = fma(x[1], wt, acc[1]);
+    }
+  }
 };
 
 // W = 3: nine 10-bit counts in three words per slot (population sizes up to
@@ -405,6 +425,15 @@ struct GenoTraits {
     }
     if constexpr (G < NG) geno_stats_group<P, G>(pp, x);
     else geno_r2(pp[0], x);
+  }
+  template <int G>
+  __device__ __forceinline__ static void accum(const FusedParams& a, const double* srt,
+                                               const Counts& c, int r, double* acc, double wt) {
+    constexpr int GSZ = G < NG ? GS : 2;
+    double x[GSZ];
+    stats<G>(a, srt, c, r, x);
+#pragma unroll
+    for (int q = 0; q < GSZ; ++q) acc[q] = fma(x[q], wt, acc[q]);
   }
 };
 
@@ -573,12 +602,7 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
   if (t.fast) {
     // every pair of the tile is valid and falls in bin bmin
 #pragma unroll 1
-    for (int r = 0; r < 16; ++r) {
-      double x[GS];
-      T::template stats<G>(a, t.srt, cts, r, x);
-#pragma unroll
-      for (int q = 0; q < GS; ++q) acc[q] += x[q];
-    }
+    for (int r = 0; r < 16; ++r) T::template accum<G>(a, t.srt, cts, r, acc, 1.0);
     reduce_to_limbs<NS, GS, S0, G == 0>(t, acc, t.bmin, 1024ull);
   } else if (t.dual) {
     // Two bins: each pair's statistics go to both accumulators with 0/1
@@ -602,14 +626,8 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
         cnt0 += (unsigned long long)__popcll(__ballot(in0));
         cnt1 += (unsigned long long)__popcll(__ballot(in1));
       }
-      const double w0 = in0 ? 1.0 : 0.0, w1 = in1 ? 1.0 : 0.0;
-      double x[GS];
-      T::template stats<G>(a, t.srt, cts, r, x);
-#pragma unroll
-      for (int q = 0; q < GS; ++q) {
-        acc[q] = fma(w0, x[q], acc[q]);
-        acc1[q] = fma(w1, x[q], acc1[q]);
-      }
+      T::template accum<G>(a, t.srt, cts, r, acc, in0 ? 1.0 : 0.0);
+      T::template accum<G>(a, t.srt, cts, r, acc1, in1 ? 1.0 : 0.0);
     }
     reduce_to_limbs<NS, GS, S0, G == 0>(t, acc, t.blo, cnt0);
     if (t.blo + 1 < nb) reduce_to_limbs<NS, GS, S0, G == 0>(t, acc1, t.blo + 1, cnt1);
@@ -624,7 +642,9 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
       const int b = (int)((w >> (8 * (r & 3))) & 0xFFu);
       if (b != 0xFF) {
         double x[GS];
-        T::template stats<G>(a, t.srt, cts, r, x);
+#pragma unroll
+        for (int q = 0; q < GS; ++q) x[q] = 0.0;
+        T::template accum<G>(a, t.srt, cts, r, x, 1.0);
         if (b != cur) {
           if (cur >= 0) stage_add<GS, G == 0>(t.stg, t.scw, cur, acc, count);
           cur = b;
@@ -824,13 +844,11 @@ __device__ __forceinline__ void window_group(const WindowParams& w, const double
 #pragma unroll 1
   for (int r = 0; r < 16; ++r) {
     const int64_t i = i0 + mfma_row(lane, r);
-    if (full || (i >= lo && i < j && j < hi)) {
-      double x[GS];
-      T::template stats<G>(w.f, srt, cts, r, x);
-#pragma unroll
-      for (int q = 0; q < GS; ++q) acc[q] += x[q];
-      ++count;
-    }
+    // every slot's statistics are finite (padded sites have zero counts), so
+    // slots outside the window take weight 0 instead of a divergent branch
+    const bool in = full || (i >= lo && i < j && j < hi);
+    T::template accum<G>(w.f, srt, cts, r, acc, in ? 1.0 : 0.0);
+    count += in;
   }
 #pragma unroll
   for (int q = 0; q < GS; ++q) acc[q] = pgld_wave_sum(acc[q]);

@@ -160,6 +160,63 @@ def emit_set(prefix, popty, P, keyfn):
     return "\n".join(lines)
 
 
+# Reciprocal falling factorials of each haplotype raw call: (argument
+# position, k) pairs, the call's value being num * prod 1/(n_arg)_k.
+HAP_RECIP = {
+    "hap_dd_same": ((0, 4),), "hap_dd_pair": ((0, 2), (1, 2)),
+    "hap_dz_same": ((0, 4),), "hap_dz_iik": ((0, 3), (1, 1)), "hap_dz_iji": ((0, 3), (1, 1)),
+    "hap_dz_ijj": ((0, 2), (1, 2)), "hap_dz_ijk": ((0, 2), (1, 1), (2, 1)),
+    "hap_pi2_same": ((0, 4),), "hap_pi2_triple": ((0, 3), (1, 1)),
+    "hap_pi2_iikk": ((0, 2), (1, 2)), "hap_pi2_iikl": ((0, 2), (1, 1), (2, 1)),
+    "hap_pi2_ijkk": ((2, 2), (0, 1), (1, 1)), "hap_pi2_ijij": ((0, 2), (1, 2)),
+    "hap_pi2_shared": ((0, 2), (1, 1), (2, 1)),
+    "hap_pi2_alldiff": ((0, 1), (1, 1), (2, 1), (3, 1)),
+}
+
+
+def emit_acc_set(P):
+    """Accumulating form of the haplotype groups: acc[s] += wt * statistic s,
+    with each statistic's reciprocal product folded into one FMA per distinct
+    product (the averaged orientations of one statistic share it, so their
+    exact numerators are summed first)."""
+    values, stats = stat_set_plan(P, hap_key)
+    ng = (len(stats) + GROUP - 1) // GROUP
+    lines = []
+    for k in range(ng):
+        grp = stats[k * GROUP:(k + 1) * GROUP]
+        used = sorted(set(v for terms in grp for v, _ in terms))
+        lines.append(f"__device__ __forceinline__ void hap_acc_P{P}_g{k}(const HapPop* p, "
+                     "double* acc, double wt) {")
+        rkeys, rdecl, ndecl, body = {}, [], [], []
+        for v in used:
+            fn, args = values[v]
+            a = ", ".join(f"p[{x}]" for x in args)
+            ndecl.append(f"  const double n{v} = {fn}_num({a});")
+        for s, terms in enumerate(grp):
+            by_r = {}
+            for v, w in terms:
+                fn, args = values[v]
+                key = tuple(sorted((args[pos], kk) for pos, kk in HAP_RECIP[fn]))
+                by_r.setdefault(key, []).append((v, w))
+            expr = f"acc[{s}]"
+            for key, tv in by_r.items():
+                if key not in rkeys:
+                    rkeys[key] = f"R{len(rkeys)}"
+                    prod = " * ".join(f"p[{x}].r{kk}" for x, kk in key)
+                    rdecl.append(f"  const double {rkeys[key]} = {prod} * wt;")
+                if len(tv) == 1 and tv[0][1] == 1.0:
+                    num = f"n{tv[0][0]}"
+                else:
+                    num = " + ".join(f"{w!r} * n{v}" for v, w in tv)
+                expr = f"fma({num}, {rkeys[key]}, {expr})"
+            body.append(f"  acc[{s}] = {expr};")
+        lines += rdecl + ndecl + body
+        lines.append("}")
+        lines.append(f"template <> __device__ __forceinline__ void hap_acc_group<{P}, {k}>("
+                     f"const HapPop* p, double* acc, double wt) {{ hap_acc_P{P}_g{k}(p, acc, wt); }}")
+    return "\n".join(lines)
+
+
 def geno_key(kind, pp):
     """Canonical device function for one raw genotype call (case dispatch of
     pg_gpu/genotype_kernels.py:561-878)."""
@@ -215,8 +272,11 @@ def main():
              f"#define STATS_GROUP {GROUP}",
              "template <int P, int G> __device__ void hap_stats_group(const HapPop* p, double* out);",
              ""]
+    parts.append("template <int P, int G> __device__ void hap_acc_group(const HapPop* p, double* acc, "
+                 "double wt);")
     for P in range(1, 5):
         parts.append(emit_set("hap", "HapPop", P, hap_key))
+        parts.append(emit_acc_set(P))
         parts.append("")
     (HERE / "stats_hap.inc").write_text("\n".join(parts))
     parts = ["// Generated by gen_stats.py -- do not edit by hand.",

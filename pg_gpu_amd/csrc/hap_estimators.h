@@ -52,99 +52,138 @@ __device__ __forceinline__ HapPop hap_pop(int n11, int n10, int n01, int n00, in
   return p;
 }
 
-// The numerators below are the reference's polynomials regrouped over the
-// shared products (identities in the counts, e.g.
-// c1(c1-1)c4(c4-1) + c2(c2-1)c3(c3-1) - 2c1c2c3c4 = D^2 - x(s14-1) - y(s23-1));
-// every intermediate is an integer, so they equal the reference's numerators
-// exactly whatever the grouping.
+// Each raw call is num * R: num the reference's polynomial in the counts
+// (times an exact power of two), regrouped over the shared products above
+// (identities in the counts, e.g.
+// c1(c1-1)c4(c4-1) + c2(c2-1)c3(c3-1) - 2c1c2c3c4 = D^2 - x(s14-1) - y(s23-1)),
+// and R the product of the reciprocal falling factorials 1/(n)_k of its
+// populations.  Every intermediate of num is an integer (or a multiple of
+// 1/16) below 2^53, so num equals the reference's numerator exactly whatever
+// the grouping; hap_X = hap_X_num * R.  The fused kernel folds R into its
+// accumulation (one FMA per statistic, see the generated hap_acc_* groups).
 
 // ---- DD -------------------------------------------------------------------
-__device__ __forceinline__ double hap_dd_same(const HapPop& a) {
-  const double num = fma(a.D, a.D, -fma(a.x, a.s14 - 1.0, a.y * (a.s23 - 1.0)));
-  return num * a.r4;
+__device__ __forceinline__ double hap_dd_same_num(const HapPop& a) {
+  return fma(a.D, a.D, -fma(a.x, a.s14 - 1.0, a.y * (a.s23 - 1.0)));
+}
+__device__ __forceinline__ double hap_dd_same(const HapPop& a) { return hap_dd_same_num(a) * a.r4; }
+__device__ __forceinline__ double hap_dd_pair_num(const HapPop& a, const HapPop& b) {
+  return a.D * b.D;
 }
 __device__ __forceinline__ double hap_dd_pair(const HapPop& a, const HapPop& b) {
-  return (a.D * b.D) * (a.r2 * b.r2);
+  return hap_dd_pair_num(a, b) * (a.r2 * b.r2);
 }
 
 // ---- Dz -------------------------------------------------------------------
 // (c1c4 - c2c3)((qA-pA)(qB-pB) + (c2+c3) - (c1+c4)) + 2(c2c3 + c1c4)
-__device__ __forceinline__ double hap_dz_same(const HapPop& a) {
-  const double num = fma(-a.D, a.w, 2.0 * (a.x + a.y));
-  return num * a.r4;
+__device__ __forceinline__ double hap_dz_same_num(const HapPop& a) {
+  return fma(-a.D, a.w, 2.0 * (a.x + a.y));
 }
+__device__ __forceinline__ double hap_dz_same(const HapPop& a) { return hap_dz_same_num(a) * a.r4; }
 // Dz(i,i,k)
+__device__ __forceinline__ double hap_dz_iik_num(const HapPop& a, const HapPop& b) {
+  return (a.uA * (-a.D)) * b.uB;
+}
 __device__ __forceinline__ double hap_dz_iik(const HapPop& a, const HapPop& b) {
-  const double num = (a.uA * (-a.D)) * b.uB;
-  return num * (a.r3 * b.r1);
+  return hap_dz_iik_num(a, b) * (a.r3 * b.r1);
 }
 // Dz(i,j,i)
+__device__ __forceinline__ double hap_dz_iji_num(const HapPop& a, const HapPop& b) {
+  return (a.uB * (-a.D)) * b.uA;
+}
 __device__ __forceinline__ double hap_dz_iji(const HapPop& a, const HapPop& b) {
-  const double num = (a.uB * (-a.D)) * b.uA;
-  return num * (a.r3 * b.r1);
+  return hap_dz_iji_num(a, b) * (a.r3 * b.r1);
 }
 // Dz(i,j,j)
+__device__ __forceinline__ double hap_dz_ijj_num(const HapPop& a, const HapPop& b) {
+  return (-a.D) * b.w;
+}
 __device__ __forceinline__ double hap_dz_ijj(const HapPop& a, const HapPop& b) {
-  return ((-a.D) * b.w) * (a.r2 * b.r2);
+  return hap_dz_ijj_num(a, b) * (a.r2 * b.r2);
 }
 // Dz(i,j,k), all distinct
+__device__ __forceinline__ double hap_dz_ijk_num(const HapPop& a, const HapPop& b,
+                                                 const HapPop& c) {
+  return -(a.D * b.uA * c.uB);
+}
 __device__ __forceinline__ double hap_dz_ijk(const HapPop& a, const HapPop& b, const HapPop& c) {
-  const double num = -(a.D * b.uA * c.uB);
-  return num * (a.r2 * b.r1 * c.r1);
+  return hap_dz_ijk_num(a, b, c) * (a.r2 * b.r1 * c.r1);
 }
 
 // ---- pi2 ------------------------------------------------------------------
 // pA pB qA qB - c1c4 (c1 + 3c2 + 3c3 + c4 - 1) - c2c3 (3c1 + c2 + c3 + 3c4 - 1)
-__device__ __forceinline__ double hap_pi2_same(const HapPop& a) {
+__device__ __forceinline__ double hap_pi2_same_num(const HapPop& a) {
   const double u = fma(3.0, a.s23, a.s14) - 1.0, v = fma(3.0, a.s14, a.s23) - 1.0;
-  const double num = fma(a.PA, a.PB, -fma(a.x, u, a.y * v));
-  return num * a.r4;
+  return fma(a.PA, a.PB, -fma(a.x, u, a.y * v));
 }
+__device__ __forceinline__ double hap_pi2_same(const HapPop& a) { return hap_pi2_same_num(a) * a.r4; }
 // pop t three times, pop s once (all "triple" cases share this formula):
 // (s.pB (PA qB - m) + s.qB (PA pB + m - 2 PA)) / 2 with PA, qB, pB, m of t and
 // m = pA c4 + c2 qA
-__device__ __forceinline__ double hap_pi2_triple(const HapPop& t, const HapPop& s) {
+__device__ __forceinline__ double hap_pi2_triple_num(const HapPop& t, const HapPop& s) {
   const double m = fma(t.pA, t.c4, t.c2 * t.qA);
   const double u = fma(t.PA, t.qB, -m);
   const double v = fma(-2.0, t.PA, fma(t.PA, t.pB, m));
-  const double num = fma(s.qB, v, s.pB * u) * 0.5;
-  return num * (s.r1 * t.r3);
+  return fma(s.qB, v, s.pB * u) * 0.5;
+}
+__device__ __forceinline__ double hap_pi2_triple(const HapPop& t, const HapPop& s) {
+  return hap_pi2_triple_num(t, s) * (s.r1 * t.r3);
 }
 // pi2(i,i,k,k)
+__device__ __forceinline__ double hap_pi2_iikk_num(const HapPop& a, const HapPop& b) {
+  return a.PA * b.PB;
+}
 __device__ __forceinline__ double hap_pi2_iikk(const HapPop& a, const HapPop& b) {
-  return (a.PA * b.PB) * (a.r2 * b.r2);
+  return hap_pi2_iikk_num(a, b) * (a.r2 * b.r2);
 }
 // pi2(i,i,k,l), k != l
+__device__ __forceinline__ double hap_pi2_iikl_num(const HapPop& a, const HapPop& b,
+                                                   const HapPop& c) {
+  return a.PA * fma(b.pB, c.qB, b.qB * c.pB) * 0.5;
+}
 __device__ __forceinline__ double hap_pi2_iikl(const HapPop& a, const HapPop& b, const HapPop& c) {
-  return a.PA * fma(b.pB, c.qB, b.qB * c.pB) * 0.5 * (a.r2 * b.r1 * c.r1);
+  return hap_pi2_iikl_num(a, b, c) * (a.r2 * b.r1 * c.r1);
 }
 // pi2(i,j,k,k), i != j
+__device__ __forceinline__ double hap_pi2_ijkk_num(const HapPop& a, const HapPop& b,
+                                                   const HapPop& c) {
+  return c.PB * fma(a.pA, b.qA, a.qA * b.pA) * 0.5;
+}
 __device__ __forceinline__ double hap_pi2_ijkk(const HapPop& a, const HapPop& b, const HapPop& c) {
-  return c.PB * fma(a.pA, b.qA, a.qA * b.pA) * 0.5 * (c.r2 * a.r1 * b.r1);
+  return hap_pi2_ijkk_num(a, b, c) * (c.r2 * a.r1 * b.r1);
 }
 // pi2(i,j,i,j) and pi2(i,j,j,i).  The reference's quartic + cubic
 // polynomial in the counts of A and B equals, with n = c1+c2+c3+c4 of each,
 //   (w_A w_B - (n_A - 1)(n_B - 1)(uA_A uA_B + uB_A uB_B - n_A n_B)) / 4
 // (uA = qA - pA, uB = qB - pB, w = uA uB + c2 + c3 - c1 - c4 are the centered
 // allele and haplotype sums up to sign), an identity in the eight counts.
+__device__ __forceinline__ double hap_pi2_ijij_num(const HapPop& A, const HapPop& B) {
+  const double m = (A.n - 1.0) * (B.n - 1.0);
+  const double x = fma(-A.n, B.n, fma(A.uB, B.uB, A.uA * B.uA));
+  return fma(-m, x, A.w * B.w) * 0.0625;
+}
 __device__ __forceinline__ double hap_pi2_ijij(const HapPop& A, const HapPop& B) {
-  const double nA = A.n, nB = B.n;
-  const double m = (nA - 1.0) * (nB - 1.0);
-  const double x = fma(-nA, nB, fma(A.uB, B.uB, A.uA * B.uA));
-  return (fma(-m, x, A.w * B.w) * 0.0625) * (A.r2 * B.r2);
+  return hap_pi2_ijij_num(A, B) * (A.r2 * B.r2);
 }
 // One pop (S) shared between the two locus pairs, A/B the other two pops.
 // The reference's polynomial equals, in the same centered sums,
 //   (uA_A uB_B w_S - (n_S - 1)(uA_S uA_A n_B + uB_S uB_B n_A - n_S n_A n_B)) / 4.
-__device__ __forceinline__ double hap_pi2_shared(const HapPop& S, const HapPop& A, const HapPop& B) {
+__device__ __forceinline__ double hap_pi2_shared_num(const HapPop& S, const HapPop& A,
+                                                     const HapPop& B) {
   const double x = fma(-S.n, A.n * B.n, fma(S.uB * B.uB, A.n, (S.uA * A.uA) * B.n));
-  return (fma(-(S.n - 1.0), x, (A.uA * B.uB) * S.w) * 0.0625) * (S.r2 * A.r1 * B.r1);
+  return fma(-(S.n - 1.0), x, (A.uA * B.uB) * S.w) * 0.0625;
+}
+__device__ __forceinline__ double hap_pi2_shared(const HapPop& S, const HapPop& A, const HapPop& B) {
+  return hap_pi2_shared_num(S, A, B) * (S.r2 * A.r1 * B.r1);
 }
 // pi2(i,j,k,l), all distinct
+__device__ __forceinline__ double hap_pi2_alldiff_num(const HapPop& a, const HapPop& b,
+                                                      const HapPop& c, const HapPop& d) {
+  return fma(a.pA, b.qA, a.qA * b.pA) * fma(c.pB, d.qB, c.qB * d.pB) * 0.25;
+}
 __device__ __forceinline__ double hap_pi2_alldiff(const HapPop& a, const HapPop& b,
                                                   const HapPop& c, const HapPop& d) {
-  return fma(a.pA, b.qA, a.qA * b.pA) * fma(c.pB, d.qB, c.qB * d.pB) * 0.25 *
-         (a.r1 * b.r1 * c.r1 * d.r1);
+  return hap_pi2_alldiff_num(a, b, c, d) * (a.r1 * b.r1 * c.r1 * d.r1);
 }
 
 // ---- r^2 (pg_gpu/haplotype_kernels.py:35-46, ld_statistics.r_squared) ------
