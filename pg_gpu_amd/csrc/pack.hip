@@ -81,8 +81,8 @@ __global__ void __launch_bounds__(256) scan_sites_kernel(
 
 // ---------------------------------------------------------------------------
 // Pack: one workgroup per (PACK_SB site blocks, pop, 64-sample k-step).
-// Phase 1 reads 64 sample rows x 32 PACK_SB sites, 16 bytes per load (long
-// contiguous row segments: the rows of a population lie far apart), and stores them
+// Phase 1 reads 64 sample rows x 32 PACK_SB sites, 16 bytes per load (rows
+// not aligned to 16 bytes: two aligned loads and a byte funnel shift), and stores them
 // transposed ([site][sample]) in LDS; phase 2 gives each lane the 32 samples
 // of one site as two 16-byte LDS reads, turns them into 4-bit indicators with
 // bytewise (SWAR) tests, and writes 16 bytes per lane, fully coalesced.
@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256) pack_kernel(
     const int8_t* __restrict__ mat, int64_t n_sites, int64_t ld,
     const int32_t* __restrict__ rows, int32_t row_begin, int32_t pop_size, int32_t mode,
     int32_t n_feat, int32_t ksteps, int64_t pop_off, int64_t block_bytes, int64_t n_blocks,
-    int32_t vec_ok, int8_t* __restrict__ panels) {
+    const int8_t* __restrict__ mat_end, int8_t* __restrict__ panels) {
   constexpr int PITCH = 64 + 16;                              // bytes per site row
   constexpr int SITES = 32 * PACK_SB, GROUPS = SITES / 16;
   __shared__ __attribute__((aligned(16))) int8_t tile[SITES * PITCH];  // [site][sample]
@@ -121,9 +121,22 @@ __global__ void __launch_bounds__(256) pack_kernel(
     uint32_t w[4] = {0xFEFEFEFEu, 0xFEFEFEFEu, 0xFEFEFEFEu, 0xFEFEFEFEu};  // -2: padding
     if (k < pop_size) {
       const int8_t* src = mat + (int64_t)rows[row_begin + k] * ld + site0;
-      if (vec_ok && site0 + 16 <= n_sites) {
+      const int m = (int)(reinterpret_cast<uintptr_t>(src) & 15u);  // row misalignment
+      if (m == 0 && site0 + 16 <= n_sites) {
         const uint4 q = *reinterpret_cast<const uint4*>(src);
         w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
+      } else if (site0 + 16 <= n_sites && src - m >= mat && src - m + 32 <= mat_end) {
+        // two aligned 16-byte loads around the segment, then a funnel shift
+        const uint4* base = reinterpret_cast<const uint4*>(src - m);
+        const uint4 q0 = base[0], q1 = base[1];
+        const uint32_t d[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+        uint32_t t[7], u[5];
+#pragma unroll
+        for (int e = 0; e < 7; ++e) t[e] = (m & 4) ? d[e + 1] : d[e];
+#pragma unroll
+        for (int e = 0; e < 5; ++e) u[e] = (m & 8) ? t[e + 2] : t[e];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) w[e] = __builtin_amdgcn_alignbyte(u[e + 1], u[e], m & 3);
       } else {
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
@@ -224,15 +237,14 @@ extern "C" int pgld_pack(const int8_t* d_mat, int64_t n_rows, int64_t n_sites, i
     return fail(PGLD_EINVAL, "matrix shape disagrees with descriptor");
   if (L.n_blocks == 0) return PGLD_OK;
   hipStream_t s = as_stream(stream);
-  // 16-byte row loads need 16-byte aligned rows
-  const int32_t vec_ok = (reinterpret_cast<uintptr_t>(d_mat) % 16 == 0) && (ld % 16 == 0);
+  const int8_t* mat_end = d_mat + (n_rows > 0 ? (n_rows - 1) * ld + n_sites : 0);
   for (int p = 0; p < desc->n_pops; ++p) {
     if (L.ksteps[p] == 0) continue;
     hipLaunchKernelGGL(pack_kernel,
                        dim3((unsigned)((L.n_blocks + PACK_SB - 1) / PACK_SB), (unsigned)L.ksteps[p]),
                        dim3(256), 0, s, d_mat, n_sites, ld, d_rows, pop_offsets[p],
                        L.pop_size[p], desc->mode, L.n_feat[p], L.ksteps[p], L.pop_off[p],
-                       L.block_bytes, L.n_blocks, vec_ok, d_panels);
+                       L.block_bytes, L.n_blocks, mat_end, d_panels);
     PGLD_LAUNCH_CHECK();
   }
   return PGLD_OK;

@@ -131,6 +131,30 @@ def test_dense_counts_bit_exact(cuda, missing):
     np.testing.assert_array_equal(got.reshape(4, -1).T, c)
 
 
+@pytest.mark.parametrize("offset", [0, 3, 8])
+def test_dense_counts_from_unaligned_views(cuda, offset):
+    """Rows of a column-sliced view start off 16-byte boundaries (row stride
+    1,000 + 37 bytes, base offset): the pack kernel's funnel-shifted loads and
+    its byte path at the row ends must give the same counts as a copy."""
+    import torch
+
+    rng = np.random.default_rng(23 + offset)
+    K, W = 90, 1037
+    full = (rng.random((K, W)) < 0.4).astype(np.int8)
+    full[rng.random(full.shape) < 0.05] = -1
+    big = torch.as_tensor(full, device=cuda)
+    view = big[:, offset:offset + 999]
+    assert view.stride(1) == 1 and view.stride(0) == W
+    hap = full[:, offset:offset + 999]
+    pan = LDPanels(view, [list(range(0, K, 2)), list(range(1, K, 2))], "hap")
+    assert pan.ld == W
+    ii, jj = np.meshgrid(np.arange(999), np.arange(0, 999, 7), indexing="ij")
+    for p, rows in enumerate([list(range(0, K, 2)), list(range(1, K, 2))]):
+        got = pan.count_block(p, 0, 999, 0, 999).cpu().numpy()
+        c, _ = o.hap_counts(hap, ii.ravel(), jj.ravel(), rows)
+        np.testing.assert_array_equal(got[:, :, ::7].reshape(4, -1).T, c)
+
+
 def test_three_pops_missing(cuda):
     rng = np.random.default_rng(31)
     hap, pos, pops = small_pops(rng, 0, 300, 30_000, 0.03, sizes=[20, 45, 70])
