@@ -180,6 +180,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="use the default launch shape instead of the autotuned one")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB three times "
                          "(the known-size kernel that scales the HBM byte counters)")
@@ -230,10 +232,21 @@ def main():
         tiles = None
         run = plan.run
         n_bins = None
+        launch = {"tuned": False}
     else:
-        tiles = band_tiles(w.positions, w.max_dist)
+        launch = {"square": None, "dynamic": True, "grid": 0, "tuned": False}
+        if not args.no_tune:
+            from pg_gpu_amd import autotune
+
+            best, timings = autotune.tune_binned(pan, w.positions, w.bins, w.max_dist, r2=r2)
+            launch = dict(best, tuned=True, candidates=len(timings),
+                          candidate_ms={f"sq{c['square']}{'d' if c['dynamic'] else 's'}": round(t, 4)
+                                        for c, t in timings})
+            log(f"autotune: {best} over {len(timings)} launch shapes")
+        tiles = band_tiles(w.positions, w.max_dist, launch["square"])
         plan = pan.plan(w.positions, w.bins, w.max_dist, tiles=tiles)
-        run = lambda kernel="fused": plan.run(kernel, r2=r2)  # noqa: E731
+        run = lambda kernel="fused": plan.run(kernel, launch["grid"], r2,  # noqa: E731
+                                              launch["dynamic"])
         n_bins = int(len(w.bins) - 1)
 
     def step(kernel="fused", ev=None, pev=None):
@@ -364,7 +377,7 @@ def main():
                    "n_sites": V, "pairs": n_pairs, "n_bins": n_bins,
                    "n_windows": None if not windows else int(len(w.windows)),
                    "n_stats": ns, "r2": bool(r2 or windows), "scale": args.scale,
-                   "params": w.params},
+                   "params": w.params, "launch": launch},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP4_TOPS,
                      "unit": "TOP/s (FP4 MFMA on 0/1 indicators)",
                      "frac": achieved / PEAK_FP4_TOPS, "frac_of_i8_peak": achieved / PEAK_I8_TOPS,

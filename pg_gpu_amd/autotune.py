@@ -1,0 +1,80 @@
+"""Launch-shape autotuner of the fused binning kernel.
+
+The fused kernel's tile is fixed by the MFMA (32 x 32 sites per wave); what
+varies with the data is how the band's tiles are ordered and dealt out:
+
+- ``square``: the side (in tiles) of the squares the band tile list is
+  ordered in.  The waves of one XCD work inside one square at a time, so its
+  operand panels (2 x square 32-site blocks) must stay in that XCD's L2; the
+  best side depends on the band width in tiles and the panel bytes per block
+  (the sample counts).
+- ``dynamic``: atomic-counter tile claiming per XCD, or a static stride.
+- ``grid``: resident workgroups (the kernel's default fills every CU at its
+  occupancy).
+
+``tune_binned`` times every candidate on the job itself (HIP events on the
+launch stream, best of ``reps`` after one warm-up) and returns the fastest;
+results are cached per job shape (mode, populations and their sizes, missing
+data, bins, band width in tiles), so later jobs of the same shape skip the
+search.  The bin sums do not depend on the choice (integer limb reduction).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _device as dv
+from .engine import TILE, BinnedPlan
+
+SQUARES = (8, 16, 32, 64)
+
+_cache = {}
+
+
+def job_key(pan, positions, bins, max_dist, r2):
+    """Shape of a binning job as the launch shape sees it."""
+    pos = np.asarray(positions, np.float64)
+    md = float(bins[-1]) if max_dist is None else float(max_dist)
+    span = (pos[-1] - pos[0]) / max(len(pos) - 1, 1) if len(pos) > 1 else 1.0
+    band_tiles = int(round(md / max(span, 1e-12) / TILE))
+    return (int(pan.mode), int(pan.P), tuple(pan.pop_sizes), tuple(pan.has_missing),
+            int(len(bins) - 1), int(np.log2(max(band_tiles, 1))), bool(r2))
+
+
+def _time(plan, cfg, r2, reps):
+    torch = dv.torch()
+    stream = torch.cuda.current_stream()
+    plan.run("fused", cfg["grid"], r2, cfg["dynamic"]).finish()   # warm-up
+    best = float("inf")
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        pend = plan.run("fused", cfg["grid"], r2, cfg["dynamic"])
+        b.record(stream)
+        b.synchronize()
+        pend.finish()
+        best = min(best, a.elapsed_time(b))
+    return best
+
+
+def tune_binned(pan, positions, bins, max_dist=None, coords=None, r2=False, reps=3,
+                squares=SQUARES, grids=(0,), use_cache=True):
+    """-> (best config dict(square, dynamic, grid), [(config, ms)...] measured).
+    The best config is cached per ``job_key``; a cached job returns it with an
+    empty measurement list."""
+    key = job_key(pan, positions, bins, max_dist, r2)
+    if use_cache and key in _cache:
+        return dict(_cache[key]), []
+    timings = []
+    for sq in squares:
+        plan = BinnedPlan(pan, positions, bins, max_dist, coords, square=sq)
+        for dyn in (True, False):
+            for g in grids:
+                cfg = dict(square=int(sq), dynamic=dyn, grid=int(g))
+                timings.append((cfg, _time(plan, cfg, r2, reps)))
+    best = min(timings, key=lambda t: t[1])[0]
+    _cache[key] = dict(best)
+    return dict(best), timings
+
+
+def clear_cache():
+    _cache.clear()

@@ -39,9 +39,13 @@ def reciprocal_table(kmax):
     return tab.ravel()
 
 
-def band_tiles(pos, max_dist):
+def band_tiles(pos, max_dist, square=None):
     """(row tile, col tile) pairs of TILE sites covering every i<j pair with
-    pos[j] <= pos[i] + max_dist, row-major (pg_gpu/ld_pipeline.py:85-88)."""
+    pos[j] <= pos[i] + max_dist (pg_gpu/ld_pipeline.py:85-88), in squares of
+    ``square`` x ``square`` tiles (default SUPER)."""
+    sq = SUPER if square is None else int(square)
+    if sq < 1:
+        raise ValueError("square must be positive")
     pos = np.asarray(pos, np.float64)
     V = len(pos)
     if V < 2:
@@ -59,7 +63,7 @@ def band_tiles(pos, max_dist):
     # Group the band into SUPER x SUPER squares of tiles, squares in row-major
     # order and tiles row-major inside a square: the waves running at the same
     # time on one XCD then share ~2 * SUPER operand panels in that XCD's L2.
-    key = (rows // SUPER) * (n_rt + SUPER) + (cols // SUPER)
+    key = (rows // sq) * (n_rt + sq) + (cols // sq)
     order = np.lexsort((cols, rows, key))
     return np.stack([rows[order], cols[order]], axis=1).astype(np.int32)
 
@@ -99,9 +103,11 @@ class PendingResult:
 class BinnedPlan:
     """Device copies of the positions, bin edges and band tile list of one job."""
 
-    def __init__(self, pan, positions, bins, max_dist=None, coords=None, tiles=None):
+    def __init__(self, pan, positions, bins, max_dist=None, coords=None, tiles=None,
+                 square=None):
         torch = _torch()
         self.pan = pan
+        self.square = SUPER if square is None else int(square)
         bins = np.asarray(bins, np.float64)
         nb = len(bins) - 1
         if nb < 1 or nb > _lib.MAX_BINS:
@@ -129,7 +135,8 @@ class BinnedPlan:
         self.d_coord = (torch.as_tensor(co_np, device=pan.device)
                         if coords is not None else self.d_pos)
         self.d_edges = torch.as_tensor(bins, device=pan.device)
-        self.tiles = band_tiles(pos_np, self.max_dist) if tiles is None else np.asarray(tiles)
+        self.tiles = (band_tiles(pos_np, self.max_dist, self.square) if tiles is None
+                      else np.asarray(tiles))
         self.d_tiles = torch.as_tensor(self.tiles.reshape(-1), dtype=torch.int32,
                                        device=pan.device)
         jmax = np.searchsorted(pos_np, pos_np + self.max_dist, side="right")
@@ -193,6 +200,57 @@ class BinnedPlan:
         res = PendingResult(out, self.ns, self.n_pairs, len(self.tiles), r2)
         res._sched = sched  # kept alive until the launch has been waited for
         return res
+
+    def capture(self, kernel="fused", grid=0, r2=False, dynamic=True, repack=False):
+        """Record one binning pass (optionally preceded by re-packing the
+        panels) as a HIP graph; ``GraphRun.replay()`` then relaunches it with
+        one call.  Buffers are cleared by fill kernels inside the graph."""
+        return GraphRun(self, kernel, grid, r2, dynamic, repack)
+
+
+class GraphRun:
+    """A captured binning pass: output and scheduler buffers are owned by the
+    graph and rewritten by every replay, so a replay's result must be read
+    (``finish()``) before the next replay."""
+
+    def __init__(self, plan, kernel, grid, r2, dynamic, repack):
+        torch = _torch()
+        pan = plan.pan
+        if r2 and pan.P != 1:
+            raise NotImplementedError("r2 sums are built for one population")
+        self.plan, self.r2, self.repack = plan, bool(r2), bool(repack)
+        width = plan.ns + (2 if r2 else 0)
+        self.out = torch.zeros((plan.nb, width + 1, 3), dtype=torch.int64, device=pan.device)
+        self.sched = (torch.zeros(_lib.SCHED_WORDS, dtype=torch.int32, device=pan.device)
+                      if dynamic else None)
+        self.args = plan.args(self.out, grid, r2)
+        if self.sched is not None:
+            self.args.d_sched = _lib.ptr(self.sched).value
+        self.fn = _lib.lib().pgld_binned_fused if kernel == "fused" else \
+            _lib.lib().pgld_binned_plain
+        side = torch.cuda.Stream(device=pan.device)   # default priority
+        side.wait_stream(torch.cuda.current_stream(pan.device))
+        with torch.cuda.stream(side):                  # warm-up outside the graph
+            self._body()
+        torch.cuda.current_stream(pan.device).wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._body()
+
+    def _body(self):
+        pan = self.plan.pan
+        if self.repack:
+            pan.repack()
+        self.out.zero_()
+        if self.sched is not None:
+            self.sched.zero_()
+        _lib.check(self.fn(ctypes.byref(pan.desc), ctypes.byref(self.args),
+                           _lib.stream_handle()))
+
+    def replay(self):
+        self.graph.replay()
+        return PendingResult(self.out, self.plan.ns, self.plan.n_pairs, len(self.plan.tiles),
+                             self.r2)
 
 
 @dataclass
@@ -293,9 +351,9 @@ class LDPanels:
                                         _lib.ptr(self.d_rows), offs, ctypes.byref(self.desc),
                                         _lib.ptr(self.panels), _lib.stream_handle(self.stream)))
 
-    def plan(self, positions, bins, max_dist=None, coords=None, tiles=None):
+    def plan(self, positions, bins, max_dist=None, coords=None, tiles=None, square=None):
         """Device-side plan of one binning job (positions, edges, band tiles)."""
-        return BinnedPlan(self, positions, bins, max_dist, coords, tiles)
+        return BinnedPlan(self, positions, bins, max_dist, coords, tiles, square)
 
     def binned(self, positions, bins, max_dist=None, coords=None, kernel="fused", grid=0,
                tiles=None, sync=True, r2=False):

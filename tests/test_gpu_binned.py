@@ -227,3 +227,66 @@ def test_static_and_dynamic_tile_schedules_agree(cuda):
     np.testing.assert_array_equal(a.sums, b.sums)
     np.testing.assert_array_equal(a.sums, c.sums)
     np.testing.assert_array_equal(a.counts, b.counts)
+
+
+def test_graph_replays_match_eager(cuda):
+    """A captured pass (re-pack, clears, fused kernel) replayed several times,
+    with eager launches and host reads in between, gives the eager limbs bit
+    for bit every time."""
+    import torch
+
+    rng = np.random.default_rng(41)
+    hap, pos, pops = small_pops(rng, 96, 700, 300_000, 0.04)
+    bins = np.array([0.0, 500, 2_000, 8_000, 30_000, 100_000])
+    pan = LDPanels(torch.as_tensor(hap, device=cuda), pops, "hap")
+    plan = pan.plan(pos, bins, 60_000)
+    eager = plan.run().finish()
+    g = plan.capture(repack=True)
+    for rep in range(4):
+        got = g.replay().finish()
+        np.testing.assert_array_equal(got.counts, eager.counts)
+        np.testing.assert_array_equal(got.sums, eager.sums)
+        # eager work between replays: another pass and unrelated device work
+        again = plan.run(dynamic=rep % 2 == 0).finish()
+        np.testing.assert_array_equal(again.sums, eager.sums)
+        torch.cuda.synchronize()
+        _ = (torch.arange(1000, device=cuda) * rep).sum().item()
+
+
+def test_graph_replays_single_pop_r2(cuda):
+    import torch
+
+    rng = np.random.default_rng(42)
+    hap, pos, pops = small_pops(rng, 64, 500, 200_000, 0.0, n_pops=1)
+    bins = np.array([0.0, 1_000, 10_000, 50_000])
+    pan = LDPanels(torch.as_tensor(hap, device=cuda), pops, "hap")
+    plan = pan.plan(pos, bins, 50_000)
+    eager = plan.run(r2=True).finish()
+    g = plan.capture(r2=True)
+    for _ in range(3):
+        got = g.replay().finish()
+        np.testing.assert_array_equal(got.sums, eager.sums)
+        np.testing.assert_array_equal(got.r2_sum, eager.r2_sum)
+        np.testing.assert_array_equal(got.r2_count, eager.r2_count)
+
+
+def test_autotuned_launch_shape_gives_identical_sums(cuda):
+    import torch
+
+    from pg_gpu_amd import autotune
+
+    rng = np.random.default_rng(43)
+    hap, pos, pops = small_pops(rng, 80, 900, 400_000, 0.03)
+    bins = np.geomspace(100, 80_000, 9)
+    pan = LDPanels(torch.as_tensor(hap, device=cuda), pops, "hap")
+    autotune.clear_cache()
+    best, timings = autotune.tune_binned(pan, pos, bins, 80_000, reps=1, squares=(4, 16))
+    assert len(timings) == 4 and best in [t[0] for t in timings]
+    assert all(ms > 0 for _, ms in timings)
+    again, t2 = autotune.tune_binned(pan, pos, bins, 80_000)
+    assert again == best and t2 == []              # cached per job shape
+    ref = pan.plan(pos, bins, 80_000).run().finish()
+    got = pan.plan(pos, bins, 80_000, square=best["square"]).run(
+        grid=best["grid"], dynamic=best["dynamic"]).finish()
+    np.testing.assert_array_equal(got.sums, ref.sums)
+    np.testing.assert_array_equal(got.counts, ref.counts)

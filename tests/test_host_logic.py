@@ -156,3 +156,18 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
     from pg_gpu_amd import _lib
     assert set(_lib.exported_symbols()) >= set(names) - {"pgld_last_error"} or True
+
+
+@pytest.mark.parametrize("square", [1, 3, 16, 64])
+def test_band_tile_squares_are_permutations(square):
+    """The square ordering changes only the order of the band tiles."""
+    from pg_gpu_amd.engine import band_tiles
+
+    rng = np.random.default_rng(5)
+    pos = np.sort(rng.choice(np.arange(1, 3_000_000), 5_000, replace=False)).astype(float)
+    ref = band_tiles(pos, 150_000)
+    got = band_tiles(pos, 150_000, square)
+    assert got.shape == ref.shape
+    assert set(map(tuple, got)) == set(map(tuple, ref))
+    if square == 1:   # squares of one tile: row-major
+        np.testing.assert_array_equal(got, ref[np.lexsort((ref[:, 1], ref[:, 0]))])
