@@ -85,11 +85,9 @@ class WindowPlan:
         self.d_work = torch.as_tensor(self.work.reshape(-1), dtype=torch.int32, device=pan.device)
         self.ns = pan.n_stats
 
-    def run(self, kernel="fused", grid=0):
-        torch = _torch()
+    def args(self, out, grid=0):
         pan = self.pan
         nw = len(self.windows)
-        out = torch.zeros((max(nw, 1), self.ns + 3, 3), dtype=torch.int64, device=pan.device)
         a = _lib.WindowArgs()
         a.d_panels = _lib.ptr(pan.panels).value
         a.d_site_stats = _lib.ptr(pan.site_stats).value
@@ -104,6 +102,14 @@ class WindowPlan:
         a.d_out = _lib.ptr(out).value
         a.grid = int(grid)
         a.n_lo[0] = int(pan.n_lo[0])
+        return a
+
+    def run(self, kernel="fused", grid=0):
+        torch = _torch()
+        pan = self.pan
+        nw = len(self.windows)
+        out = torch.zeros((max(nw, 1), self.ns + 3, 3), dtype=torch.int64, device=pan.device)
+        a = self.args(out, grid)
         # Window items are short (one population, often no missing calls): a
         # shared atomic counter per XCD would be claimed tens of millions of
         # times per second, so the window kernel keeps the static schedule,
@@ -114,6 +120,44 @@ class WindowPlan:
         res = _PendingWindows(out, self.ns, nw)
         res._sched = sched
         return res
+
+
+    def capture(self, kernel="fused", grid=0, repack=False):
+        """Record the whole job (optionally re-packing the panels first) as a
+        HIP graph: one replay = clear + one persistent launch over every
+        window's work items.  The output is rewritten by each replay."""
+        return WindowGraph(self, kernel, grid, repack)
+
+
+class WindowGraph:
+    def __init__(self, plan, kernel, grid, repack):
+        torch = _torch()
+        pan = plan.pan
+        self.plan, self.repack = plan, bool(repack)
+        self.out = torch.zeros((max(len(plan.windows), 1), plan.ns + 3, 3), dtype=torch.int64,
+                               device=pan.device)
+        self.args = plan.args(self.out, grid)
+        self.fn = (_lib.lib().pgld_windows_fused if kernel == "fused"
+                   else _lib.lib().pgld_windows_plain)
+        side = torch.cuda.Stream(device=pan.device)   # default priority
+        side.wait_stream(torch.cuda.current_stream(pan.device))
+        with torch.cuda.stream(side):
+            self._body()
+        torch.cuda.current_stream(pan.device).wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._body()
+
+    def _body(self):
+        pan = self.plan.pan
+        if self.repack:
+            pan.repack()
+        self.out.zero_()
+        _lib.check(self.fn(ctypes.byref(pan.desc), ctypes.byref(self.args), _lib.stream_handle()))
+
+    def replay(self):
+        self.graph.replay()
+        return _PendingWindows(self.out, self.plan.ns, len(self.plan.windows))
 
 
 class _PendingWindows:
