@@ -53,6 +53,7 @@ struct FusedParams {
   int32_t rt_off[PGLD_MAX_POPS];  // first entry of pop p in the LDS table
   int32_t rt_len;                 // total LDS table entries
   uint32_t* sched;                // per-XCD tile counters (null: static schedule)
+  int32_t use_ring;               // stream operands through the LDS ring (traits with RING)
 };
 
 // 16 packed count words per lane (one per MFMA output slot).  Kept as a
@@ -92,15 +93,95 @@ __device__ __forceinline__ void mfma_row_pass(const v4i* R, const v4i* C1, const
   }
 }
 
+// Operand ring in LDS: each wave streams its tile's fragments through a
+// private ring of RING k-steps x 4 fragments (1 KiB each) with
+// global_load_lds (lane l's 16 bytes land at byte 16 l of the slot), so loads
+// run RING k-steps ahead without holding registers.  The LDS writes of these
+// loads are tracked by vmcnt only, hence the explicit waits.
+typedef __attribute__((address_space(1))) void gmem_void;
+typedef __attribute__((address_space(3))) void lmem_void;
+
+__device__ __forceinline__ void lds_dma16(const v4i* g_lane, v4i* lds_slot) {
+  __builtin_amdgcn_global_load_lds((gmem_void*)g_lane, (lmem_void*)lds_slot, 16, 0, 0);
+}
+// Wait until at most n (0, 4, 8, 12 or 16) vector-memory operations of this
+// wave are outstanding; also a compiler barrier for memory accesses.
+__device__ __forceinline__ void wait_vm(int n) {
+  if (n >= 16) asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+  else if (n >= 12) asm volatile("s_waitcnt vmcnt(12)" ::: "memory");
+  else if (n >= 8) asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+  else if (n >= 4) asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+  else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+// The four fragments of one ring slot (1 KiB apart).  Read in one asm block
+// that waits for its own results: the compiler cannot tell these reads from
+// the in-flight ring loads of later k-steps and would otherwise wait for all
+// of them (vmcnt(0)) before every read.
+__device__ __forceinline__ void ring_read4(const v4i* slot_lane, v4i& f0, v4i& f1, v4i& f2,
+                                           v4i& f3) {
+  typedef const __attribute__((address_space(3))) v4i lds_v4i;
+  const uint32_t addr = (uint32_t)(uintptr_t)(lds_v4i*)slot_lane;  // LDS byte address
+  asm volatile(
+      "ds_read_b128 %0, %4\n\t"
+      "ds_read_b128 %1, %4 offset:1024\n\t"
+      "ds_read_b128 %2, %4 offset:2048\n\t"
+      "ds_read_b128 %3, %4 offset:3072\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(f0), "=&v"(f1), "=&v"(f2), "=&v"(f3)
+      : "v"(addr)
+      : "memory");
+}
+
+// hh, hv, vh, vv over the ks k-steps, operands streamed through the ring
+template <int RING>
+__device__ __forceinline__ void ring_products(const v4i* A, const v4i* Av, const v4i* B,
+                                              const v4i* Bv, int ks, int lane, v4i* ring,
+                                              v16f& hh, v16f& hv, v16f& vh, v16f& vv) {
+  const v4i* src[4] = {A, Av, B, Bv};
+  const int pre = ks < RING ? ks : RING;
+  for (int d = 0; d < pre; ++d) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f) lds_dma16(src[f] + d * 64, ring + (d * 4 + f) * 64);
+  }
+  for (int s = 0; s < ks; ++s) {
+    const int ahead = ks - 1 - s < RING - 1 ? ks - 1 - s : RING - 1;  // later k-steps in flight
+    wait_vm(4 * ahead);
+    v4i ah, av, bh, bv;
+    ring_read4(ring + (s % RING) * 4 * 64 + lane, ah, av, bh, bv);
+    hh = pgld_mfma_fp4(ah, bh, hh);
+    hv = pgld_mfma_fp4(ah, bv, hv);
+    vh = pgld_mfma_fp4(av, bh, vh);
+    vv = pgld_mfma_fp4(av, bv, vv);
+    if (s + RING < ks) {
+      asm volatile("" ::: "memory");
+      v4i* dst = ring + (s % RING) * 4 * 64;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) lds_dma16(src[f] + (s + RING) * 64, dst + f * 64);
+    }
+  }
+}
+
 // Counts of population p for tile (rb, cb), packed per slot as
-// w0 = n11 | n10 << 16, w1 = n01 | n00 << 16.
-template <bool TWO_PASS>
+// w0 = n11 | n10 << 16, w1 = n01 | n00 << 16.  With RING > 0 (and missing
+// calls) the operands stream through the wave's LDS ring.
+template <bool TWO_PASS, int RING = 0>
 __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t rb, int64_t cb,
-                                            int lane, v16u& w0, v16u& w1) {
+                                            int lane, v16u& w0, v16u& w1, v4i* ring = nullptr) {
   const int ks = a.ksteps[p];
   const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[p]) + lane;
   const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[p]) + lane;
-  if (a.missing[p] && !TWO_PASS) {
+  if (RING > 0 && ring != nullptr && a.missing[p] && !TWO_PASS) {
+    v16f hh = zero16f(), hv = zero16f(), vh = zero16f(), vv = zero16f();
+    ring_products<(RING > 0 ? RING : 1)>(A, A + ks * 64, B, B + ks * 64, ks, lane, ring, hh, hv,
+                                         vh, vv);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int n11 = (int)hh[r], h_v = (int)hv[r], v_h = (int)vh[r], v_v = (int)vv[r];
+      w0[r] = (uint32_t)n11 | ((uint32_t)(h_v - n11) << 16);
+      w1[r] = (uint32_t)(v_h - n11) | ((uint32_t)(v_v - h_v - v_h + n11) << 16);
+    }
+  } else if (a.missing[p] && !TWO_PASS) {
     v16f hh = zero16f(), hv = zero16f(), vh = zero16f(), vv = zero16f();
     const v4i* Av = A + ks * 64;
     const v4i* Bv = B + ks * 64;
@@ -331,11 +412,15 @@ struct HapTraits {
   static constexpr int GS = NS < STATS_GROUP ? NS : STATS_GROUP;  // statistics per group
   static constexpr int NSO = NS + (R2 ? 2 : 0);
   static constexpr int NGT = NG + (R2 ? 1 : 0);
+  // k-steps of the LDS operand ring per wave (0: register ring)
+  static constexpr int RING = P >= 3 ? 4 : 0;
+  static constexpr int RING_BYTES = RING * 4 * 1024;  // per wave
   struct Counts { v16u w0[P], w1[P]; };
   __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane,
-                                                Counts& c) {
+                                                Counts& c, v4i* ring) {
 #pragma unroll
-    for (int p = 0; p < P; ++p) tile_counts<(MINB >= 3)>(a, p, rb, cb, lane, c.w0[p], c.w1[p]);
+    for (int p = 0; p < P; ++p)
+      tile_counts<(MINB >= 3), RING>(a, p, rb, cb, lane, c.w0[p], c.w1[p], ring);
   }
   // statistics G*GS .. G*GS+GS-1 of the pair in accumulator slot r
   template <int G>
@@ -389,9 +474,11 @@ struct GenoTraits {
   static constexpr int GS = NS < STATS_GROUP ? NS : STATS_GROUP;
   static constexpr int NSO = NS + (R2 ? 2 : 0);
   static constexpr int NGT = NG + (R2 ? 1 : 0);
+  static constexpr int RING = 0;
+  static constexpr int RING_BYTES = 0;
   struct Counts { v16u w[P][W]; };
   __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane,
-                                                Counts& c) {
+                                                Counts& c, v4i*) {
 #pragma unroll
     for (int p = 0; p < P; ++p) geno_tile_counts<W>(a, p, rb, cb, lane, c.w[p]);
   }
@@ -685,6 +772,12 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
   int* scnt = reinterpret_cast<int*>(slimb + nb * (NS + 1) * 3);    // [4 waves][nb]
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // this wave's operand ring (16-byte aligned, after the tables)
+  v4i* ring = nullptr;
+  if (T::RING > 0 && a.use_ring) {
+    const uintptr_t end = reinterpret_cast<uintptr_t>(scnt + 4 * nb);
+    ring = reinterpret_cast<v4i*>((end + 15) & ~uintptr_t(15)) + wave * (T::RING_BYTES / 16);
+  }
   for (int e = tid; e <= nb; e += blockDim.x) sedges[e] = a.edges[e];
   for (int p = 0; p < T::NP; ++p) {
     const int len = a.K[p] - a.rt_lo[p] + 1;
@@ -723,7 +816,7 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
     if (ctr && lane == 0) claim = atomicAdd(ctr, 1u);  // the next tile, claimed early
     const int64_t rb = a.tiles[2 * tile], cb = a.tiles[2 * tile + 1];
     typename T::Counts cts;
-    T::counts(a, rb, cb, lane, cts);
+    T::counts(a, rb, cb, lane, cts, ring);
 
     // ---- tile classification (wave-uniform).  Coordinates are
     // non-decreasing along the sites, so the corner pairs (i1, j0) and
@@ -797,11 +890,15 @@ static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStre
     if (p < P) off += L.pop_size[p] - lo + 1;
   }
   a.rt_len = off;
-  const size_t lds = (PGLD_MAX_BINS + 8) * sizeof(double) + (size_t)off * 4 * sizeof(double) +
-                     (size_t)4 * g->n_bins * T::GS * sizeof(double) +
-                     (size_t)g->n_bins * (NS + 1) * 3 * sizeof(unsigned long long) +
-                     (size_t)4 * g->n_bins * sizeof(int);
+  size_t lds = (PGLD_MAX_BINS + 8) * sizeof(double) + (size_t)off * 4 * sizeof(double) +
+               (size_t)4 * g->n_bins * T::GS * sizeof(double) +
+               (size_t)g->n_bins * (NS + 1) * 3 * sizeof(unsigned long long) +
+               (size_t)4 * g->n_bins * sizeof(int);
   if (lds > 160 * 1024) return fail(PGLD_EINVAL, "fused kernel: LDS tables exceed 160 KiB");
+  // the operand ring only where the tables leave room for it
+  const size_t ring_bytes = T::RING > 0 ? 16 + (size_t)4 * T::RING_BYTES : 0;
+  a.use_ring = ring_bytes > 0 && lds + ring_bytes <= 160 * 1024;
+  if (a.use_ring) lds += ring_bytes;
   PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(fused_kernel<T>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int grid = g->grid > 0 ? g->grid : T::MINB * num_cus();
@@ -907,7 +1004,7 @@ __global__ void __launch_bounds__(256, T::MINB) window_kernel(WindowParams w) {
     const int64_t win = w.work[3 * k], rb = w.work[3 * k + 1], cb = w.work[3 * k + 2];
     const int64_t lo = w.win[2 * win], hi = w.win[2 * win + 1];
     typename T::Counts cts;
-    T::counts(a, rb, cb, lane, cts);
+    T::counts(a, rb, cb, lane, cts, nullptr);  // one population: no LDS ring
     const int64_t i0 = rb * 32, j0 = cb * 32;
     const bool full = cb > rb && i0 >= lo && j0 + 32 <= hi;
     window_groups<T, 0>(w, srt, cts, lane, full, i0, j0, lo, hi,

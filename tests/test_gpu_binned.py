@@ -161,6 +161,15 @@ def test_three_pops_missing(cuda):
     run_case(cuda, hap, pos, pops, synth.log_bins(100, 12_000, 6), 12_000.0)
 
 
+def test_four_pops_many_bins(cuda):
+    """36 bins x 106 outputs of limbs leave no LDS room for the operand ring:
+    the fused kernel falls back to register-streamed operands."""
+    rng = np.random.default_rng(32)
+    hap, pos, pops = small_pops(rng, 0, 260, 26_000, 0.04, sizes=[12, 30, 18, 44])
+    run_case(cuda, hap, pos, pops, synth.log_bins(50, 10_000, 36), 10_000.0,
+             kernels=("fused",))
+
+
 def test_four_pops_workload_c4_scaled(cuda):
     """40/120/300/800 haplotypes, 3% missing, rare-allele sites: the 105-statistic set."""
     w = synth.make("c4", scale=0.0008)
