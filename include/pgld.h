@@ -138,7 +138,10 @@ int pgld_n_stats(int32_t mode, int32_t n_pops);
  * n_stats + 2 with with_r2 (r^2 sum and defined-pair count at n_stats and
  * n_stats + 1; haplotype r^2 as ld_statistics.r_squared,
  * pg_gpu/ld_statistics.py:190-217, genotype r^2 the pairwise-complete dosage
- * Pearson r^2, equal to Rogers-Huff r^2 without missing calls, :492-521). */
+ * Pearson r^2, equal to Rogers-Huff r^2 without missing calls, :492-521).
+ * Where the per-bin limbs of all bins do not fit in LDS (four populations,
+ * more than ~50 bins) the bins are covered by consecutive launches over bin
+ * ranges, with the static tile schedule. */
 int pgld_binned_fused(const pgld_panel_desc* desc, const pgld_binned_args* args, void* stream);
 
 /* Arguments of the ragged-window kernels (one population). */

@@ -24,6 +24,7 @@
 // With two waves per SIMD (haplotypes, and genotypes of one population) one
 // wave's MFMA/loads overlap the other's FP64 epilogue; two-population
 // genotype sets need the full register file of a one-wave-per-SIMD launch.  No pair list and no per-pair statistic ever reaches HBM.
+#include <algorithm>
 #include "pgld_internal.h"
 #include "stats_hap.inc"
 #include "stats_geno.inc"
@@ -890,24 +891,38 @@ static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStre
     if (p < P) off += L.pop_size[p] - lo + 1;
   }
   a.rt_len = off;
-  size_t lds = (PGLD_MAX_BINS + 8) * sizeof(double) + (size_t)off * 4 * sizeof(double) +
-               (size_t)4 * g->n_bins * T::GS * sizeof(double) +
-               (size_t)g->n_bins * (NS + 1) * 3 * sizeof(unsigned long long) +
-               (size_t)4 * g->n_bins * sizeof(int);
-  if (lds > 160 * 1024) return fail(PGLD_EINVAL, "fused kernel: LDS tables exceed 160 KiB");
-  // the operand ring only where the tables leave room for it
-  const size_t ring_bytes = T::RING > 0 ? 16 + (size_t)4 * T::RING_BYTES : 0;
-  a.use_ring = ring_bytes > 0 && lds + ring_bytes <= 160 * 1024;
-  if (a.use_ring) lds += ring_bytes;
-  PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(fused_kernel<T>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  constexpr size_t kLds = 160 * 1024;
+  const size_t fixed = (PGLD_MAX_BINS + 8) * sizeof(double) + (size_t)off * 4 * sizeof(double);
+  const size_t per_bin = (size_t)4 * T::GS * sizeof(double) +
+                         (size_t)(NS + 1) * 3 * sizeof(unsigned long long) + 4 * sizeof(int);
+  if (fixed + per_bin > kLds) return fail(PGLD_EINVAL, "fused kernel: LDS tables exceed 160 KiB");
+  // Bins whose per-workgroup limbs do not fit in LDS together are covered by
+  // consecutive launches over ranges of bins (each re-counts the band; pairs
+  // outside a launch's edges are skipped by its binning).  Those launches use
+  // the static tile schedule: the caller's scheduler words serve one launch.
+  const int bins_per = (int)std::min<size_t>((kLds - fixed) / per_bin, (size_t)g->n_bins);
+  const int n_launch = (g->n_bins + bins_per - 1) / bins_per;
   int grid = g->grid > 0 ? g->grid : T::MINB * num_cus();
   grid = ((grid + 7) / 8) * 8;
   const int64_t need = ((g->n_tiles + 31) / 32) * 8;  // at least 4 tiles per wave
   if (grid > need) grid = (int)need;
   if (grid < 8) grid = 8;
-  hipLaunchKernelGGL(fused_kernel<T>, dim3(grid), dim3(256), lds, s, a);
-  PGLD_LAUNCH_CHECK();
+  for (int b0 = 0; b0 < g->n_bins; b0 += bins_per) {
+    FusedParams c = a;
+    c.n_bins = std::min(bins_per, g->n_bins - b0);
+    c.edges = g->d_edges + b0;
+    c.out = a.out + (int64_t)b0 * (NS + 1) * 3;
+    if (n_launch > 1) c.sched = nullptr;
+    size_t lds = fixed + (size_t)c.n_bins * per_bin;
+    // the operand ring only where the tables leave room for it
+    const size_t ring_bytes = T::RING > 0 ? 16 + (size_t)4 * T::RING_BYTES : 0;
+    c.use_ring = ring_bytes > 0 && lds + ring_bytes <= kLds;
+    if (c.use_ring) lds += ring_bytes;
+    PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(fused_kernel<T>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(fused_kernel<T>, dim3(grid), dim3(256), lds, s, c);
+    PGLD_LAUNCH_CHECK();
+  }
   return PGLD_OK;
 }
 

@@ -170,6 +170,14 @@ def test_four_pops_many_bins(cuda):
              kernels=("fused",))
 
 
+def test_four_pops_64_bins_split_launches(cuda):
+    """64 bins x 106 outputs of limbs exceed LDS: the fused entry point covers
+    the bins in consecutive launches over bin ranges."""
+    rng = np.random.default_rng(33)
+    hap, pos, pops = small_pops(rng, 0, 240, 24_000, 0.03, sizes=[10, 26, 20, 40])
+    run_case(cuda, hap, pos, pops, synth.log_bins(20, 9_000, 64), 9_000.0)
+
+
 def test_four_pops_workload_c4_scaled(cuda):
     """40/120/300/800 haplotypes, 3% missing, rare-allele sites: the 105-statistic set."""
     w = synth.make("c4", scale=0.0008)
