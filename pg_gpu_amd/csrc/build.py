@@ -26,6 +26,9 @@ FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unknown-warning-option",
+    # a wave-uniform atomic returns its value to one lane; the optimizer's
+    # rewrite would make every lane wait for it at once
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
 ]
 
 

@@ -811,13 +811,27 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
   const int64_t seg_lo = a.n_tiles * xcd / nx, seg_hi = a.n_tiles * (xcd + 1) / nx;
   uint32_t* ctr = a.sched ? a.sched + xcd * 32 : nullptr;
 
-  for (int64_t local = (blockIdx.x / nx) * 4 + wave; seg_lo + local < seg_hi;) {
-    const int64_t tile = seg_lo + local;
+  // The next tile is claimed at the start of the current one and its list
+  // entry loaded after the current counts: neither round trip is waited for
+  // at a tile start.
+  int64_t local = (blockIdx.x / nx) * 4 + wave;
+  int32_t nrb = 0, ncb = 0;
+  if (seg_lo + local < seg_hi) {
+    nrb = a.tiles[2 * (seg_lo + local)];
+    ncb = a.tiles[2 * (seg_lo + local) + 1];
+  }
+  while (seg_lo + local < seg_hi) {
+    const int64_t rb = __builtin_amdgcn_readfirstlane(nrb), cb = __builtin_amdgcn_readfirstlane(ncb);
     uint32_t claim = 0;
-    if (ctr && lane == 0) claim = atomicAdd(ctr, 1u);  // the next tile, claimed early
-    const int64_t rb = a.tiles[2 * tile], cb = a.tiles[2 * tile + 1];
+    if (ctr && lane == 0) claim = atomicAdd(ctr, 1u);  // the next tile
     typename T::Counts cts;
     T::counts(a, rb, cb, lane, cts, ring);
+    const int64_t next = ctr ? waves_per_xcd + (int64_t)__builtin_amdgcn_readfirstlane(__shfl(claim, 0, 64))
+                             : local + waves_per_xcd;
+    if (seg_lo + next < seg_hi) {
+      nrb = a.tiles[2 * (seg_lo + next)];
+      ncb = a.tiles[2 * (seg_lo + next) + 1];
+    }
 
     // ---- tile classification (wave-uniform).  Coordinates are
     // non-decreasing along the sites, so the corner pairs (i1, j0) and
@@ -833,8 +847,7 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
     t.blo = t.bmin < 0 ? 0 : t.bmin;
     t.dual = !t.fast && t.blo < nb && (t.bmax >= nb ? nb - 1 : t.bmax) - t.blo <= 1;
     tile_groups<T, 0>(a, t, cts);
-    local = ctr ? waves_per_xcd + (int64_t)__builtin_amdgcn_readfirstlane(__shfl(claim, 0, 64))
-                : local + waves_per_xcd;
+    local = next;
   }
   // nothing may stay staged
   stage_drain<NS, GS>(t.stg, t.scw, slimb, lane, 0, nb - 1, 0);
