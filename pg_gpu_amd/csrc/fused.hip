@@ -824,6 +824,11 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
     const int64_t rb = __builtin_amdgcn_readfirstlane(nrb), cb = __builtin_amdgcn_readfirstlane(ncb);
     uint32_t claim = 0;
     if (ctr && lane == 0) claim = atomicAdd(ctr, 1u);  // the next tile
+    // corner coordinates for the classification, loaded before the counts
+    const int64_t i0 = rb * 32, j0 = cb * 32;
+    const int64_t i1 = min(i0 + 31, a.V - 1), j1 = min(j0 + 31, a.V - 1);
+    const double d_min = coord_diff(a, i1, j0), d_max = coord_diff(a, i0, j1);
+    const bool in_cut = a.pos[j1] <= a.pos[i0] + a.max_dist;
     typename T::Counts cts;
     T::counts(a, rb, cb, lane, cts, ring);
     const int64_t next = ctr ? waves_per_xcd + (int64_t)__builtin_amdgcn_readfirstlane(__shfl(claim, 0, 64))
@@ -836,13 +841,11 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
     // ---- tile classification (wave-uniform).  Coordinates are
     // non-decreasing along the sites, so the corner pairs (i1, j0) and
     // (i0, j1) bound the bins of every pair in the tile.
-    const int64_t i0 = rb * 32, j0 = cb * 32;
-    const int64_t i1 = min(i0 + 31, a.V - 1), j1 = min(j0 + 31, a.V - 1);
     t.i0 = i0;
     t.j0 = j0;
-    t.bmin = pgld_bin_scan(coord_diff(a, i1, j0), sedges, nb, 0, nb - 1);
-    t.bmax = pgld_bin_scan(coord_diff(a, i0, j1), sedges, nb, t.bmin < 0 ? 0 : t.bmin, nb - 1);
-    t.fast = (cb > rb) && (j1 == j0 + 31) && (a.pos[j1] <= a.pos[i0] + a.max_dist) &&
+    t.bmin = pgld_bin_scan(d_min, sedges, nb, 0, nb - 1);
+    t.bmax = pgld_bin_scan(d_max, sedges, nb, t.bmin < 0 ? 0 : t.bmin, nb - 1);
+    t.fast = (cb > rb) && (j1 == j0 + 31) && in_cut &&
              t.bmin == t.bmax && t.bmin >= 0 && t.bmin < nb;
     t.blo = t.bmin < 0 ? 0 : t.bmin;
     t.dual = !t.fast && t.blo < nb && (t.bmax >= nb ? nb - 1 : t.bmax) - t.blo <= 1;
