@@ -199,12 +199,23 @@ def emit_acc_set(P):
                 key = tuple(sorted((args[pos], kk) for pos, kk in HAP_RECIP[fn]))
                 by_r.setdefault(key, []).append((v, w))
             expr = f"acc[{s}]"
+            for key, tv in list(by_r.items()):
+                # (Dz(i,i,k) + Dz(i,k,i)) / 2: one merged numerator
+                fns = sorted((values[v][0], values[v][1], w) for v, w in tv)
+                if (len(tv) == 2 and [f for f, _, _ in fns] == ["hap_dz_iik", "hap_dz_iji"] and
+                        fns[0][1] == fns[1][1] and fns[0][2] == fns[1][2] == 0.5):
+                    args = ", ".join(f"p[{x}]" for x in fns[0][1])
+                    name = f"m{s}"
+                    ndecl.append(f"  const double {name} = hap_dz_iik_iji_avg_num({args});")
+                    by_r[key] = [(name, None)]
             for key, tv in by_r.items():
                 if key not in rkeys:
                     rkeys[key] = f"R{len(rkeys)}"
                     prod = " * ".join(f"p[{x}].r{kk}" for x, kk in key)
                     rdecl.append(f"  const double {rkeys[key]} = {prod} * wt;")
-                if len(tv) == 1 and tv[0][1] == 1.0:
+                if len(tv) == 1 and tv[0][1] is None:
+                    num = tv[0][0]
+                elif len(tv) == 1 and tv[0][1] == 1.0:
                     num = f"n{tv[0][0]}"
                 else:
                     num = " + ".join(f"{w!r} * n{v}" for v, w in tv)

@@ -62,9 +62,14 @@ __device__ __forceinline__ HapPop hap_pop(int n11, int n10, int n01, int n00, in
 // the grouping; hap_X = hap_X_num * R.  The fused kernel folds R into its
 // accumulation (one FMA per statistic, see the generated hap_acc_* groups).
 
+// x (s14 - 1) + y (s23 - 1), shared by the single-population DD and pi2
+__device__ __forceinline__ double hap_x1(const HapPop& a) {
+  return fma(a.x, a.s14 - 1.0, a.y * (a.s23 - 1.0));
+}
+
 // ---- DD -------------------------------------------------------------------
 __device__ __forceinline__ double hap_dd_same_num(const HapPop& a) {
-  return fma(a.D, a.D, -fma(a.x, a.s14 - 1.0, a.y * (a.s23 - 1.0)));
+  return fma(a.D, a.D, -hap_x1(a));
 }
 __device__ __forceinline__ double hap_dd_same(const HapPop& a) { return hap_dd_same_num(a) * a.r4; }
 __device__ __forceinline__ double hap_dd_pair_num(const HapPop& a, const HapPop& b) {
@@ -86,6 +91,11 @@ __device__ __forceinline__ double hap_dz_iik_num(const HapPop& a, const HapPop& 
 }
 __device__ __forceinline__ double hap_dz_iik(const HapPop& a, const HapPop& b) {
   return hap_dz_iik_num(a, b) * (a.r3 * b.r1);
+}
+// (Dz(i,i,k) + Dz(i,k,i)) / 2, the averaged statistic: both share -D_i and
+// the reciprocals, so their numerators merge
+__device__ __forceinline__ double hap_dz_iik_iji_avg_num(const HapPop& a, const HapPop& b) {
+  return ((-a.D) * fma(a.uA, b.uB, a.uB * b.uA)) * 0.5;
 }
 // Dz(i,j,i)
 __device__ __forceinline__ double hap_dz_iji_num(const HapPop& a, const HapPop& b) {
@@ -112,9 +122,10 @@ __device__ __forceinline__ double hap_dz_ijk(const HapPop& a, const HapPop& b, c
 
 // ---- pi2 ------------------------------------------------------------------
 // pA pB qA qB - c1c4 (c1 + 3c2 + 3c3 + c4 - 1) - c2c3 (3c1 + c2 + c3 + 3c4 - 1)
+// = PA PB - [x (s14 - 1) + y (s23 - 1)] - 3 (x s23 + y s14)
 __device__ __forceinline__ double hap_pi2_same_num(const HapPop& a) {
-  const double u = fma(3.0, a.s23, a.s14) - 1.0, v = fma(3.0, a.s14, a.s23) - 1.0;
-  return fma(a.PA, a.PB, -fma(a.x, u, a.y * v));
+  const double z = fma(a.x, a.s23, a.y * a.s14);
+  return fma(-3.0, z, fma(a.PA, a.PB, -hap_x1(a)));
 }
 __device__ __forceinline__ double hap_pi2_same(const HapPop& a) { return hap_pi2_same_num(a) * a.r4; }
 // pop t three times, pop s once (all "triple" cases share this formula):
