@@ -623,19 +623,25 @@ template <int NS, int GS, int S0, bool COUNT>
 __device__ __forceinline__ void reduce_to_limbs(const TileCtx& t, double (&acc)[GS], int bin,
                                                 unsigned long long pairs) {
   const int lane = t.lane;
+  // one butterfly reduce-scatter of the GS sums (padded to NV): lane l ends
+  // with statistic bitreverse(l mod NV); the first padding slot carries the
+  // pair count
+  constexpr int NV = GS <= 2 ? 2 : GS <= 4 ? 4 : GS <= 8 ? 8 : 16;
+  constexpr int LOG = NV == 2 ? 1 : NV == 4 ? 2 : NV == 8 ? 3 : 4;
+  static_assert(GS <= 16 && (!COUNT || GS < NV), "group size and count slot");
+  double v[NV];
 #pragma unroll
-  for (int q = 0; q < GS; ++q) acc[q] = pgld_wave_sum(acc[q]);  // valid in lanes 0..31
-  double v = acc[0];
-#pragma unroll
-  for (int q = 1; q < GS; ++q) v = lane == q ? acc[q] : v;
-  if (lane < GS) {
-    unsigned long long* dst = t.slimb + ((int64_t)bin * (NS + 1) + S0 + lane) * 3;
+  for (int q = 0; q < NV; ++q) v[q] = q < GS ? acc[q] : 0.0;
+  const double tot = pgld_wave_sums<NV>(v, lane);
+  const int q = pgld_bitrev(lane & (NV - 1), LOG);
+  if (lane < NV && q < GS) {
+    unsigned long long* dst = t.slimb + ((int64_t)bin * (NS + 1) + S0 + q) * 3;
     long long x2, x1, x0;
-    pgld_to_limbs(v, x2, x1, x0);
+    pgld_to_limbs(tot, x2, x1, x0);
     atomicAdd(dst + 0, (unsigned long long)x0);
     atomicAdd(dst + 1, (unsigned long long)x1);
     atomicAdd(dst + 2, (unsigned long long)x2);
-  } else if (COUNT && lane == GS) {
+  } else if (COUNT && lane < NV && q == GS) {
     atomicAdd(t.slimb + ((int64_t)bin * (NS + 1) + NS) * 3, pairs);
   }
 }

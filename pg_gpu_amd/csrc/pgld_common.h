@@ -114,6 +114,71 @@ __device__ __forceinline__ double pgld_wave_sum(double v) {
   return v + __builtin_bit_cast(double, ((uint64_t)hh << 32) | hl);
 }
 
+// Exchange a double with lane ^ M (M = 1, 2: DPP quad_perm; 4, 8, 16:
+// ds_swizzle bitmask xor; 32: ds_bpermute).
+template <int M>
+__device__ __forceinline__ double pgld_xchg_f64(double v) {
+  if constexpr (M == 1) {
+    return pgld_dpp_f64<0xB1>(v);
+  } else if constexpr (M == 2) {
+    return pgld_dpp_f64<0x4E>(v);
+  } else if constexpr (M == 4 || M == 8 || M == 16) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)(uint32_t)u, (M << 10) | 0x1F);
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(uint32_t)(u >> 32), (M << 10) | 0x1F);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+  } else {
+    static_assert(M == 32, "lane masks 1..32");
+    return __shfl_xor(v, 32, 64);
+  }
+}
+
+// One butterfly step of a wave reduce-scatter: lanes with bit M clear keep
+// v[0..CNT) and send v[CNT..2 CNT) to lane ^ M, which keeps the upper half;
+// each adds what it receives to what it keeps.
+template <int CNT, int M>
+__device__ __forceinline__ void pgld_butterfly(double* v, int lane) {
+  const bool up = (lane & M) != 0;
+#pragma unroll
+  for (int k = 0; k < CNT; ++k) {
+    const double keep = up ? v[k + CNT] : v[k];
+    const double send = up ? v[k] : v[k + CNT];
+    v[k] = keep + pgld_xchg_f64<M>(send);
+  }
+}
+
+// Wave (wave64) sums of NV (2, 4, 8 or 16) per-lane doubles with one
+// butterfly reduce-scatter: lane l ends with the total of value
+// bitreverse(l mod NV) (over log2 NV bits) -- every lane of the wave, as the
+// remaining lane bits are summed by plain exchanges.  A fixed summation tree,
+// so the result is deterministic; about 7 NV instructions instead of 16 NV
+// for NV separate wave sums.
+template <int NV>
+__device__ __forceinline__ double pgld_wave_sums(double (&v)[NV], int lane) {
+  static_assert(NV == 2 || NV == 4 || NV == 8 || NV == 16, "NV must be 2, 4, 8 or 16");
+  if constexpr (NV >= 16) pgld_butterfly<NV / 2, 1>(v, lane);
+  if constexpr (NV >= 8) pgld_butterfly<(NV >= 16 ? NV / 4 : NV / 2), (NV >= 16 ? 2 : 1)>(v, lane);
+  if constexpr (NV >= 4) {
+    constexpr int CNT = NV >= 16 ? NV / 8 : NV >= 8 ? NV / 4 : NV / 2;
+    constexpr int M = NV >= 16 ? 4 : NV >= 8 ? 2 : 1;
+    pgld_butterfly<CNT, M>(v, lane);
+  }
+  constexpr int MLAST = NV / 2;  // mask of the last butterfly step
+  pgld_butterfly<1, MLAST>(v, lane);
+  double t = v[0];
+  if constexpr (NV <= 1) t += pgld_xchg_f64<1>(t);
+  if constexpr (NV <= 2) t += pgld_xchg_f64<2>(t);
+  if constexpr (NV <= 4) t += pgld_xchg_f64<4>(t);
+  if constexpr (NV <= 8) t += pgld_xchg_f64<8>(t);
+  t += pgld_xchg_f64<16>(t);
+  t += pgld_xchg_f64<32>(t);
+  return t;
+}
+
+__device__ __forceinline__ int pgld_bitrev(int x, int bits) {
+  return (int)(__builtin_bitreverse32((uint32_t)x) >> (32 - bits));
+}
+
 __device__ __forceinline__ int pgld_wave_sum_int(int v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
