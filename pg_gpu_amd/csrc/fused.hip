@@ -984,21 +984,26 @@ __device__ __forceinline__ void window_group(const WindowParams& w, const double
     T::template accum<G>(w.f, srt, cts, r, acc, in ? 1.0 : 0.0);
     count += in;
   }
+  // butterfly reduce-scatter (see reduce_to_limbs); for group 0 the pair
+  // count rides in the first padding slot (an exact small integer)
+  constexpr int NV = GS <= 2 ? 2 : GS <= 4 ? 4 : GS <= 8 ? 8 : 16;
+  constexpr int LOG = NV == 2 ? 1 : NV == 4 ? 2 : NV == 8 ? 3 : 4;
+  static_assert(G != 0 || GS < NV, "count slot");
+  double v[NV];
 #pragma unroll
-  for (int q = 0; q < GS; ++q) acc[q] = pgld_wave_sum(acc[q]);
-  double v = acc[0];
-#pragma unroll
-  for (int q = 1; q < GS; ++q) v = lane == q ? acc[q] : v;
-  const int c = G == 0 ? pgld_wave_sum_int(count) : 0;
-  if (lane < GS) {
-    unsigned long long* dst = dst_w + (int64_t)(S0 + lane) * 3;
+  for (int q = 0; q < NV; ++q) v[q] = q < GS ? acc[q] : (q == GS && G == 0 ? (double)count : 0.0);
+  const double tot = pgld_wave_sums<NV>(v, lane);
+  const int q = pgld_bitrev(lane & (NV - 1), LOG);
+  if (lane < NV && q < GS) {
+    unsigned long long* dst = dst_w + (int64_t)(S0 + q) * 3;
     long long x2, x1, x0;
-    pgld_to_limbs(v, x2, x1, x0);
+    pgld_to_limbs(tot, x2, x1, x0);
     if (x0) atomicAdd(dst + 0, (unsigned long long)x0);
     if (x1) atomicAdd(dst + 1, (unsigned long long)x1);
     if (x2) atomicAdd(dst + 2, (unsigned long long)x2);
-  } else if (G == 0 && lane == GS && c) {
-    atomicAdd(dst_w + (int64_t)NS * 3, (unsigned long long)c);
+  } else if (G == 0 && lane < NV && q == GS) {
+    const unsigned long long c = (unsigned long long)tot;
+    if (c) atomicAdd(dst_w + (int64_t)NS * 3, c);
   }
 }
 
