@@ -410,7 +410,8 @@ struct HapTraits {
                                                                                      : HAP_NSTATS_P4;
   static constexpr int NG = P == 1 ? HAP_NGROUPS_P1 : P == 2 ? HAP_NGROUPS_P2 : P == 3 ? HAP_NGROUPS_P3
                                                                                        : HAP_NGROUPS_P4;
-  static constexpr int GS = NS < STATS_GROUP ? NS : STATS_GROUP;  // statistics per group
+  static constexpr int GS = P == 1 ? HAP_GS_P1 : P == 2 ? HAP_GS_P2 : P == 3 ? HAP_GS_P3
+                                                                         : HAP_GS_P4;  // per group
   static constexpr int NSO = NS + (R2 ? 2 : 0);
   static constexpr int NGT = NG + (R2 ? 1 : 0);
   // k-steps of the LDS operand ring per wave (0: register ring)
@@ -472,7 +473,8 @@ struct GenoTraits {
                                                                                         : GENO_NSTATS_P4;
   static constexpr int NG = P == 1 ? GENO_NGROUPS_P1 : P == 2 ? GENO_NGROUPS_P2 : P == 3 ? GENO_NGROUPS_P3
                                                                                           : GENO_NGROUPS_P4;
-  static constexpr int GS = NS < STATS_GROUP ? NS : STATS_GROUP;
+  static constexpr int GS = P == 1 ? GENO_GS_P1 : P == 2 ? GENO_GS_P2 : P == 3 ? GENO_GS_P3
+                                                                          : GENO_GS_P4;
   static constexpr int NSO = NS + (R2 ? 2 : 0);
   static constexpr int NGT = NG + (R2 ? 1 : 0);
   static constexpr int RING = 0;
@@ -626,9 +628,9 @@ __device__ __forceinline__ void reduce_to_limbs(const TileCtx& t, double (&acc)[
   // one butterfly reduce-scatter of the GS sums (padded to NV): lane l ends
   // with statistic bitreverse(l mod NV); the first padding slot carries the
   // pair count
-  constexpr int NV = GS <= 2 ? 2 : GS <= 4 ? 4 : GS <= 8 ? 8 : 16;
-  constexpr int LOG = NV == 2 ? 1 : NV == 4 ? 2 : NV == 8 ? 3 : 4;
-  static_assert(GS <= 16 && (!COUNT || GS < NV), "group size and count slot");
+  constexpr int NV = GS <= 2 ? 2 : GS <= 4 ? 4 : GS <= 8 ? 8 : GS <= 16 ? 16 : GS <= 32 ? 32 : 64;
+  constexpr int LOG = NV == 2 ? 1 : NV == 4 ? 2 : NV == 8 ? 3 : NV == 16 ? 4 : NV == 32 ? 5 : 6;
+  static_assert(GS <= 64 && (!COUNT || GS < NV), "group size and count slot");
   double v[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) v[q] = q < GS ? acc[q] : 0.0;
@@ -986,8 +988,8 @@ __device__ __forceinline__ void window_group(const WindowParams& w, const double
   }
   // butterfly reduce-scatter (see reduce_to_limbs); for group 0 the pair
   // count rides in the first padding slot (an exact small integer)
-  constexpr int NV = GS <= 2 ? 2 : GS <= 4 ? 4 : GS <= 8 ? 8 : 16;
-  constexpr int LOG = NV == 2 ? 1 : NV == 4 ? 2 : NV == 8 ? 3 : 4;
+  constexpr int NV = GS <= 2 ? 2 : GS <= 4 ? 4 : GS <= 8 ? 8 : GS <= 16 ? 16 : GS <= 32 ? 32 : 64;
+  constexpr int LOG = NV == 2 ? 1 : NV == 4 ? 2 : NV == 8 ? 3 : NV == 16 ? 4 : NV == 32 ? 5 : 6;
   static_assert(G != 0 || GS < NV, "count slot");
   double v[NV];
 #pragma unroll

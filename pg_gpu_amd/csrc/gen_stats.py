@@ -121,6 +121,13 @@ def stat_set_plan(P, keyfn):
 
 
 GROUP = 15  # statistics evaluated per pass of the fused epilogue
+# larger groups where the registers allow (one wave per SIMD): each pass
+# re-derives every population's basis from the counts
+GROUP_OF = {("hap", 4): 30}
+
+
+def group_size(prefix, P):
+    return GROUP_OF.get((prefix, P), GROUP)
 
 
 def emit_set(prefix, popty, P, keyfn):
@@ -129,13 +136,15 @@ def emit_set(prefix, popty, P, keyfn):
     re-evaluation.  Group k writes out[0 .. size-1] for statistics
     k*GROUP .. k*GROUP+size-1."""
     values, stats = stat_set_plan(P, keyfn)
-    ng = (len(stats) + GROUP - 1) // GROUP
+    gs = group_size(prefix, P)
+    ng = (len(stats) + gs - 1) // gs
     lines = [f"// P = {P}: {len(stats)} statistics from {len(values)} distinct raw-call values, "
-             f"{ng} group(s) of up to {GROUP}",
+             f"{ng} group(s) of up to {gs}",
              f"#define {prefix.upper()}_NSTATS_P{P} {len(stats)}",
-             f"#define {prefix.upper()}_NGROUPS_P{P} {ng}"]
+             f"#define {prefix.upper()}_NGROUPS_P{P} {ng}",
+             f"#define {prefix.upper()}_GS_P{P} {min(gs, len(stats))}"]
     for k in range(ng):
-        grp = stats[k * GROUP:(k + 1) * GROUP]
+        grp = stats[k * gs:(k + 1) * gs]
         used = sorted(set(v for terms in grp for v, _ in terms))
         lines.append(f"__device__ __forceinline__ void {prefix}_stats_P{P}_g{k}(const {popty}* p, "
                      "double* out) {")
@@ -155,7 +164,7 @@ def emit_set(prefix, popty, P, keyfn):
     # whole set (used by the per-pair and plain kernels)
     lines.append(f"__device__ __forceinline__ void {prefix}_stats_P{P}(const {popty}* p, double* out) {{")
     for k in range(ng):
-        lines.append(f"  {prefix}_stats_P{P}_g{k}(p, out + {k * GROUP});")
+        lines.append(f"  {prefix}_stats_P{P}_g{k}(p, out + {k * gs});")
     lines.append("}")
     return "\n".join(lines)
 
@@ -180,10 +189,11 @@ def emit_acc_set(P):
     product (the averaged orientations of one statistic share it, so their
     exact numerators are summed first)."""
     values, stats = stat_set_plan(P, hap_key)
-    ng = (len(stats) + GROUP - 1) // GROUP
+    gs = group_size("hap", P)
+    ng = (len(stats) + gs - 1) // gs
     lines = []
     for k in range(ng):
-        grp = stats[k * GROUP:(k + 1) * GROUP]
+        grp = stats[k * gs:(k + 1) * gs]
         used = sorted(set(v for terms in grp for v, _ in terms))
         lines.append(f"__device__ __forceinline__ void hap_acc_P{P}_g{k}(const HapPop* p, "
                      "double* acc, double wt) {")

@@ -147,32 +147,32 @@ __device__ __forceinline__ void pgld_butterfly(double* v, int lane) {
   }
 }
 
-// Wave (wave64) sums of NV (2, 4, 8 or 16) per-lane doubles with one
-// butterfly reduce-scatter: lane l ends with the total of value
+// All butterfly steps of masks M, 2M, ..., NV/2.
+template <int NV, int M>
+__device__ __forceinline__ void pgld_butterflies(double* v, int lane) {
+  if constexpr (M < NV) {
+    pgld_butterfly<NV / (2 * M), M>(v, lane);
+    pgld_butterflies<NV, 2 * M>(v, lane);
+  }
+}
+// Plain exchange-and-add over the lane masks M .. 32.
+template <int M>
+__device__ __forceinline__ double pgld_xsum(double t) {
+  if constexpr (M <= 32) return pgld_xsum<2 * M>(t + pgld_xchg_f64<M>(t));
+  else return t;
+}
+
+// Wave (wave64) sums of NV (2 .. 64, a power of two) per-lane doubles with
+// one butterfly reduce-scatter: lane l ends with the total of value
 // bitreverse(l mod NV) (over log2 NV bits) -- every lane of the wave, as the
 // remaining lane bits are summed by plain exchanges.  A fixed summation tree,
 // so the result is deterministic; about 7 NV instructions instead of 16 NV
 // for NV separate wave sums.
 template <int NV>
 __device__ __forceinline__ double pgld_wave_sums(double (&v)[NV], int lane) {
-  static_assert(NV == 2 || NV == 4 || NV == 8 || NV == 16, "NV must be 2, 4, 8 or 16");
-  if constexpr (NV >= 16) pgld_butterfly<NV / 2, 1>(v, lane);
-  if constexpr (NV >= 8) pgld_butterfly<(NV >= 16 ? NV / 4 : NV / 2), (NV >= 16 ? 2 : 1)>(v, lane);
-  if constexpr (NV >= 4) {
-    constexpr int CNT = NV >= 16 ? NV / 8 : NV >= 8 ? NV / 4 : NV / 2;
-    constexpr int M = NV >= 16 ? 4 : NV >= 8 ? 2 : 1;
-    pgld_butterfly<CNT, M>(v, lane);
-  }
-  constexpr int MLAST = NV / 2;  // mask of the last butterfly step
-  pgld_butterfly<1, MLAST>(v, lane);
-  double t = v[0];
-  if constexpr (NV <= 1) t += pgld_xchg_f64<1>(t);
-  if constexpr (NV <= 2) t += pgld_xchg_f64<2>(t);
-  if constexpr (NV <= 4) t += pgld_xchg_f64<4>(t);
-  if constexpr (NV <= 8) t += pgld_xchg_f64<8>(t);
-  t += pgld_xchg_f64<16>(t);
-  t += pgld_xchg_f64<32>(t);
-  return t;
+  static_assert(NV == 2 || NV == 4 || NV == 8 || NV == 16 || NV == 32 || NV == 64, "NV: 2 .. 64");
+  pgld_butterflies<NV, 1>(v, lane);
+  return pgld_xsum<NV>(v[0]);
 }
 
 __device__ __forceinline__ int pgld_bitrev(int x, int bits) {
