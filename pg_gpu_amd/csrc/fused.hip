@@ -410,10 +410,15 @@ struct HapTraits {
                                                                                      : HAP_NSTATS_P4;
   static constexpr int NG = P == 1 ? HAP_NGROUPS_P1 : P == 2 ? HAP_NGROUPS_P2 : P == 3 ? HAP_NGROUPS_P3
                                                                                        : HAP_NGROUPS_P4;
-  static constexpr int GS = P == 1 ? HAP_GS_P1 : P == 2 ? HAP_GS_P2 : P == 3 ? HAP_GS_P3
-                                                                         : HAP_GS_P4;  // per group
+  // With one statistic group the r^2 pair joins it (statistics NS, NS+1),
+  // so the population bases are derived once per pair; otherwise it is a
+  // group of its own after the NG groups.
+  static constexpr bool MERGE_R2 = R2 && (P == 1 ? HAP_NGROUPS_P1 : 2) == 1;
+  static constexpr int GS = (P == 1 ? HAP_GS_P1 : P == 2 ? HAP_GS_P2 : P == 3 ? HAP_GS_P3
+                                                                          : HAP_GS_P4) +
+                            (MERGE_R2 ? 2 : 0);  // per group
   static constexpr int NSO = NS + (R2 ? 2 : 0);
-  static constexpr int NGT = NG + (R2 ? 1 : 0);
+  static constexpr int NGT = NG + (R2 && !MERGE_R2 ? 1 : 0);
   // k-steps of the LDS operand ring per wave (0: register ring)
   static constexpr int RING = P >= 3 ? 4 : 0;
   static constexpr int RING_BYTES = RING * 4 * 1024;  // per wave
@@ -435,8 +440,12 @@ struct HapTraits {
       const int n01 = c.w1[p][r] & 0xffff, n00 = c.w1[p][r] >> 16;
       pp[p] = hap_pop(n11, n10, n01, n00, 0, rt_entry(a, srt, p, n11 + n10 + n01 + n00));
     }
-    if constexpr (G < NG) hap_stats_group<P, G>(pp, x);
-    else hap_r2(pp[0], x);
+    if constexpr (G < NG) {
+      hap_stats_group<P, G>(pp, x);
+      if constexpr (MERGE_R2) hap_r2(pp[0], x + GS - 2);
+    } else {
+      hap_r2(pp[0], x);
+    }
   }
   // acc[q] += wt * statistic G*GS+q (wt 0 or 1)
   template <int G>
@@ -451,6 +460,12 @@ struct HapTraits {
         pp[p] = hap_pop(n11, n10, n01, n00, 0, rt_entry(a, srt, p, n11 + n10 + n01 + n00));
       }
       hap_acc_group<P, G>(pp, acc, wt);
+      if constexpr (MERGE_R2) {
+        double x[2];
+        hap_r2(pp[0], x);
+        acc[GS - 2] = fma(x[0], wt, acc[GS - 2]);
+        acc[GS - 1] = fma(x[1], wt, acc[GS - 1]);
+      }
     } else {
       double x[2];
       stats<G>(a, srt, c, r, x);
@@ -473,10 +488,13 @@ struct GenoTraits {
                                                                                         : GENO_NSTATS_P4;
   static constexpr int NG = P == 1 ? GENO_NGROUPS_P1 : P == 2 ? GENO_NGROUPS_P2 : P == 3 ? GENO_NGROUPS_P3
                                                                                           : GENO_NGROUPS_P4;
-  static constexpr int GS = P == 1 ? GENO_GS_P1 : P == 2 ? GENO_GS_P2 : P == 3 ? GENO_GS_P3
-                                                                          : GENO_GS_P4;
+  // the r^2 pair joins a single statistic group (as in HapTraits)
+  static constexpr bool MERGE_R2 = R2 && (P == 1 ? GENO_NGROUPS_P1 : 2) == 1;
+  static constexpr int GS = (P == 1 ? GENO_GS_P1 : P == 2 ? GENO_GS_P2 : P == 3 ? GENO_GS_P3
+                                                                           : GENO_GS_P4) +
+                            (MERGE_R2 ? 2 : 0);
   static constexpr int NSO = NS + (R2 ? 2 : 0);
-  static constexpr int NGT = NG + (R2 ? 1 : 0);
+  static constexpr int NGT = NG + (R2 && !MERGE_R2 ? 1 : 0);
   static constexpr int RING = 0;
   static constexpr int RING_BYTES = 0;
   struct Counts { v16u w[P][W]; };
@@ -513,8 +531,12 @@ struct GenoTraits {
       geno_centered(g, y);  // y[0] = valid individuals
       pp[p] = geno_pop(g, y, 0, rt_entry(a, srt, p, y[0]));
     }
-    if constexpr (G < NG) geno_stats_group<P, G>(pp, x);
-    else geno_r2(pp[0], x);
+    if constexpr (G < NG) {
+      geno_stats_group<P, G>(pp, x);
+      if constexpr (MERGE_R2) geno_r2(pp[0], x + GS - 2);
+    } else {
+      geno_r2(pp[0], x);
+    }
   }
   template <int G>
   __device__ __forceinline__ static void accum(const FusedParams& a, const double* srt,
