@@ -171,3 +171,16 @@ def test_band_tile_squares_are_permutations(square):
     assert set(map(tuple, got)) == set(map(tuple, ref))
     if square == 1:   # squares of one tile: row-major
         np.testing.assert_array_equal(got, ref[np.lexsort((ref[:, 1], ref[:, 0]))])
+
+
+def test_smoke_case_runs_on_cpu():
+    """The oracle side of __graft_entry__.smoke() (workload and expected sums)."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location(
+        "graft_entry", Path(__file__).resolve().parents[1] / "__graft_entry__.py")
+    ge = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(ge)
+    w, ref, cnt, n_enum, absr = ge.smoke_case()
+    assert ref.shape == (len(w.bins) - 1, 15) and cnt.sum() <= n_enum and n_enum > 0
+    assert np.all(absr >= np.abs(ref) * (1 - 1e-12))
