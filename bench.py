@@ -320,6 +320,8 @@ def main():
             prec = dict(max_rel_err=float(err.max()),
                         max_rel_err_vs_sum=float(np.max(np.abs(got - ref) /
                                                         np.maximum(np.abs(ref), 1e-300))),
+                        per_stat_max_rel_err={n: float(e) for n, e in
+                                              zip(("DD", "Dz", "pi2"), err.max(axis=0))},
                         r2_sum_max_rel_err=float(np.max(np.abs(rg.r2_sum - cpu["sums"][:, 3]) /
                                                         np.maximum(cpu["sums"][:, 3], 1e-300))),
                         counts_equal=bool(np.array_equal(rg.n_pairs, cpu["counts"]) and
@@ -332,9 +334,15 @@ def main():
             ref = cpu["sums"][:, :ns]
             scale = np.maximum(np.abs(ref), 1e-3 * cpu["abs"][:, :ns])
             err = np.abs(rg.sums - ref) / np.maximum(scale, 1e-300)
+            from pg_gpu_amd.ld_pipeline import ld_names
+
+            names = ld_names(pan.P)
             prec = dict(max_rel_err=float(err.max()),
                         max_rel_err_vs_sum=float(np.max(np.abs(rg.sums - ref) /
                                                         np.maximum(np.abs(ref), 1e-300))),
+                        # per statistic, max over the bins (same scaled error)
+                        per_stat_max_rel_err={n: float(e) for n, e in zip(names, err.max(axis=0))}
+                        if len(names) == ns else None,
                         counts_equal=bool(np.array_equal(rg.counts, cpu["counts"])),
                         sample_pairs=int(cpu["n_pairs"]))
             if r2:
