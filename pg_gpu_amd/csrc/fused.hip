@@ -276,6 +276,21 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
 // Packed per slot in moments order g1..g9 (n22, n21, n20, n12, n11, n10, n02,
 // n01, n00) as five 16-bit pairs.
 // ---------------------------------------------------------------------------
+// Pack the nine class counts of slot r (moments order) into w[.][r].
+template <int W>
+__device__ __forceinline__ void geno_pack_slot(v16u (&w)[W], int r, const int (&c)[9]) {
+  if constexpr (W == 3) {  // 10-bit fields: every count <= population size <= 1023
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      w[q][r] = (uint32_t)c[3 * q] | ((uint32_t)c[3 * q + 1] << 10) |
+                ((uint32_t)c[3 * q + 2] << 20);
+  } else {  // 16-bit fields
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q][r] = (uint32_t)c[2 * q] | ((uint32_t)c[2 * q + 1] << 16);
+    w[4][r] = (uint32_t)c[8];
+  }
+}
+
 template <int W>
 __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, int64_t rb,
                                                  int64_t cb, int lane, v16u (&w)[W]) {
@@ -284,10 +299,11 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
   const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[p]) + lane;
   const int fs = ks * 64;  // feature stride in v4i
   v16f bb = zero16f(), bc = zero16f(), cbm = zero16f(), cc = zero16f();
-  int cnt[16][9];
   if (a.missing[p]) {
     v16f bv = zero16f(), cv = zero16f(), vb = zero16f(), vc = zero16f(), vv = zero16f();
-    constexpr int PF = 2;
+    // one k-step ahead: nine accumulators leave no registers for more (two
+    // steps ahead spilled and was no faster)
+    constexpr int PF = 1;
     v4i fa[PF][3], fb[PF][3];
 #pragma unroll
     for (int d = 0; d < PF; ++d)
@@ -323,8 +339,8 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
       const int n10 = b_v - n11 - n12, n20 = c_v - n21 - n22;
       const int n01 = (int)vb[r] - n11 - n21, n02 = (int)vc[r] - n12 - n22;
       const int n00 = (int)vv[r] - b_v - c_v - n01 - n02;
-      cnt[r][0] = n22; cnt[r][1] = n21; cnt[r][2] = n20; cnt[r][3] = n12; cnt[r][4] = n11;
-      cnt[r][5] = n10; cnt[r][6] = n02; cnt[r][7] = n01; cnt[r][8] = n00;
+      const int c[9] = {n22, n21, n20, n12, n11, n10, n02, n01, n00};
+      geno_pack_slot(w, r, c);
     }
   } else {
     constexpr int PF = 2;
@@ -363,22 +379,8 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
       const int n10 = bvi - n11 - n12, n20 = cvi - n21 - n22;
       const int n01 = vbj - n11 - n21, n02 = vcj - n12 - n22;
       const int n00 = K - bvi - cvi - n01 - n02;
-      cnt[r][0] = n22; cnt[r][1] = n21; cnt[r][2] = n20; cnt[r][3] = n12; cnt[r][4] = n11;
-      cnt[r][5] = n10; cnt[r][6] = n02; cnt[r][7] = n01; cnt[r][8] = n00;
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    if constexpr (W == 3) {  // 10-bit fields: every count <= population size <= 1023
-#pragma unroll
-      for (int q = 0; q < 3; ++q)
-        w[q][r] = (uint32_t)cnt[r][3 * q] | ((uint32_t)cnt[r][3 * q + 1] << 10) |
-                  ((uint32_t)cnt[r][3 * q + 2] << 20);
-    } else {  // 16-bit fields
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        w[q][r] = (uint32_t)cnt[r][2 * q] | ((uint32_t)cnt[r][2 * q + 1] << 16);
-      w[4][r] = (uint32_t)cnt[r][8];
+      const int c[9] = {n22, n21, n20, n12, n11, n10, n02, n01, n00};
+      geno_pack_slot(w, r, c);
     }
   }
 }
