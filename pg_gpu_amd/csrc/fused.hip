@@ -186,7 +186,9 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
     v16f hh = zero16f(), hv = zero16f(), vh = zero16f(), vv = zero16f();
     const v4i* Av = A + ks * 64;
     const v4i* Bv = B + ks * 64;
-    constexpr int PF = 2;
+    // one k-step ahead: the registers a second step would hold cost more
+    // (spills, occupancy of the epilogue) than its latency cover gains
+    constexpr int PF = 1;
     v4i ah[PF], av[PF], bh[PF], bv[PF];
 #pragma unroll
     for (int d = 0; d < PF; ++d) {
