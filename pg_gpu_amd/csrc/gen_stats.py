@@ -123,7 +123,7 @@ def stat_set_plan(P, keyfn):
 GROUP = 15  # statistics evaluated per pass of the fused epilogue
 # larger groups where the registers allow (one wave per SIMD): each pass
 # re-derives every population's basis from the counts
-GROUP_OF = {("hap", 4): 30}
+GROUP_OF = {("hap", 4): 53}
 
 
 def group_size(prefix, P):
