@@ -424,7 +424,7 @@ struct HapTraits {
   static constexpr int NSO = NS + (R2 ? 2 : 0);
   static constexpr int NGT = NG + (R2 && !MERGE_R2 ? 1 : 0);
   // k-steps of the LDS operand ring per wave (0: register ring)
-  static constexpr int RING = P >= 3 ? 4 : 0;
+  static constexpr int RING = P >= 4 ? 4 : 0;
   static constexpr int RING_BYTES = RING * 4 * 1024;  // per wave
   struct Counts { v16u w0[P], w1[P]; };
   __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane,
