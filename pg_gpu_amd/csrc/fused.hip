@@ -426,6 +426,13 @@ struct HapTraits {
   // k-steps of the LDS operand ring per wave (0: register ring)
   static constexpr int RING = P >= 4 ? 4 : 0;
   static constexpr int RING_BYTES = RING * 4 * 1024;  // per wave
+  // the group's sums hold statistic q times 2^e (raw numerators; 0 past the
+  // statistics and for the r^2 slots)
+  template <int G>
+  __device__ __forceinline__ static int scale_exp(int q) {
+    if constexpr (G < NG) return hap_scale_group<P, G>(q);
+    else return 0;
+  }
   struct Counts { v16u w0[P], w1[P]; };
   __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane,
                                                 Counts& c, v4i* ring) {
@@ -501,6 +508,8 @@ struct GenoTraits {
   static constexpr int NGT = NG + (R2 && !MERGE_R2 ? 1 : 0);
   static constexpr int RING = 0;
   static constexpr int RING_BYTES = 0;
+  template <int G>
+  __device__ __forceinline__ static int scale_exp(int) { return 0; }
   struct Counts { v16u w[P][W]; };
   __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane,
                                                 Counts& c, v4i*) {
@@ -575,7 +584,7 @@ __device__ __forceinline__ void stage_add(double* stg, int* scnt, int bin, const
 
 // Move staged bins [blo, bhi] of this wave into the workgroup limbs of
 // statistics s0 .. s0+GS-1 (and the pair counters).
-template <int NS, int GS>
+template <class T, int G, int NS, int GS>
 __device__ __forceinline__ void stage_drain(double* stg, int* scnt, unsigned long long* slimb,
                                             int lane, int blo, int bhi, int s0) {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -589,7 +598,7 @@ __device__ __forceinline__ void stage_drain(double* stg, int* scnt, unsigned lon
         stg[b * GS + q] = 0.0;
         unsigned long long* dst = slimb + ((int64_t)b * (NS + 1) + s0 + q) * 3;
         long long x2, x1, x0;
-        pgld_to_limbs(v, x2, x1, x0);
+        pgld_to_limbs(v, x2, x1, x0, pgld_limb_scale(T::template scale_exp<G>(q)));
         atomicAdd(dst + 0, (unsigned long long)x0);
         atomicAdd(dst + 1, (unsigned long long)x1);
         atomicAdd(dst + 2, (unsigned long long)x2);
@@ -647,7 +656,7 @@ struct TileCtx {
 // Wave totals of a group's per-lane sums for one bin, added to the workgroup
 // limbs (lane q < GS: statistic S0+q); with COUNT, `pairs` joins the bin's
 // pair counter.
-template <int NS, int GS, int S0, bool COUNT>
+template <class T, int G, int NS, int GS, int S0, bool COUNT>
 __device__ __forceinline__ void reduce_to_limbs(const TileCtx& t, double (&acc)[GS], int bin,
                                                 unsigned long long pairs) {
   const int lane = t.lane;
@@ -665,7 +674,7 @@ __device__ __forceinline__ void reduce_to_limbs(const TileCtx& t, double (&acc)[
   if (lane < NV && q < GS) {
     unsigned long long* dst = t.slimb + ((int64_t)bin * (NS + 1) + S0 + q) * 3;
     long long x2, x1, x0;
-    pgld_to_limbs(tot, x2, x1, x0);
+    pgld_to_limbs(tot, x2, x1, x0, pgld_limb_scale(T::template scale_exp<G>(q)));
     atomicAdd(dst + 0, (unsigned long long)x0);
     atomicAdd(dst + 1, (unsigned long long)x1);
     atomicAdd(dst + 2, (unsigned long long)x2);
@@ -725,7 +734,7 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
     // every pair of the tile is valid and falls in bin bmin
 #pragma unroll 1
     for (int r = 0; r < 16; ++r) T::template accum<G>(a, t.srt, cts, r, acc, 1.0);
-    reduce_to_limbs<NS, GS, S0, G == 0>(t, acc, t.bmin, 1024ull);
+    reduce_to_limbs<T, G, NS, GS, S0, G == 0>(t, acc, t.bmin, 1024ull);
   } else if (t.dual) {
     // Two bins: each pair's statistics go to both accumulators with 0/1
     // weights (the estimators are finite for every slot: counts of padded
@@ -751,8 +760,8 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
       T::template accum<G>(a, t.srt, cts, r, acc, in0 ? 1.0 : 0.0);
       T::template accum<G>(a, t.srt, cts, r, acc1, in1 ? 1.0 : 0.0);
     }
-    reduce_to_limbs<NS, GS, S0, G == 0>(t, acc, t.blo, cnt0);
-    if (t.blo + 1 < nb) reduce_to_limbs<NS, GS, S0, G == 0>(t, acc1, t.blo + 1, cnt1);
+    reduce_to_limbs<T, G, NS, GS, S0, G == 0>(t, acc, t.blo, cnt0);
+    if (t.blo + 1 < nb) reduce_to_limbs<T, G, NS, GS, S0, G == 0>(t, acc1, t.blo + 1, cnt1);
   } else {
     uint32_t bw[4];
     tile_bins(a, t, bw);
@@ -780,7 +789,7 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
       }
     }
     if (cur >= 0) stage_add<GS, G == 0>(t.stg, t.scw, cur, acc, count);
-    stage_drain<NS, GS>(t.stg, t.scw, t.slimb, lane, t.bmin < 0 ? 0 : t.bmin,
+    stage_drain<T, G, NS, GS>(t.stg, t.scw, t.slimb, lane, t.bmin < 0 ? 0 : t.bmin,
                         t.bmax >= nb ? nb - 1 : t.bmax, S0);
   }
 }
@@ -887,7 +896,7 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
     local = next;
   }
   // nothing may stay staged
-  stage_drain<NS, GS>(t.stg, t.scw, slimb, lane, 0, nb - 1, 0);
+  stage_drain<T, 0, NS, GS>(t.stg, t.scw, slimb, lane, 0, nb - 1, 0);
   __syncthreads();
   flush_global<NS>(slimb, a.out, nb);
 }
@@ -1025,7 +1034,7 @@ __device__ __forceinline__ void window_group(const WindowParams& w, const double
   if (lane < NV && q < GS) {
     unsigned long long* dst = dst_w + (int64_t)(S0 + q) * 3;
     long long x2, x1, x0;
-    pgld_to_limbs(tot, x2, x1, x0);
+    pgld_to_limbs(tot, x2, x1, x0, pgld_limb_scale(T::template scale_exp<G>(q)));
     if (x0) atomicAdd(dst + 0, (unsigned long long)x0);
     if (x1) atomicAdd(dst + 1, (unsigned long long)x1);
     if (x2) atomicAdd(dst + 2, (unsigned long long)x2);

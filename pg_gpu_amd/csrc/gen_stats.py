@@ -12,6 +12,8 @@ Run:  python pg_gpu_amd/csrc/gen_stats.py   (writes stats_hap.inc beside it)
 """
 from __future__ import annotations
 
+import math
+
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
@@ -183,58 +185,98 @@ HAP_RECIP = {
 }
 
 
+# Constant power of two of each raw call whose device function has a _raw
+# form (hap_X_num = hap_X_raw * INNER[X]); the merged Dz(i,i,k)/Dz(i,k,i)
+# average carries 1/2 the same way.
+HAP_INNER = {
+    "hap_pi2_triple": 0.5, "hap_pi2_iikl": 0.5, "hap_pi2_ijkk": 0.5,
+    "hap_pi2_ijij": 0.0625, "hap_pi2_shared": 0.0625, "hap_pi2_alldiff": 0.25,
+}
+HAP_AVG_INNER = 0.5
+# population counts whose groups accumulate raw numerators; with four
+# populations the scaled form compiles to fewer spilled registers (measured:
+# C4 169 ms scaled, 175 ms raw)
+HAP_FOLD_P = (1, 2, 3)
+
+
+def _log2_exact(x):
+    e = math.frexp(x)[1] - 1
+    if x <= 0 or 2.0 ** e != x:
+        raise ValueError(f"{x} is not a power of two")
+    return e
+
+
 def emit_acc_set(P):
-    """Accumulating form of the haplotype groups: acc[s] += wt * statistic s,
-    with each statistic's reciprocal product folded into one FMA per distinct
-    product (the averaged orientations of one statistic share it, so their
-    exact numerators are summed first)."""
+    """Accumulating form of the haplotype groups: acc[s] += wt * statistic s
+    / 2^-e_s, with each statistic's reciprocal product folded into one FMA per
+    distinct product (the averaged orientations of one statistic share it, so
+    their exact numerators are summed first).  e_s is the statistic's largest
+    constant power of two (the orientation weights times the raw calls' own
+    constants); the sums are scaled by 2^-e_s where they become limbs
+    (hap_scale_group)."""
     values, stats = stat_set_plan(P, hap_key)
     gs = group_size("hap", P)
     ng = (len(stats) + gs - 1) // gs
+    fold = P in HAP_FOLD_P
+    inner = HAP_INNER if fold else {}
     lines = []
     for k in range(ng):
         grp = stats[k * gs:(k + 1) * gs]
         used = sorted(set(v for terms in grp for v, _ in terms))
         lines.append(f"__device__ __forceinline__ void hap_acc_P{P}_g{k}(const HapPop* p, "
                      "double* acc, double wt) {")
-        rkeys, rdecl, ndecl, body = {}, [], [], []
+        rkeys, rdecl, ndecl, body, exps = {}, [], [], [], []
         for v in used:
             fn, args = values[v]
             a = ", ".join(f"p[{x}]" for x in args)
-            ndecl.append(f"  const double n{v} = {fn}_num({a});")
+            form = "raw" if fn in inner else "num"
+            ndecl.append(f"  const double n{v} = {fn}_{form}({a});")
         for s, terms in enumerate(grp):
+            # R key -> [(numerator name, constant multiplier)]
             by_r = {}
             for v, w in terms:
                 fn, args = values[v]
                 key = tuple(sorted((args[pos], kk) for pos, kk in HAP_RECIP[fn]))
-                by_r.setdefault(key, []).append((v, w))
-            expr = f"acc[{s}]"
+                by_r.setdefault(key, []).append((f"n{v}", w * inner.get(fn, 1.0), (fn, args, w)))
             for key, tv in list(by_r.items()):
                 # (Dz(i,i,k) + Dz(i,k,i)) / 2: one merged numerator
-                fns = sorted((values[v][0], values[v][1], w) for v, w in tv)
+                fns = sorted(x[2] for x in tv)
                 if (len(tv) == 2 and [f for f, _, _ in fns] == ["hap_dz_iik", "hap_dz_iji"] and
                         fns[0][1] == fns[1][1] and fns[0][2] == fns[1][2] == 0.5):
                     args = ", ".join(f"p[{x}]" for x in fns[0][1])
                     name = f"m{s}"
-                    ndecl.append(f"  const double {name} = hap_dz_iik_iji_avg_num({args});")
-                    by_r[key] = [(name, None)]
+                    form = "raw" if fold else "num"
+                    ndecl.append(f"  const double {name} = hap_dz_iik_iji_avg_{form}({args});")
+                    by_r[key] = [(name, HAP_AVG_INNER if fold else 1.0, None)]
+            scale = max(m for tv in by_r.values() for _, m, _ in tv) if fold else 1.0
+            exps.append(-_log2_exact(scale))
+            expr = f"acc[{s}]"
             for key, tv in by_r.items():
                 if key not in rkeys:
                     rkeys[key] = f"R{len(rkeys)}"
                     prod = " * ".join(f"p[{x}].r{kk}" for x, kk in key)
                     rdecl.append(f"  const double {rkeys[key]} = {prod} * wt;")
-                if len(tv) == 1 and tv[0][1] is None:
-                    num = tv[0][0]
-                elif len(tv) == 1 and tv[0][1] == 1.0:
-                    num = f"n{tv[0][0]}"
-                else:
-                    num = " + ".join(f"{w!r} * n{v}" for v, w in tv)
+                parts = []
+                for name, m, _ in tv:
+                    rel = m / scale
+                    _log2_exact(rel)
+                    parts.append(name if rel == 1.0 else f"{rel!r} * {name}")
+                num = " + ".join(parts)
                 expr = f"fma({num}, {rkeys[key]}, {expr})"
             body.append(f"  acc[{s}] = {expr};")
         lines += rdecl + ndecl + body
         lines.append("}")
         lines.append(f"template <> __device__ __forceinline__ void hap_acc_group<{P}, {k}>("
                      f"const HapPop* p, double* acc, double wt) {{ hap_acc_P{P}_g{k}(p, acc, wt); }}")
+        # e_s in 4-bit fields, 16 statistics per word; 0 past the group's end
+        assert all(0 <= e < 16 for e in exps)
+        words = [sum(e << (4 * i) for i, e in enumerate(exps[w:w + 16]))
+                 for w in range(0, len(exps), 16)]
+        sel = "".join(f"q < {16 * (i + 1)} ? 0x{w:x}ull : " for i, w in enumerate(words)) + "0ull"
+        lines.append(f"template <> __device__ __forceinline__ int hap_scale_group<{P}, {k}>(int q) {{")
+        lines.append(f"  const unsigned long long w = {sel};")
+        lines.append("  return (int)((w >> (4 * (q & 15))) & 15ull);")
+        lines.append("}")
     return "\n".join(lines)
 
 
@@ -295,6 +337,8 @@ def main():
              ""]
     parts.append("template <int P, int G> __device__ void hap_acc_group(const HapPop* p, double* acc, "
                  "double wt);")
+    parts.append("// exponent e of the 2^-e the fused kernel applies to statistic q of group G")
+    parts.append("template <int P, int G> __device__ int hap_scale_group(int q);")
     for P in range(1, 5):
         parts.append(emit_set("hap", "HapPop", P, hap_key))
         parts.append(emit_acc_set(P))

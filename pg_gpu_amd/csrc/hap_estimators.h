@@ -61,6 +61,11 @@ __device__ __forceinline__ HapPop hap_pop(int n11, int n10, int n01, int n00, in
 // 1/16) below 2^53, so num equals the reference's numerator exactly whatever
 // the grouping; hap_X = hap_X_num * R.  The fused kernel folds R into its
 // accumulation (one FMA per statistic, see the generated hap_acc_* groups).
+// Where num carries a constant power of two, hap_X_raw is num without it
+// (num = raw * 2^-k): the fused kernel accumulates raw numerators and applies
+// 2^-k once, where a tile's sums become fixed-point limbs.  Scaling by a power
+// of two commutes with every rounding (no value comes near the subnormal
+// range), so the limbs are the ones the scaled accumulation would give.
 
 // x (s14 - 1) + y (s23 - 1), shared by the single-population DD and pi2
 __device__ __forceinline__ double hap_x1(const HapPop& a) {
@@ -94,8 +99,11 @@ __device__ __forceinline__ double hap_dz_iik(const HapPop& a, const HapPop& b) {
 }
 // (Dz(i,i,k) + Dz(i,k,i)) / 2, the averaged statistic: both share -D_i and
 // the reciprocals, so their numerators merge
+__device__ __forceinline__ double hap_dz_iik_iji_avg_raw(const HapPop& a, const HapPop& b) {
+  return (-a.D) * fma(a.uA, b.uB, a.uB * b.uA);
+}
 __device__ __forceinline__ double hap_dz_iik_iji_avg_num(const HapPop& a, const HapPop& b) {
-  return ((-a.D) * fma(a.uA, b.uB, a.uB * b.uA)) * 0.5;
+  return hap_dz_iik_iji_avg_raw(a, b) * 0.5;
 }
 // Dz(i,j,i)
 __device__ __forceinline__ double hap_dz_iji_num(const HapPop& a, const HapPop& b) {
@@ -131,11 +139,14 @@ __device__ __forceinline__ double hap_pi2_same(const HapPop& a) { return hap_pi2
 // pop t three times, pop s once (all "triple" cases share this formula):
 // (s.pB (PA qB - m) + s.qB (PA pB + m - 2 PA)) / 2 with PA, qB, pB, m of t and
 // m = pA c4 + c2 qA
-__device__ __forceinline__ double hap_pi2_triple_num(const HapPop& t, const HapPop& s) {
+__device__ __forceinline__ double hap_pi2_triple_raw(const HapPop& t, const HapPop& s) {
   const double m = fma(t.pA, t.c4, t.c2 * t.qA);
   const double u = fma(t.PA, t.qB, -m);
   const double v = fma(-2.0, t.PA, fma(t.PA, t.pB, m));
-  return fma(s.qB, v, s.pB * u) * 0.5;
+  return fma(s.qB, v, s.pB * u);
+}
+__device__ __forceinline__ double hap_pi2_triple_num(const HapPop& t, const HapPop& s) {
+  return hap_pi2_triple_raw(t, s) * 0.5;
 }
 __device__ __forceinline__ double hap_pi2_triple(const HapPop& t, const HapPop& s) {
   return hap_pi2_triple_num(t, s) * (s.r1 * t.r3);
@@ -148,17 +159,25 @@ __device__ __forceinline__ double hap_pi2_iikk(const HapPop& a, const HapPop& b)
   return hap_pi2_iikk_num(a, b) * (a.r2 * b.r2);
 }
 // pi2(i,i,k,l), k != l
+__device__ __forceinline__ double hap_pi2_iikl_raw(const HapPop& a, const HapPop& b,
+                                                   const HapPop& c) {
+  return a.PA * fma(b.pB, c.qB, b.qB * c.pB);
+}
 __device__ __forceinline__ double hap_pi2_iikl_num(const HapPop& a, const HapPop& b,
                                                    const HapPop& c) {
-  return a.PA * fma(b.pB, c.qB, b.qB * c.pB) * 0.5;
+  return hap_pi2_iikl_raw(a, b, c) * 0.5;
 }
 __device__ __forceinline__ double hap_pi2_iikl(const HapPop& a, const HapPop& b, const HapPop& c) {
   return hap_pi2_iikl_num(a, b, c) * (a.r2 * b.r1 * c.r1);
 }
 // pi2(i,j,k,k), i != j
+__device__ __forceinline__ double hap_pi2_ijkk_raw(const HapPop& a, const HapPop& b,
+                                                   const HapPop& c) {
+  return c.PB * fma(a.pA, b.qA, a.qA * b.pA);
+}
 __device__ __forceinline__ double hap_pi2_ijkk_num(const HapPop& a, const HapPop& b,
                                                    const HapPop& c) {
-  return c.PB * fma(a.pA, b.qA, a.qA * b.pA) * 0.5;
+  return hap_pi2_ijkk_raw(a, b, c) * 0.5;
 }
 __device__ __forceinline__ double hap_pi2_ijkk(const HapPop& a, const HapPop& b, const HapPop& c) {
   return hap_pi2_ijkk_num(a, b, c) * (c.r2 * a.r1 * b.r1);
@@ -168,10 +187,13 @@ __device__ __forceinline__ double hap_pi2_ijkk(const HapPop& a, const HapPop& b,
 //   (w_A w_B - (n_A - 1)(n_B - 1)(uA_A uA_B + uB_A uB_B - n_A n_B)) / 4
 // (uA = qA - pA, uB = qB - pB, w = uA uB + c2 + c3 - c1 - c4 are the centered
 // allele and haplotype sums up to sign), an identity in the eight counts.
-__device__ __forceinline__ double hap_pi2_ijij_num(const HapPop& A, const HapPop& B) {
+__device__ __forceinline__ double hap_pi2_ijij_raw(const HapPop& A, const HapPop& B) {
   const double m = (A.n - 1.0) * (B.n - 1.0);
   const double x = fma(-A.n, B.n, fma(A.uB, B.uB, A.uA * B.uA));
-  return fma(-m, x, A.w * B.w) * 0.0625;
+  return fma(-m, x, A.w * B.w);
+}
+__device__ __forceinline__ double hap_pi2_ijij_num(const HapPop& A, const HapPop& B) {
+  return hap_pi2_ijij_raw(A, B) * 0.0625;
 }
 __device__ __forceinline__ double hap_pi2_ijij(const HapPop& A, const HapPop& B) {
   return hap_pi2_ijij_num(A, B) * (A.r2 * B.r2);
@@ -179,18 +201,26 @@ __device__ __forceinline__ double hap_pi2_ijij(const HapPop& A, const HapPop& B)
 // One pop (S) shared between the two locus pairs, A/B the other two pops.
 // The reference's polynomial equals, in the same centered sums,
 //   (uA_A uB_B w_S - (n_S - 1)(uA_S uA_A n_B + uB_S uB_B n_A - n_S n_A n_B)) / 4.
-__device__ __forceinline__ double hap_pi2_shared_num(const HapPop& S, const HapPop& A,
+__device__ __forceinline__ double hap_pi2_shared_raw(const HapPop& S, const HapPop& A,
                                                      const HapPop& B) {
   const double x = fma(-S.n, A.n * B.n, fma(S.uB * B.uB, A.n, (S.uA * A.uA) * B.n));
-  return fma(-(S.n - 1.0), x, (A.uA * B.uB) * S.w) * 0.0625;
+  return fma(-(S.n - 1.0), x, (A.uA * B.uB) * S.w);
+}
+__device__ __forceinline__ double hap_pi2_shared_num(const HapPop& S, const HapPop& A,
+                                                     const HapPop& B) {
+  return hap_pi2_shared_raw(S, A, B) * 0.0625;
 }
 __device__ __forceinline__ double hap_pi2_shared(const HapPop& S, const HapPop& A, const HapPop& B) {
   return hap_pi2_shared_num(S, A, B) * (S.r2 * A.r1 * B.r1);
 }
 // pi2(i,j,k,l), all distinct
+__device__ __forceinline__ double hap_pi2_alldiff_raw(const HapPop& a, const HapPop& b,
+                                                      const HapPop& c, const HapPop& d) {
+  return fma(a.pA, b.qA, a.qA * b.pA) * fma(c.pB, d.qB, c.qB * d.pB);
+}
 __device__ __forceinline__ double hap_pi2_alldiff_num(const HapPop& a, const HapPop& b,
                                                       const HapPop& c, const HapPop& d) {
-  return fma(a.pA, b.qA, a.qA * b.pA) * fma(c.pB, d.qB, c.qB * d.pB) * 0.25;
+  return hap_pi2_alldiff_raw(a, b, c, d) * 0.25;
 }
 __device__ __forceinline__ double hap_pi2_alldiff(const HapPop& a, const HapPop& b,
                                                   const HapPop& c, const HapPop& d) {

@@ -60,10 +60,16 @@ __device__ __forceinline__ double pgld_int_f64(int32_t v) {
 
 #define PGLD_LIMBS 3
 
+// 2^(8-e) as a double (0 <= e < 16): the limb scale of a sum held as
+// v = 2^e * value
+__device__ __forceinline__ double pgld_limb_scale(int e) {
+  return __builtin_bit_cast(double, (unsigned long long)(1023 + 8 - e) << 52);
+}
+
 __device__ __forceinline__ void pgld_to_limbs(double v, long long& x2, long long& x1,
-                                              long long& x0) {
+                                              long long& x0, double scale = 256.0) {
   const double a = fabs(v);
-  const double t = a * 256.0;                 // exact power-of-two scaling
+  const double t = a * scale;                 // exact power-of-two scaling
   const double h = floor(t);
   const double r = (t - h) * 4294967296.0;    // fraction in [0, 2^32), exact
   const double m = floor(r);
