@@ -81,11 +81,13 @@ __global__ void __launch_bounds__(256) scan_sites_kernel(
 
 // ---------------------------------------------------------------------------
 // Pack: one workgroup per (PACK_SB site blocks, pop, 64-sample k-step).
-// Phase 1 reads 64 sample rows x 32 PACK_SB sites, 16 bytes per load (rows
-// not aligned to 16 bytes: two aligned loads and a byte funnel shift), and stores them
-// transposed ([site][sample]) in LDS; phase 2 gives each lane the 32 samples
-// of one site as two 16-byte LDS reads, turns them into 4-bit indicators with
-// bytewise (SWAR) tests, and writes 16 bytes per lane, fully coalesced.
+// Phase 1: thread (row quad j, 16-site group g) reads 16 bytes of each of the
+// four sample rows 4j..4j+3 (rows not aligned to 16 bytes: two aligned loads
+// and a byte funnel shift), transposes the 4 x 4 byte blocks in registers and
+// stores one dword (four samples) per site into the LDS tile [site][sample].
+// Phase 2 gives each lane the 32 samples of one site, turns them into 4-bit
+// indicators with bytewise (SWAR) tests, and writes 16 bytes per lane, fully
+// coalesced.
 // ---------------------------------------------------------------------------
 // Bit 0 of byte b set where byte b of x equals c (exact per byte: the 7-bit
 // sums cannot carry across bytes).
@@ -99,55 +101,83 @@ __device__ __forceinline__ uint32_t flags_to_nibbles(uint32_t m) {
   return (y & 0x11u) | ((y >> 8) & 0x1100u);
 }
 
-constexpr int PACK_SB = 4;  // site blocks per workgroup
+constexpr int PACK_SB = 8;  // site blocks per workgroup (256 sites, 16 groups of 16)
+// Dwords per site row of the LDS tile: 64 samples + 3 dwords of padding.  The
+// odd pitch puts the 32 sites one lane pair reads (phase 2) and the two
+// halves of the groups one wave writes (phase 1) on distinct banks.
+constexpr int PACK_PITCH = 19;
+
+// Sites site0 .. site0+15 of the row at src, as four little-endian dwords;
+// sites at or past n_sites read as -2 (no feature set).
+__device__ __forceinline__ uint4 pack_load16(const int8_t* mat, const int8_t* mat_end,
+                                             const int8_t* src, int64_t site0, int64_t n_sites) {
+  const int m = (int)(reinterpret_cast<uintptr_t>(src) & 15u);  // row misalignment
+  uint32_t w[4];
+  if (m == 0 && site0 + 16 <= n_sites) {
+    return *reinterpret_cast<const uint4*>(src);
+  } else if (site0 + 16 <= n_sites && src - m >= mat && src - m + 32 <= mat_end) {
+    // two aligned 16-byte loads around the segment, then a funnel shift
+    // (named values, not arrays: a select between neighbouring array
+    // elements would become an indexed load through scratch memory)
+    const uint4* base = reinterpret_cast<const uint4*>(src - m);
+    const uint4 q0 = base[0], q1 = base[1];
+    const bool s4 = (m & 4) != 0, s8 = (m & 8) != 0;
+    const uint32_t t0 = s4 ? q0.y : q0.x, t1 = s4 ? q0.z : q0.y, t2 = s4 ? q0.w : q0.z,
+                   t3 = s4 ? q1.x : q0.w, t4 = s4 ? q1.y : q1.x, t5 = s4 ? q1.z : q1.y,
+                   t6 = s4 ? q1.w : q1.z;
+    const uint32_t u0 = s8 ? t2 : t0, u1 = s8 ? t3 : t1, u2 = s8 ? t4 : t2, u3 = s8 ? t5 : t3,
+                   u4 = s8 ? t6 : t4;
+    const uint32_t sh = (uint32_t)(m & 3);
+    return make_uint4(__builtin_amdgcn_alignbyte(u1, u0, sh), __builtin_amdgcn_alignbyte(u2, u1, sh),
+                      __builtin_amdgcn_alignbyte(u3, u2, sh), __builtin_amdgcn_alignbyte(u4, u3, sh));
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) w[e] = 0xFEFEFEFEu;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const uint32_t v = site0 + e < n_sites ? (uint8_t)src[e] : 0xFEu;
+      w[e >> 2] = (w[e >> 2] & ~(0xFFu << (8 * (e & 3)))) | (v << (8 * (e & 3)));
+    }
+  }
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
 
 __global__ void __launch_bounds__(256) pack_kernel(
     const int8_t* __restrict__ mat, int64_t n_sites, int64_t ld,
     const int32_t* __restrict__ rows, int32_t row_begin, int32_t pop_size, int32_t mode,
     int32_t n_feat, int32_t ksteps, int64_t pop_off, int64_t block_bytes, int64_t n_blocks,
     const int8_t* __restrict__ mat_end, int8_t* __restrict__ panels) {
-  constexpr int PITCH = 64 + 16;                              // bytes per site row
-  constexpr int SITES = 32 * PACK_SB, GROUPS = SITES / 16;
-  __shared__ __attribute__((aligned(16))) int8_t tile[SITES * PITCH];  // [site][sample]
+  constexpr int SITES = 32 * PACK_SB;
+  __shared__ uint32_t tile[SITES * PACK_PITCH];  // [site][4-sample dword]
   const int64_t blk0 = (int64_t)blockIdx.x * PACK_SB;
   const int ks = blockIdx.y;
   const int t = threadIdx.x;
-  // Phase 1: item (sample row s, 16-site group g); consecutive threads take
-  // consecutive groups of one row.
-  for (int it = t; it < 64 * GROUPS; it += 256) {
-    const int s = it / GROUPS, g = it % GROUPS;
-    const int k = ks * 64 + s;
+  {
+    // Phase 1: 16 row quads x 16 site groups = 256 threads
+    const int j = t & 15, g = t >> 4;
     const int64_t site0 = blk0 * 32 + g * 16;
-    uint32_t w[4] = {0xFEFEFEFEu, 0xFEFEFEFEu, 0xFEFEFEFEu, 0xFEFEFEFEu};  // -2: padding
-    if (k < pop_size) {
-      const int8_t* src = mat + (int64_t)rows[row_begin + k] * ld + site0;
-      const int m = (int)(reinterpret_cast<uintptr_t>(src) & 15u);  // row misalignment
-      if (m == 0 && site0 + 16 <= n_sites) {
-        const uint4 q = *reinterpret_cast<const uint4*>(src);
-        w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
-      } else if (site0 + 16 <= n_sites && src - m >= mat && src - m + 32 <= mat_end) {
-        // two aligned 16-byte loads around the segment, then a funnel shift
-        const uint4* base = reinterpret_cast<const uint4*>(src - m);
-        const uint4 q0 = base[0], q1 = base[1];
-        const uint32_t d[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-        uint32_t t[7], u[5];
+    uint32_t w[4][4];
 #pragma unroll
-        for (int e = 0; e < 7; ++e) t[e] = (m & 4) ? d[e + 1] : d[e];
+    for (int i = 0; i < 4; ++i) {
+      const int k = ks * 64 + 4 * j + i;
+      const uint4 r = k < pop_size ? pack_load16(mat, mat_end,
+                                                 mat + (int64_t)rows[row_begin + k] * ld + site0,
+                                                 site0, n_sites)
+                                   : make_uint4(0xFEFEFEFEu, 0xFEFEFEFEu, 0xFEFEFEFEu,
+                                                0xFEFEFEFEu);  // -2: padding samples
+      w[i][0] = r.x; w[i][1] = r.y; w[i][2] = r.z; w[i][3] = r.w;
+    }
+    // byte b of dword q of row i -> byte i of the dword of site 4q + b
 #pragma unroll
-        for (int e = 0; e < 5; ++e) u[e] = (m & 8) ? t[e + 2] : t[e];
+    for (int q = 0; q < 4; ++q) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) w[e] = __builtin_amdgcn_alignbyte(u[e + 1], u[e], m & 3);
-      } else {
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-          const uint32_t v = site0 + e < n_sites ? (uint8_t)src[e] : 0xFEu;
-          w[e >> 2] = (w[e >> 2] & ~(0xFFu << (8 * (e & 3)))) | (v << (8 * (e & 3)));
-        }
+      for (int b = 0; b < 4; ++b) {
+        const uint32_t v = ((w[0][q] >> (8 * b)) & 0xFFu) | (((w[1][q] >> (8 * b)) & 0xFFu) << 8) |
+                           (((w[2][q] >> (8 * b)) & 0xFFu) << 16) |
+                           (((w[3][q] >> (8 * b)) & 0xFFu) << 24);
+        tile[(g * 16 + 4 * q + b) * PACK_PITCH + j] = v;
       }
     }
-#pragma unroll
-    for (int e = 0; e < 16; ++e)
-      tile[(g * 16 + e) * PITCH + s] = (int8_t)(w[e >> 2] >> (8 * (e & 3)));
   }
   __syncthreads();
   // Phase 2: (block, feature, lane) -> 16 bytes = 32 samples of one site.
@@ -155,10 +185,10 @@ __global__ void __launch_bounds__(256) pack_kernel(
     const int lane = w & 63, f = (w >> 6) % n_feat, bl = (w >> 6) / n_feat;
     if (blk0 + bl >= n_blocks) continue;
     const int site = bl * 32 + (lane & 31);
-    const int s0 = (lane >> 5) * 32;
-    const uint4* src = reinterpret_cast<const uint4*>(tile + site * PITCH + s0);
-    const uint4 lo = src[0], hi = src[1];
-    const uint32_t x[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};  // samples 4j..4j+3
+    const uint32_t* src = tile + site * PACK_PITCH + (lane >> 5) * 8;
+    uint32_t x[8];  // samples s0 + 4j .. s0 + 4j + 3, s0 = 32 (lane >> 5)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) x[e] = src[e];
     // feature code: hap {0: [v==1], 1: [v>=0]}; geno {0: [v==1], 1: [v==2], 2: [v>=0]}
     const bool valid_f = f == n_feat - 1 && f > 0 && (mode == PGLD_MODE_HAP ? f == 1 : f == 2);
     const uint32_t c = (mode == PGLD_MODE_GENO && f == 1) ? 0x02020202u : 0x01010101u;
