@@ -183,32 +183,50 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
       w1[r] = (uint32_t)(v_h - n11) | ((uint32_t)(v_v - h_v - v_h + n11) << 16);
     }
   } else if (a.missing[p] && !TWO_PASS) {
-    v16f hh = zero16f(), hv = zero16f(), vh = zero16f(), vv = zero16f();
+    v16f hh, hv, vh, vv;
     const v4i* Av = A + ks * 64;
     const v4i* Bv = B + ks * 64;
     // one k-step ahead: the registers a second step would hold cost more
-    // (spills, occupancy of the epilogue) than its latency cover gains
-    constexpr int PF = 1;
-    v4i ah[PF], av[PF], bh[PF], bv[PF];
-#pragma unroll
-    for (int d = 0; d < PF; ++d) {
-      if (d < ks) {
-        ah[d] = A[d * 64]; av[d] = Av[d * 64]; bh[d] = B[d * 64]; bv[d] = Bv[d * 64];
-      }
-    }
-    for (int s0 = 0; s0 < ks; s0 += PF) {
-#pragma unroll
-      for (int d = 0; d < PF; ++d) {
-        const int s = s0 + d;
-        if (s < ks) {
-          hh = pgld_mfma_fp4(ah[d], bh[d], hh);
-          hv = pgld_mfma_fp4(ah[d], bv[d], hv);
-          vh = pgld_mfma_fp4(av[d], bh[d], vh);
-          vv = pgld_mfma_fp4(av[d], bv[d], vv);
-          if (s + PF < ks) {
-            const int n = (s + PF) * 64;
-            ah[d] = A[n]; av[d] = Av[n]; bh[d] = B[n]; bv[d] = Bv[n];
+    // (spills, occupancy of the epilogue) than its latency cover gains.  The
+    // next k-step's loads go to the operand registers as soon as the MFMAs of
+    // this step have read them.
+    if constexpr (RING == 0) {
+      // the first k-step starts from zero accumulators (an inline constant
+      // of the MFMA), so the 64 accumulator registers need no clearing
+      if (ks > 0) {
+        v4i ah = A[0], av = Av[0], bh = B[0], bv = Bv[0];
+        hh = pgld_mfma_fp4(ah, bh, zero16f());
+        hv = pgld_mfma_fp4(ah, bv, zero16f());
+        vh = pgld_mfma_fp4(av, bh, zero16f());
+        vv = pgld_mfma_fp4(av, bv, zero16f());
+        if (ks > 1) { ah = A[64]; av = Av[64]; bh = B[64]; bv = Bv[64]; }
+        for (int s = 1; s < ks; ++s) {
+          hh = pgld_mfma_fp4(ah, bh, hh);
+          hv = pgld_mfma_fp4(ah, bv, hv);
+          vh = pgld_mfma_fp4(av, bh, vh);
+          vv = pgld_mfma_fp4(av, bv, vv);
+          if (s + 1 < ks) {
+            const int n = (s + 1) * 64;
+            ah = A[n]; av = Av[n]; bh = B[n]; bv = Bv[n];
           }
+        }
+      } else {
+        hh = hv = vh = vv = zero16f();
+      }
+    } else {
+      // fallback of the LDS ring traits: the form their register allocation
+      // was measured with
+      hh = hv = vh = vv = zero16f();
+      v4i ah, av, bh, bv;
+      if (ks > 0) { ah = A[0]; av = Av[0]; bh = B[0]; bv = Bv[0]; }
+      for (int s = 0; s < ks; ++s) {
+        hh = pgld_mfma_fp4(ah, bh, hh);
+        hv = pgld_mfma_fp4(ah, bv, hv);
+        vh = pgld_mfma_fp4(av, bh, vh);
+        vv = pgld_mfma_fp4(av, bv, vv);
+        if (s + 1 < ks) {
+          const int n = (s + 1) * 64;
+          ah = A[n]; av = Av[n]; bh = B[n]; bv = Bv[n];
         }
       }
     }
@@ -300,39 +318,48 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
   const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[p]) + lane;
   const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[p]) + lane;
   const int fs = ks * 64;  // feature stride in v4i
-  v16f bb = zero16f(), bc = zero16f(), cbm = zero16f(), cc = zero16f();
+  v16f bb, bc, cbm, cc;
   if (a.missing[p]) {
-    v16f bv = zero16f(), cv = zero16f(), vb = zero16f(), vc = zero16f(), vv = zero16f();
+    v16f bv, cv, vb, vc, vv;
     // one k-step ahead: nine accumulators leave no registers for more (two
-    // steps ahead spilled and was no faster)
-    constexpr int PF = 1;
-    v4i fa[PF][3], fb[PF][3];
+    // steps ahead spilled and was no faster).  The first k-step starts from
+    // zero accumulators (an inline constant of the MFMA), so the 144
+    // accumulator registers need no clearing; the next step's loads go to the
+    // operand registers once this step's MFMAs have read them.
+    if (ks > 0) {
+      v4i fa[3], fb[3];
 #pragma unroll
-    for (int d = 0; d < PF; ++d)
-      if (d < ks)
+      for (int f = 0; f < 3; ++f) { fa[f] = A[f * fs]; fb[f] = B[f * fs]; }
+      bb = pgld_mfma_fp4(fa[0], fb[0], zero16f());
+      bc = pgld_mfma_fp4(fa[0], fb[1], zero16f());
+      bv = pgld_mfma_fp4(fa[0], fb[2], zero16f());
+      cbm = pgld_mfma_fp4(fa[1], fb[0], zero16f());
+      cc = pgld_mfma_fp4(fa[1], fb[1], zero16f());
+      cv = pgld_mfma_fp4(fa[1], fb[2], zero16f());
+      vb = pgld_mfma_fp4(fa[2], fb[0], zero16f());
+      vc = pgld_mfma_fp4(fa[2], fb[1], zero16f());
+      vv = pgld_mfma_fp4(fa[2], fb[2], zero16f());
+      if (ks > 1)
 #pragma unroll
-        for (int f = 0; f < 3; ++f) { fa[d][f] = A[f * fs + d * 64]; fb[d][f] = B[f * fs + d * 64]; }
-    for (int s0 = 0; s0 < ks; s0 += PF) {
+        for (int f = 0; f < 3; ++f) { fa[f] = A[f * fs + 64]; fb[f] = B[f * fs + 64]; }
+      for (int s = 1; s < ks; ++s) {
+        bb = pgld_mfma_fp4(fa[0], fb[0], bb);
+        bc = pgld_mfma_fp4(fa[0], fb[1], bc);
+        bv = pgld_mfma_fp4(fa[0], fb[2], bv);
+        cbm = pgld_mfma_fp4(fa[1], fb[0], cbm);
+        cc = pgld_mfma_fp4(fa[1], fb[1], cc);
+        cv = pgld_mfma_fp4(fa[1], fb[2], cv);
+        vb = pgld_mfma_fp4(fa[2], fb[0], vb);
+        vc = pgld_mfma_fp4(fa[2], fb[1], vc);
+        vv = pgld_mfma_fp4(fa[2], fb[2], vv);
+        if (s + 1 < ks) {
+          const int n = (s + 1) * 64;
 #pragma unroll
-      for (int d = 0; d < PF; ++d) {
-        const int s = s0 + d;
-        if (s < ks) {
-          bb = pgld_mfma_fp4(fa[d][0], fb[d][0], bb);
-          bc = pgld_mfma_fp4(fa[d][0], fb[d][1], bc);
-          bv = pgld_mfma_fp4(fa[d][0], fb[d][2], bv);
-          cbm = pgld_mfma_fp4(fa[d][1], fb[d][0], cbm);
-          cc = pgld_mfma_fp4(fa[d][1], fb[d][1], cc);
-          cv = pgld_mfma_fp4(fa[d][1], fb[d][2], cv);
-          vb = pgld_mfma_fp4(fa[d][2], fb[d][0], vb);
-          vc = pgld_mfma_fp4(fa[d][2], fb[d][1], vc);
-          vv = pgld_mfma_fp4(fa[d][2], fb[d][2], vv);
-          if (s + PF < ks) {
-            const int n = (s + PF) * 64;
-#pragma unroll
-            for (int f = 0; f < 3; ++f) { fa[d][f] = A[f * fs + n]; fb[d][f] = B[f * fs + n]; }
-          }
+          for (int f = 0; f < 3; ++f) { fa[f] = A[f * fs + n]; fb[f] = B[f * fs + n]; }
         }
       }
+    } else {
+      bb = bc = cbm = cc = bv = cv = vb = vc = vv = zero16f();
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
@@ -345,6 +372,7 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
       geno_pack_slot(w, r, c);
     }
   } else {
+    bb = bc = cbm = cc = zero16f();
     constexpr int PF = 2;
     v4i fa[PF][2], fb[PF][2];
 #pragma unroll
