@@ -65,6 +65,21 @@ def traffic_from_profile(name):
     return None
 
 
+def fp64_flops_from_profile(name):
+    """FP64 vector FLOPs per launch of the kernel ((2 FMA + ADD + MUL) x 64
+    lanes, from the rocprofv3 SQ_INSTS_VALU_*_F64 pass in
+    profiles/<name>_fp64_valu.md; None if not profiled).  The count is fixed
+    by the workload, so it pairs with the live kernel time."""
+    path = os.path.join(ROOT, "profiles", f"{name}_fp64_valu.md")
+    try:
+        for line in open(path):
+            if line.startswith("- fp64_flops_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def utilisation_from_profile(name):
     """MFMA / VALU utilisation of the fused kernel recorded by the rocprofv3
     counter pass in profiles/<name>_utilisation.md (None if not profiled)."""
@@ -366,6 +381,11 @@ def main():
     prof = f"{'windows' if windows else 'fused'}_{w.mode}{pan.P}_{w.name}"
     traffic = traffic_from_profile(prof) if args.scale == 1.0 else None
     util = utilisation_from_profile(prof) if args.scale == 1.0 else None
+    f64 = fp64_flops_from_profile(prof) if args.scale == 1.0 else None
+    fp64_valu = None if f64 is None else {
+        "achieved": f64 / (k_ms * 1e-3) / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        "frac": f64 / (k_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, "flops_per_launch": f64,
+        "profile": f"profiles/{prof}_fp64_valu.md"}
     line = {
         "metric": "site-pairs/sec binned DD/Dz/pi2, I8-MFMA fused vs plain FP64; max rel err",
         "value": n_pairs / (ms_step * 1e-3),
@@ -393,6 +413,8 @@ def main():
                      "traffic_profile": f"profiles/{prof}_hbm_traffic.md" if traffic else None,
                      "kernel": kname, "ops_per_pair": ops_per_pair, "kernel_ms": k_ms,
                      "counters": util,
+                     # the estimator epilogue's bound: FP64 vector work
+                     "fp64_valu": fp64_valu,
                      "counters_profile": f"profiles/{prof}_utilisation.md" if util else None},
         "pack": {"ms": pack_ms, "bytes": pack_bytes,
                  "GB/s": pack_bytes / (pack_ms * 1e-3) / 1e9,
