@@ -418,11 +418,21 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
 // ---------------------------------------------------------------------------
 // Traits: what a tile's counts are and how a pair's statistics follow.
 // ---------------------------------------------------------------------------
+// Entry rt_off + max(n - rt_lo, 0): n below the table's range only occurs in
+// padded slots (n = 0), whose statistics are weighted out.  FOLD adds the
+// offset before the clamp (two instructions fewer per population and pair;
+// the four-population kernel keeps the other form, which it spills less with).
+template <bool FOLD = true>
 __device__ __forceinline__ const double* rt_entry(const FusedParams& a, const double* srt, int p,
                                                   int n) {
-  int e = n - a.rt_lo[p];
-  e = e < 0 ? 0 : e;
-  return srt + 4 * (a.rt_off[p] + e);
+  if constexpr (FOLD) {
+    const int e = n + (a.rt_off[p] - a.rt_lo[p]);
+    return srt + 4 * (e < a.rt_off[p] ? a.rt_off[p] : e);
+  } else {
+    int e = n - a.rt_lo[p];
+    e = e < 0 ? 0 : e;
+    return srt + 4 * (a.rt_off[p] + e);
+  }
 }
 
 // Statistic groups of a traits class T: groups 0 .. NG-1 hold the LD
@@ -477,7 +487,7 @@ struct HapTraits {
     for (int p = 0; p < P; ++p) {
       const int n11 = c.w0[p][r] & 0xffff, n10 = c.w0[p][r] >> 16;
       const int n01 = c.w1[p][r] & 0xffff, n00 = c.w1[p][r] >> 16;
-      pp[p] = hap_pop(n11, n10, n01, n00, 0, rt_entry(a, srt, p, n11 + n10 + n01 + n00));
+      pp[p] = hap_pop(n11, n10, n01, n00, 0, rt_entry<(P < 4)>(a, srt, p, n11 + n10 + n01 + n00));
     }
     if constexpr (G < NG) {
       hap_stats_group<P, G>(pp, x);
@@ -496,7 +506,7 @@ struct HapTraits {
       for (int p = 0; p < P; ++p) {
         const int n11 = c.w0[p][r] & 0xffff, n10 = c.w0[p][r] >> 16;
         const int n01 = c.w1[p][r] & 0xffff, n00 = c.w1[p][r] >> 16;
-        pp[p] = hap_pop(n11, n10, n01, n00, 0, rt_entry(a, srt, p, n11 + n10 + n01 + n00));
+        pp[p] = hap_pop(n11, n10, n01, n00, 0, rt_entry<(P < 4)>(a, srt, p, n11 + n10 + n01 + n00));
       }
       hap_acc_group<P, G>(pp, acc, wt);
       if constexpr (MERGE_R2) {
