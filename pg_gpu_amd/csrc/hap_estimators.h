@@ -39,6 +39,29 @@ __device__ __forceinline__ void hap_basis(HapPop& p) {
   p.n = p.pA + p.qA;
 }
 
+// The same population from the marginals the indicator products give
+// directly: c1 = n11 (H.H), pA = n11 + n10 (H.V), pB = n11 + n01 (V.H) and
+// n = n11 + n10 + n01 + n00 (V.V); rt is the table entry of n.  Every field is
+// the same integer as hap_pop's (exact in FP64 whatever the grouping).
+__device__ __forceinline__ HapPop hap_pop_marg(int hh, int hv, int vh, int vv,
+                                               const double* __restrict__ rt) {
+  HapPop p;
+  p.c1 = pgld_count_f64(hh); p.pA = pgld_count_f64(hv);
+  p.pB = pgld_count_f64(vh); p.n = pgld_count_f64(vv);
+  p.qA = p.n - p.pA; p.qB = p.n - p.pB;
+  p.c2 = p.pA - p.c1; p.c3 = p.pB - p.c1; p.c4 = p.qA - p.c3;
+  p.x = p.c1 * p.c4; p.y = p.c2 * p.c3;
+  p.D = p.y - p.x;
+  p.s14 = p.c1 + p.c4; p.s23 = p.c2 + p.c3;
+  p.uA = p.qA - p.pA; p.uB = p.qB - p.pB;
+  p.PA = p.pA * p.qA; p.PB = p.pB * p.qB;
+  p.w = fma(p.uA, p.uB, p.s23 - p.s14);
+  const double2* t = reinterpret_cast<const double2*>(rt);
+  const double2 a = t[0], b = t[1];
+  p.r1 = a.x; p.r2 = a.y; p.r3 = b.x; p.r4 = b.y;
+  return p;
+}
+
 // rtab holds 4 doubles per n: 1/(n)_1, 1/(n)_2, 1/(n)_3, 1/(n)_4.
 __device__ __forceinline__ HapPop hap_pop(int n11, int n10, int n01, int n00, int n,
                                           const double* __restrict__ rtab) {
