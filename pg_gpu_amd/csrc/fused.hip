@@ -172,10 +172,10 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
   // MARG: slot words hold the products' marginals (n11 | pA << 16, pB | n << 16
   // with pA = n11 + n10, pB = n11 + n01, n the valid pairs); otherwise the
   // four class counts (n11 | n10 << 16, n01 | n00 << 16).  Measured per
-  // traits: marginals for two and three populations (C1 3.36 -> 3.19 ms);
-  // class counts for one (the ragged-window kernel 3.17 -> 3.29 ms with
-  // marginals) and for the LDS ring traits.
-  constexpr bool MARG = RING == 0 && !TWO_PASS;
+  // traits: marginals for two to four populations (C1 3.36 -> 3.19 ms, C4
+  // 169.3 -> 166.9 ms); class counts for one (the ragged-window kernel
+  // 3.17 -> 3.29 ms with marginals).
+  constexpr bool MARG = !TWO_PASS;
   const int ks = a.ksteps[p];
   const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[p]) + lane;
   const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[p]) + lane;
@@ -186,8 +186,8 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int n11 = (int)hh[r], h_v = (int)hv[r], v_h = (int)vh[r], v_v = (int)vv[r];
-      w0[r] = (uint32_t)n11 | ((uint32_t)(h_v - n11) << 16);
-      w1[r] = (uint32_t)(v_h - n11) | ((uint32_t)(v_v - h_v - v_h + n11) << 16);
+      w0[r] = (uint32_t)n11 | ((uint32_t)h_v << 16);
+      w1[r] = (uint32_t)v_h | ((uint32_t)v_v << 16);
     }
   } else if (a.missing[p] && !TWO_PASS) {
     v16f hh, hv, vh, vv;
@@ -497,7 +497,7 @@ struct HapTraits {
   __device__ __forceinline__ static HapPop pop(const FusedParams& a, const double* srt,
                                                const Counts& c, int p, int r) {
     const uint32_t w0 = c.w0[p][r], w1 = c.w1[p][r];
-    if constexpr (RING == 0 && MINB < 3) {  // tile_counts' MARG
+    if constexpr (MINB < 3) {  // tile_counts' MARG
       const int vv = (int)(w1 >> 16);
       return hap_pop_marg((int)(w0 & 0xffff), (int)(w0 >> 16), (int)(w1 & 0xffff), vv,
                           rt_entry(a, srt, p, vv));
