@@ -184,3 +184,90 @@ def test_smoke_case_runs_on_cpu():
     w, ref, cnt, n_enum, absr = ge.smoke_case()
     assert ref.shape == (len(w.bins) - 1, 15) and cnt.sum() <= n_enum and n_enum > 0
     assert np.all(absr >= np.abs(ref) * (1 - 1e-12))
+
+
+def _header_constants():
+    """{raw call: k} for every `hap_X_num = hap_X_raw(...) * k` of the device
+    header (the constants the fused kernel moves to the limb conversion)."""
+    src = (ROOT / "pg_gpu_amd/csrc/hap_estimators.h").read_text()
+    pat = re.compile(r"double (hap_\w+)_num\([^)]*\)\s*\{\s*return \1_raw\([^;]*\) \* ([0-9.]+);")
+    return {name: float(k) for name, k in pat.findall(src)}
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4])
+def test_generated_accumulation_reproduces_reference_stats(P):
+    """Evaluates the generated hap_acc_P*_g* bodies and hap_scale_group tables
+    of stats_hap.inc on oracle numerators: raw numerators from the header's
+    constants, reciprocal products, the FMA chains and the 2^-e limb scale
+    together must give the reference's statistics (a dropped 1/2 or a wrong
+    exponent fails by a factor of two)."""
+    gen = load_gen()
+    inner = _header_constants()
+    assert set(gen.HAP_INNER) | {"hap_dz_iik_iji_avg"} == set(inner)
+    assert all(inner[k] == v for k, v in gen.HAP_INNER.items())
+    assert inner["hap_dz_iik_iji_avg"] == gen.HAP_AVG_INNER
+    inc = (ROOT / "pg_gpu_amd/csrc/stats_hap.inc").read_text()
+    rng = np.random.default_rng(40 + P)
+    counts = [rng.integers(1, 25, (200, 4)) for _ in range(P)]
+    pops = [o.HapPop(c) for c in counts]
+    ns = [c.sum(axis=1).astype(np.float64) for c in counts]
+
+    def recip(p, k):
+        return 1.0 / np.prod([ns[p] - j for j in range(k)], axis=0)
+
+    def ref_call(fn, args):  # the reference value num * R of one raw call
+        sub = [pops[x] for x in args]
+        idx = list(range(len(args)))
+        if fn.startswith("hap_dd"):
+            return o.hap_dd(sub, idx[0], idx[-1])
+        if fn.startswith("hap_dz"):
+            i, j, k = {"same": (0, 0, 0), "iik": (0, 0, 1), "iji": (0, 1, 0), "ijj": (0, 1, 1),
+                       "ijk": (0, 1, 2)}[fn[7:]]
+            return o.hap_dz(sub, i, j, k)
+        q = {"same": (0, 0, 0, 0), "triple": (0, 0, 0, 1), "iikk": (0, 0, 1, 1),
+             "iikl": (0, 0, 1, 2), "ijkk": (0, 1, 2, 2), "ijij": (0, 1, 0, 1),
+             "shared": (0, 1, 0, 2), "alldiff": (0, 1, 2, 3)}[fn[8:]]
+        return o.hap_pi2(sub, *q)
+
+    def num(fn, args):  # reference numerator: value / prod 1/(n)_k
+        r = np.prod([recip(args[pos], k) for pos, k in gen.HAP_RECIP[fn]], axis=0)
+        return ref_call(fn, args) / r
+
+    ref = o.multi_pop_stats_hap(counts, [None] * P)
+    got, mag = [], []
+    for k in range(int(re.search(rf"#define HAP_NGROUPS_P{P} (\d+)", inc).group(1))):
+        body = re.search(rf"void hap_acc_P{P}_g{k}\(.*?\n(.*?)\n\}}", inc, re.S).group(1)
+        env = {}
+        for name, expr in re.findall(r"const double (R\d+) = (.*?) \* wt;", body):
+            env[name] = np.prod([recip(int(p), int(kk))
+                                 for p, kk in re.findall(r"p\[(\d)\]\.r(\d)", expr)], axis=0)
+        for name, fn, form, args in re.findall(
+                r"const double ([nm]\d+) = (hap_\w+?)_(num|raw)\(([^)]*)\);", body):
+            a = [int(x) for x in re.findall(r"p\[(\d)\]", args)]
+            if fn == "hap_dz_iik_iji_avg":
+                v = 0.5 * (num("hap_dz_iik", a) + num("hap_dz_iji", a))
+            else:
+                v = num(fn, a)
+            env[name] = v / inner[fn] if form == "raw" else v
+        accs, absl = {}, {}
+        for s, chain in re.findall(r"acc\[(\d+)\] = (fma\(.*\));", body):
+            terms = re.findall(r"fma\(((?:(?!fma\().)*?), (R\d+), ", chain)
+            vals = [eval(e, {}, dict(env)) * env[r] for e, r in terms]
+            accs[int(s)] = sum(vals)
+            absl[int(s)] = sum(np.abs(v) for v in vals)
+        words = re.search(rf"hap_scale_group<{P}, {k}>\(int q\) \{{\n  const unsigned long long w = "
+                          r"(.*?);", inc).group(1)
+        w = [int(x, 16) for x in re.findall(r"0x([0-9a-f]+)ull", words)]
+        for s in sorted(accs):
+            e = (w[s // 16] >> (4 * (s % 16))) & 15
+            got.append(accs[s] * 2.0 ** -e)
+            mag.append(absl[s] * 2.0 ** -e)
+    got, mag = np.stack(got, axis=1), np.stack(mag, axis=1)
+    assert got.shape == ref.shape
+    # relative to the terms' own magnitude where orientations cancel, with a
+    # floor at 1e-4 of the statistic's largest value (the numerators here are
+    # recovered from rounded reference values, which need not vanish exactly
+    # where the reference's own sum does)
+    floor = 1e-4 * np.max(np.abs(ref), axis=0)
+    scale = np.maximum(np.maximum(np.abs(ref), mag), np.maximum(floor, 1e-300))
+    assert np.max(np.abs(got - ref) / scale) < 1e-12
