@@ -801,8 +801,15 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
   for (int q = 0; q < GS; ++q) acc[q] = 0.0;
   if (t.fast) {
     // every pair of the tile is valid and falls in bin bmin
+    if constexpr (T::NP <= 2) {
+      // two slots per iteration give the FP64 chains of two pairs to
+      // interleave (C1 3.21 -> 3.18 ms); more populations spill with it
+#pragma unroll 2
+      for (int r = 0; r < 16; ++r) T::template accum<G>(a, t.srt, cts, r, acc, 1.0);
+    } else {
 #pragma unroll 1
-    for (int r = 0; r < 16; ++r) T::template accum<G>(a, t.srt, cts, r, acc, 1.0);
+      for (int r = 0; r < 16; ++r) T::template accum<G>(a, t.srt, cts, r, acc, 1.0);
+    }
     reduce_to_limbs<T, G, NS, GS, S0, G == 0>(t, acc, t.bmin, 1024ull);
   } else if (t.dual) {
     // Two bins: each pair's statistics go to both accumulators with 0/1
