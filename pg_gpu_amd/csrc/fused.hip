@@ -795,15 +795,17 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
   constexpr int NS = T::NSO;
   constexpr int GS = G < T::NG ? T::GS : 2;
   constexpr int S0 = G < T::NG ? G * T::GS : T::NS;
+  // slot loops unrolled by two (two pairs' FP64 chains interleave) for the
+  // traits that do not spill with it: one population, or two haplotype
+  // populations at two waves per SIMD
+  constexpr bool U2 = T::NP == 1 || (T::NP == 2 && T::MINB == 2);
   const int nb = a.n_bins, lane = t.lane;
   double acc[GS];
 #pragma unroll
   for (int q = 0; q < GS; ++q) acc[q] = 0.0;
   if (t.fast) {
     // every pair of the tile is valid and falls in bin bmin
-    if constexpr (T::NP <= 2) {
-      // two slots per iteration give the FP64 chains of two pairs to
-      // interleave (C1 3.21 -> 3.18 ms); more populations spill with it
+    if constexpr (U2) {  // C1 3.21 -> 3.18 ms
 #pragma unroll 2
       for (int r = 0; r < 16; ++r) T::template accum<G>(a, t.srt, cts, r, acc, 1.0);
     } else {
@@ -823,18 +825,34 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
 #pragma unroll
     for (int q = 0; q < GS; ++q) acc1[q] = 0.0;
     unsigned long long cnt0 = 0, cnt1 = 0;
-#pragma unroll 1
-    for (int r = 0; r < 16; ++r) {
-      const int rq = r >> 2;
-      const uint32_t w = rq == 0 ? bw[0] : rq == 1 ? bw[1] : rq == 2 ? bw[2] : bw[3];
-      const int b = (int)((w >> (8 * (r & 3))) & 0xFFu);
-      const bool in0 = b == t.blo, in1 = b == t.blo + 1;
-      if constexpr (G == 0) {
-        cnt0 += (unsigned long long)__popcll(__ballot(in0));
-        cnt1 += (unsigned long long)__popcll(__ballot(in1));
+    if constexpr (U2) {
+#pragma unroll 2
+      for (int r = 0; r < 16; ++r) {
+        const int rq = r >> 2;
+        const uint32_t w = rq == 0 ? bw[0] : rq == 1 ? bw[1] : rq == 2 ? bw[2] : bw[3];
+        const int b = (int)((w >> (8 * (r & 3))) & 0xFFu);
+        const bool in0 = b == t.blo, in1 = b == t.blo + 1;
+        if constexpr (G == 0) {
+          cnt0 += (unsigned long long)__popcll(__ballot(in0));
+          cnt1 += (unsigned long long)__popcll(__ballot(in1));
+        }
+        T::template accum<G>(a, t.srt, cts, r, acc, in0 ? 1.0 : 0.0);
+        T::template accum<G>(a, t.srt, cts, r, acc1, in1 ? 1.0 : 0.0);
       }
-      T::template accum<G>(a, t.srt, cts, r, acc, in0 ? 1.0 : 0.0);
-      T::template accum<G>(a, t.srt, cts, r, acc1, in1 ? 1.0 : 0.0);
+    } else {
+#pragma unroll 1
+      for (int r = 0; r < 16; ++r) {
+        const int rq = r >> 2;
+        const uint32_t w = rq == 0 ? bw[0] : rq == 1 ? bw[1] : rq == 2 ? bw[2] : bw[3];
+        const int b = (int)((w >> (8 * (r & 3))) & 0xFFu);
+        const bool in0 = b == t.blo, in1 = b == t.blo + 1;
+        if constexpr (G == 0) {
+          cnt0 += (unsigned long long)__popcll(__ballot(in0));
+          cnt1 += (unsigned long long)__popcll(__ballot(in1));
+        }
+        T::template accum<G>(a, t.srt, cts, r, acc, in0 ? 1.0 : 0.0);
+        T::template accum<G>(a, t.srt, cts, r, acc1, in1 ? 1.0 : 0.0);
+      }
     }
     reduce_to_limbs<T, G, NS, GS, S0, G == 0>(t, acc, t.blo, cnt0);
     if (t.blo + 1 < nb) reduce_to_limbs<T, G, NS, GS, S0, G == 0>(t, acc1, t.blo + 1, cnt1);
@@ -1088,7 +1106,8 @@ __device__ __forceinline__ void window_group(const WindowParams& w, const double
   for (int q = 0; q < GS; ++q) acc[q] = 0.0;
   int count = 0;
   const int64_t j = j0 + (lane & 31);
-#pragma unroll 1
+  // two slots per iteration: two pairs' FP64 chains interleave
+#pragma unroll 2
   for (int r = 0; r < 16; ++r) {
     const int64_t i = i0 + mfma_row(lane, r);
     // every slot's statistics are finite (padded sites have zero counts), so
