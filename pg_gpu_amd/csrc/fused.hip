@@ -292,10 +292,15 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
     const int64_t j = cb * 32 + (lane & 31);
     const int kj = j < a.V ? cnt1[j] : 0;
     const int K = a.K[p];
+    // row i = ib + o, o = mfma_row(lane, r) - 4 (lane >> 5) a constant per r:
+    // immediate load offsets from one base, o < lim <=> i < V
+    const int64_t ib = rb * 32 + 4 * (lane >> 5);
+    const int32_t* ci = cnt1 + ib;
+    const int lim = (int)(a.V - ib < 32 ? a.V - ib : 32);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int64_t i = rb * 32 + mfma_row(lane, r);
-      const int ki = i < a.V ? cnt1[i] : 0;
+      const int o = 8 * (r >> 2) + (r & 3);
+      const int ki = o < lim ? ci[o] : 0;
       const int n11 = (int)hh[r];
       if constexpr (MARG) {
         w0[r] = (uint32_t)n11 | ((uint32_t)ki << 16);
@@ -422,10 +427,15 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
     const int64_t j = cb * 32 + (lane & 31);
     const int vbj = j < a.V ? st[j] : 0, vcj = j < a.V ? st[a.V + j] : 0;
     const int K = a.K[p];
+    // rows as in tile_counts: i = ib + o with o constant per r
+    const int64_t ib = rb * 32 + 4 * (lane >> 5);
+    const int32_t* si = st + ib;
+    const int32_t* sci = st + a.V + ib;
+    const int lim = (int)(a.V - ib < 32 ? a.V - ib : 32);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int64_t i = rb * 32 + mfma_row(lane, r);
-      const int bvi = i < a.V ? st[i] : 0, cvi = i < a.V ? st[a.V + i] : 0;
+      const int o = 8 * (r >> 2) + (r & 3);
+      const int bvi = o < lim ? si[o] : 0, cvi = o < lim ? sci[o] : 0;
       const int n11 = (int)bb[r], n12 = (int)bc[r], n21 = (int)cbm[r], n22 = (int)cc[r];
       const int n10 = bvi - n11 - n12, n20 = cvi - n21 - n22;
       const int n01 = vbj - n11 - n21, n02 = vcj - n12 - n22;
@@ -763,16 +773,22 @@ __device__ __forceinline__ void tile_bins(const FusedParams& a, const TileCtx& t
   const double pos_j = jok ? a.pos[j] : 0.0;
   const double co_j = (jok && !a.coord_is_pos) ? a.coord[j] : pos_j;
   int guess = nb - 1;
+  // row i = ib + o with o = 8q + k a constant (mfma_row(lane, 4q + k)): the
+  // loads take immediate offsets from one base and i < j compares o with jr
+  const int64_t ib = t.i0 + 4 * (lane >> 5);
+  const double* pos_i = a.pos + ib;
+  const double* co_i = a.coord + ib;
+  const int jr = (int)(j - ib < 32 ? j - ib : 32);  // o < jr <=> i < j (o < 32)
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     uint32_t w = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int64_t i = t.i0 + 8 * q + 4 * (lane >> 5) + k;  // mfma_row(lane, 4q + k)
+      const int o = 8 * q + k;
       uint32_t b8 = 0xFF;
-      if (jok && i < j) {
-        const double pi = a.pos[i];
-        const double d = a.coord_is_pos ? (pos_j - pi) : fabs(co_j - a.coord[i]);
+      if (jok && o < jr) {
+        const double pi = pos_i[o];
+        const double d = a.coord_is_pos ? (pos_j - pi) : fabs(co_j - co_i[o]);
         const int b = pgld_bin_scan(d, t.sedges, nb, guess, nb - 1);
         if (pos_j <= pi + a.max_dist && b >= 0 && b < nb) {
           b8 = (uint32_t)b;
