@@ -1122,13 +1122,19 @@ __device__ __forceinline__ void window_group(const WindowParams& w, const double
   for (int q = 0; q < GS; ++q) acc[q] = 0.0;
   int count = 0;
   const int64_t j = j0 + (lane & 31);
+  // row i = ib + o with o = 8 (r >> 2) + (r & 3) wave-uniform: lo <= i < j
+  // becomes lo_o <= o < j_o on 32-bit bounds clamped to [-1, 32]
+  const int64_t ib = i0 + 4 * (lane >> 5);
+  const int lo_o = (int)(lo - ib < -1 ? -1 : lo - ib > 32 ? 32 : lo - ib);
+  const int j_o = (int)(j - ib < -1 ? -1 : j - ib > 32 ? 32 : j - ib);
+  const bool jin = j < hi;
   // two slots per iteration: two pairs' FP64 chains interleave
 #pragma unroll 2
   for (int r = 0; r < 16; ++r) {
-    const int64_t i = i0 + mfma_row(lane, r);
+    const int o = 8 * (r >> 2) + (r & 3);
     // every slot's statistics are finite (padded sites have zero counts), so
     // slots outside the window take weight 0 instead of a divergent branch
-    const bool in = full || (i >= lo && i < j && j < hi);
+    const bool in = full || (o >= lo_o && o < j_o && jin);
     T::template accum<G>(w.f, srt, cts, r, acc, in ? 1.0 : 0.0);
     count += in;
   }
