@@ -172,9 +172,10 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
   // MARG: slot words hold the products' marginals (n11 | pA << 16, pB | n << 16
   // with pA = n11 + n10, pB = n11 + n01, n the valid pairs); otherwise the
   // four class counts (n11 | n10 << 16, n01 | n00 << 16).  Measured per
-  // traits: marginals for two to four populations (C1 3.36 -> 3.19 ms, C4
-  // 169.3 -> 166.9 ms); class counts for one (the ragged-window kernel
-  // 3.17 -> 3.29 ms with marginals).
+  // traits: marginals for two to four populations (2 x 500 haplotypes over
+  // 200k sites: 3.36 -> 3.19 ms; 4 populations of 40-800 over 500k sites:
+  // 169.3 -> 166.9 ms); class counts for one (300 haplotypes in 5,000 ragged
+  // windows: 3.17 -> 3.29 ms with marginals).
   constexpr bool MARG = !TWO_PASS;
   const int ks = a.ksteps[p];
   const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[p]) + lane;
@@ -821,7 +822,7 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
   for (int q = 0; q < GS; ++q) acc[q] = 0.0;
   if (t.fast) {
     // every pair of the tile is valid and falls in bin bmin
-    if constexpr (U2) {  // C1 3.21 -> 3.18 ms
+    if constexpr (U2) {  // 2 x 500 haplotypes, 200k sites: 3.21 -> 3.18 ms
 #pragma unroll 2
       for (int r = 0; r < 16; ++r) T::template accum<G>(a, t.srt, cts, r, acc, 1.0);
     } else {
