@@ -397,8 +397,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "i8 counts -> i32 accum, f64 estimators" + (", f32 r^2 ratios" if
-                                                            (r2 or windows) else ""),
+        # counting runs on FP4 (E2M1) 0/1 indicators with FP32 accumulation,
+        # exact for counts below 2^24; the estimators and bin sums are FP64
+        "dtype": ("fp4 indicators -> f32 accum (exact counts), f64 estimators" +
+                  (", f32 r^2 ratios" if (r2 or windows) else "")),
         "data": "synthetic (founder-copying generator, seeded)",
         "config": {"workload": WORKLOADS.get(w.name, w.name) if args.scale == 1.0 else
                    f"{w.name} at scale {args.scale}",
