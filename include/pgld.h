@@ -223,6 +223,39 @@ int pgld_heterozygosity(const pgld_panel_desc* desc, const int32_t* d_site_stats
                         long long* d_out, int32_t* d_nonfinite, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Container passes (once per dataset)
+ * ------------------------------------------------------------------------- */
+
+#define PGLD_FILTER_HAP_BIALLELIC 0   /* no value >= 2, a 0 and a 1 present */
+#define PGLD_FILTER_GENO_BIALLELIC 1  /* 0 < alt < 2 n_valid, n_valid >= 2 */
+
+/* d_keep[s] (uint8, n_sites) = 1 if site s passes `rule` over the selected
+ * rows d_rows[0..n_sel) (NULL: all n_rows rows; entries must lie in
+ * [0, n_rows)), else 0.  PGLD_FILTER_HAP_BIALLELIC replaces the site test of
+ * HaplotypeMatrix.apply_biallelic_filter (pg_gpu/haplotype_matrix.py:751-765);
+ * PGLD_FILTER_GENO_BIALLELIC the one of GenotypeMatrix.apply_biallelic_filter
+ * (pg_gpu/genotype_matrix.py:516-523) and, over the union of the selected
+ * populations' rows, the genotype union filter of moments_ld._compute_ld_sums
+ * (pg_gpu/moments_ld.py:256-276).  The caller compacts the kept columns. */
+int pgld_biallelic_mask(const int8_t* d_mat, int64_t n_rows, int64_t n_sites, int64_t ld,
+                        const int32_t* d_rows, int32_t n_sel, int32_t rule, uint8_t* d_keep,
+                        void* stream);
+
+/* Diploid genotypes from consecutive haplotype pairs (2i, 2i+1):
+ * g = max(h0, 0) + max(h1, 0), -1 if either is missing.  d_geno is
+ * (n_hap / 2) x n_sites with row stride ld_geno.  Replaces the conversion
+ * loop of GenotypeMatrix.from_haplotype_matrix
+ * (pg_gpu/genotype_matrix.py:263-291). */
+int pgld_hap_to_geno(const int8_t* d_hap, int64_t n_hap, int64_t n_sites, int64_t ld_hap,
+                     int8_t* d_geno, int64_t ld_geno, void* stream);
+
+/* Two haplotypes per individual: rows 2i = [g >= 1], 2i+1 = [g >= 2], both -1
+ * where g is missing.  Replaces GenotypeMatrix.to_haplotype_matrix
+ * (pg_gpu/genotype_matrix.py:307-340). */
+int pgld_geno_to_hap(const int8_t* d_geno, int64_t n_ind, int64_t n_sites, int64_t ld_geno,
+                     int8_t* d_hap, int64_t ld_hap, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Per-pair API (not the binned hot path)
  * ------------------------------------------------------------------------- */
 

@@ -128,8 +128,9 @@ def main(ref_root):
     ref = Path(ref_root)
     readable = readable_tables((ref / "pg_gpu/ld_statistics_genotype.py").read_text())
     kern, parts = kernel_tables((ref / "pg_gpu/genotype_kernels.py").read_text())
-    # product-side data for the HIP code generator (pg_gpu_amd/csrc/gen_stats.py)
-    csrc = HERE.parent / "pg_gpu_amd" / "csrc" / "geno_kernel_polys.json"
+    # kernel sub-polynomial tables, checked against the device's centred forms
+    # by tests/test_oracle_geno.py
+    csrc = HERE / "geno_kernel_polys.json"
     csrc.write_text(json.dumps({
         "source": "integer sub-polynomials of the single-population genotype estimators "
                   "(pg_gpu/genotype_kernels.py:27-71, :235-272, :364-405); variables g0_1..g0_9 "

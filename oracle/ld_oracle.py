@@ -870,3 +870,159 @@ def windowed_r_squared(hap, positions, bp_bins, percentile=50, rows=None, estima
         if len(sel):
             res[k] = np.percentile(sel, pct)
     return (res[:, 0] if res.shape[1] == 1 else res), cnt
+
+
+# ---------------------------------------------------------------------------
+# Container passes and the moments driver's host steps
+# (genotype_matrix.py:247-340, :506-539; haplotype_matrix.py:1320-1589;
+#  moments_ld.py:166-245, :256-279, :346-388)
+# ---------------------------------------------------------------------------
+
+
+def hap_to_geno(hap):
+    """GenotypeMatrix.from_haplotype_matrix (genotype_matrix.py:263-291):
+    rows (2i, 2i+1) summed after max(., 0); -1 where either is negative."""
+    h = np.asarray(hap)
+    h1, h2 = h[0::2], h[1::2]
+    g = (np.maximum(h1, 0).astype(np.int16) + np.maximum(h2, 0)).astype(np.int8)
+    g[(h1 < 0) | (h2 < 0)] = -1
+    return g
+
+
+def geno_to_hap(geno):
+    """GenotypeMatrix.to_haplotype_matrix (genotype_matrix.py:319-336)."""
+    g = np.asarray(geno)
+    gg = np.maximum(g, 0)
+    hap = np.zeros((2 * g.shape[0], g.shape[1]), np.int8)
+    hap[0::2] = (gg >= 1)
+    hap[1::2] = (gg >= 2)
+    hap[0::2][g < 0] = -1
+    hap[1::2][g < 0] = -1
+    return hap
+
+
+def biallelic_filter_geno(geno, rows=None):
+    """Kept columns of GenotypeMatrix.apply_biallelic_filter
+    (genotype_matrix.py:516-523); with ``rows`` the union filter of
+    moments_ld._compute_ld_sums over those individuals, each counted once
+    (moments_ld.py:256-276)."""
+    g = np.asarray(geno)
+    if rows is not None:
+        g = g[np.asarray(sorted(set(int(r) for r in rows)), np.int64)]
+    valid = g >= 0
+    alt = np.where(valid, g, 0).astype(np.int64).sum(axis=0)
+    nv = valid.sum(axis=0)
+    return np.nonzero((alt > 0) & (alt < 2 * nv) & (nv >= 2))[0]
+
+
+def heterozygosity_geno(geno, pop_rows):
+    """H_i_j sums over sites for genotype data (moments_ld.py:356-388,
+    genotype branch): alt = sum of valid genotypes, n = 2 x valid
+    individuals, ref = n - alt.  Sites where a term is not finite (a
+    population with fewer than one valid individual, or one for H_i_i)
+    make the sum NaN, as the reference's cp.sum does."""
+    alt, ref, nh = [], [], []
+    for rows in pop_rows:
+        g = np.asarray(geno)[np.asarray(rows, np.int64), :]
+        v = g >= 0
+        a = np.where(v, g, 0).astype(np.int64).sum(axis=0).astype(np.float64)
+        n = 2.0 * v.sum(axis=0).astype(np.float64)
+        alt.append(a)
+        ref.append(n - a)
+        nh.append(n)
+    out = {}
+    P = len(pop_rows)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for i in range(P):
+            for j in range(i, P):
+                if i == j:
+                    v = np.sum(2.0 * ref[i] * alt[i] / (nh[i] * (nh[i] - 1)))
+                else:
+                    v = np.sum((ref[i] * alt[j] + alt[i] * ref[j]) / (nh[i] * nh[j]))
+                out[f"H_{i}_{j}"] = float(v)
+    return out
+
+
+def read_rec_map(path):
+    """(positions, Morgans) from a whitespace map file of (pos, cM) rows;
+    lines that do not parse are skipped (moments_ld.py:211-223)."""
+    mp, mv = [], []
+    with open(path) as fh:
+        for line in fh:
+            parts = line.strip().split()
+            if len(parts) >= 2:
+                try:
+                    mp.append(float(parts[0]))
+                    mv.append(float(parts[1]) / 100.0)
+                except ValueError:
+                    continue
+    return np.array(mp), np.array(mv)
+
+
+def interpolate_genetic_distances(positions, rec_map_file):
+    """Per-variant map position in Morgans (moments_ld.py:211-223)."""
+    mp, mv = read_rec_map(rec_map_file)
+    return np.interp(positions, mp, mv)
+
+
+def max_bp_for_r_dist(positions, gen_dists, max_r):
+    """Conservative bp cutoff for a Morgan cutoff (moments_ld.py:226-245):
+    max_r / (smallest positive local rate) x 1.1, capped at the span."""
+    chrom_len = float(positions[-1] - positions[0])
+    if len(positions) < 2:
+        return chrom_len
+    bp = np.diff(positions).astype(np.float64)
+    rd = np.diff(gen_dists)
+    ok = bp > 0
+    if not np.any(ok):
+        return chrom_len
+    rates = rd[ok] / bp[ok]
+    min_rate = np.min(rates[rates > 0]) if np.any(rates > 0) else max_r / chrom_len
+    return min(max_r / min_rate * 1.1, chrom_len)
+
+
+def moments_ld_stats(mat, positions, pop_rows, mode="hap", bp_bins=None, r_bins=None,
+                     rec_map_file=None):
+    """The {'bins', 'sums', 'stats'} content of moments_ld.compute_ld_statistics
+    (moments_ld.py:148-203) for an already loaded matrix: bp or Morgan bins,
+    the genotype union filter before the pair loop, heterozygosity over the
+    unfiltered matrix.  Also returned: per-bin pair counts, per-bin sums of
+    |value| ("abs", the scale for rounding checks) and the bp cutoff."""
+    pos = np.asarray(positions, np.int64)
+    if r_bins is not None:
+        bins = np.asarray(r_bins, np.float64)
+        gen = interpolate_genetic_distances(pos, rec_map_file)
+        max_bp = max_bp_for_r_dist(pos, gen, float(bins[-1]))
+    else:
+        bins = np.asarray(bp_bins, np.float64)
+        gen = None
+        max_bp = float(bins[-1])
+    P = len(pop_rows)
+    if mode == "geno":
+        keep = biallelic_filter_geno(mat, [r for pr in pop_rows for r in pr])
+        sums, counts, _, absr = binned_sums(np.asarray(mat)[:, keep], pos[keep], pop_rows, bins,
+                                            max_dist=max_bp,
+                                            coords=None if gen is None else gen[keep],
+                                            mode="geno", with_abs=True)
+        het = heterozygosity_geno(mat, pop_rows)
+    else:
+        sums, counts, _, absr = binned_sums(mat, pos, pop_rows, bins, max_dist=max_bp,
+                                            coords=gen, mode="hap", with_abs=True)
+        het = heterozygosity_hap(mat, pop_rows)
+    hn = het_names(P)
+    return {"bins": [(float(bins[i]), float(bins[i + 1])) for i in range(len(bins) - 1)],
+            "sums": [sums[i] for i in range(len(bins) - 1)] + [np.array([het[h] for h in hn])],
+            "stats": (ld_names(P), hn), "counts": counts, "abs": absr, "max_bp": max_bp}
+
+
+def tally_upper(hap, rows=None):
+    """(counts [V(V-1)/2, 4], n_valid or None) of every variant pair i < j in
+    np.triu_indices order (haplotype_matrix.py:1320-1433), counting only
+    haplotypes valid at both variants; n_valid is None when the rows hold no
+    missing call.  (The reference's no-missing branch uses an int8 matmul
+    whose n11 wraps above 127; this restatement keeps exact counts.)"""
+    h = np.asarray(hap)
+    if rows is not None:
+        h = h[np.asarray(rows, np.int64)]
+    ii, jj = np.triu_indices(h.shape[1], 1)
+    return hap_counts(h, ii, jj)

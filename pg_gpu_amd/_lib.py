@@ -17,6 +17,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libpgld.so"
 PGLD_OK, PGLD_EINVAL, PGLD_EDEVICE, PGLD_EUNSUPPORTED = 0, 1, 2, 3
 MODE_HAP, MODE_GENO = 0, 1
 PAIR_SIGMA_D2, PAIR_R2_NAIVE, PAIR_R2_DOSAGE, PAIR_R2 = 0, 1, 2, 3
+FILTER_HAP_BIALLELIC, FILTER_GENO_BIALLELIC = 0, 1
 MAX_POPS = 4
 MAX_BINS = 64
 
@@ -113,6 +114,12 @@ _SIGS = {
                                 c_void_p]),
     "pgld_heterozygosity": (c_int32, [POINTER(PanelDesc), c_void_p, c_void_p, c_void_p,
                                       c_void_p]),
+    "pgld_biallelic_mask": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int32,
+                                      c_int32, c_void_p, c_void_p]),
+    "pgld_hap_to_geno": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64,
+                                   c_void_p]),
+    "pgld_geno_to_hap": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64,
+                                   c_void_p]),
 }
 
 _lib = None

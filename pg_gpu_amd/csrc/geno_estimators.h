@@ -8,7 +8,7 @@
 // estimators are integer polynomials of degree 3-4 in the class counts,
 // evaluated here in centered dosage moments (below) as FP64 expressions whose
 // intermediates are integers, exact for populations below ~10^4 individuals
-// (the reference's sub-polynomials, in pg_gpu_amd/csrc/geno_kernel_polys.json,
+// (the reference's sub-polynomials, tabulated in oracle/geno_kernel_polys.json,
 // are checked against these forms by the test suite).  Divisions by falling
 // factorials use the reciprocal table (0 where the factorial is not
 // positive); where the reference divides without a guard (Dz between

@@ -156,7 +156,7 @@ class BinnedPlan:
     def args(self, out, grid=0, r2=False):
         pan = self.pan
         a = _lib.BinnedArgs()
-        a.d_panels = _lib.ptr(pan.panels).value
+        a.d_panels = _lib.ptr(pan.panels).value if pan.panels is not None else None
         a.d_site_stats = _lib.ptr(pan.site_stats).value
         a.d_mat = _lib.ptr(pan.mat).value
         a.ld = pan.ld
@@ -196,7 +196,7 @@ class BinnedPlan:
             sched = torch.zeros(_lib.SCHED_WORDS, dtype=torch.int32, device=pan.device)
             a.d_sched = _lib.ptr(sched).value
         fn = _lib.lib().pgld_binned_fused if kernel == "fused" else _lib.lib().pgld_binned_plain
-        _lib.check(fn(ctypes.byref(pan.desc), ctypes.byref(a), _lib.stream_handle(pan.stream)))
+        _lib.check(fn(ctypes.byref(pan.desc), ctypes.byref(a), _lib.stream_handle()))
         res = PendingResult(out, self.ns, self.n_pairs, len(self.tiles), r2)
         res._sched = sched  # kept alive until the launch has been waited for
         return res
@@ -277,9 +277,14 @@ class LDPanels:
     mat:      torch int8 tensor [n_rows, n_sites] on a CUDA device (sample-major)
     pop_rows: list (1..4) of row-index lists, one per population
     mode:     "hap" (values 0/1/-1) or "geno" (0/1/2/-1)
+    pack:     False keeps only the per-site tallies (heterozygosity needs no
+              panels)
+
+    Every launch goes to torch's current stream at the time of the call, the
+    stream the buffers are allocated, cleared and read on.
     """
 
-    def __init__(self, mat, pop_rows, mode="hap", stream=None):
+    def __init__(self, mat, pop_rows, mode="hap", pack=True):
         torch = _torch()
         if mat.dtype != torch.int8 or mat.dim() != 2 or not mat.is_cuda:
             raise ValueError("mat must be a 2-D int8 CUDA tensor")
@@ -300,13 +305,12 @@ class LDPanels:
         flat = np.concatenate([np.asarray(r, np.int32) for r in pop_rows]) if sum(sizes) else \
             np.zeros(0, np.int32)
         self.d_rows = torch.as_tensor(flat, dtype=torch.int32, device=self.device)
-        self.stream = stream
         L = _lib.lib()
         desc = self._desc([0] * self.P)
         offs = _lib.int32_array(self.row_offsets)
         self.site_stats = torch.zeros((self.P, 3, self.V), dtype=torch.int32, device=self.device)
         flags = torch.zeros(1, dtype=torch.int32, device=self.device)
-        h = _lib.stream_handle(stream)
+        h = _lib.stream_handle()
         _lib.check(L.pgld_scan_sites(_lib.ptr(self.mat), self.n_rows, self.V, self.ld,
                                      _lib.ptr(self.d_rows), offs, ctypes.byref(desc),
                                      _lib.ptr(self.site_stats), _lib.ptr(flags), h))
@@ -320,6 +324,11 @@ class LDPanels:
               else np.zeros(self.P, np.int64))
         self.n_lo = [max(0, int(k) - 2 * int(m)) for k, m in zip(sizes, mx)]
         self.desc = self._desc(self.has_missing)
+        kmax = max(sizes) if sizes else 0
+        self.rtab = torch.as_tensor(reciprocal_table(kmax), device=self.device)
+        self.panels = None
+        if not pack:
+            return
         nbytes = L.pgld_panel_bytes(ctypes.byref(self.desc))
         if nbytes < 0:
             _lib.check(_lib.PGLD_EINVAL)
@@ -327,8 +336,6 @@ class LDPanels:
         _lib.check(L.pgld_pack(_lib.ptr(self.mat), self.n_rows, self.V, self.ld,
                                _lib.ptr(self.d_rows), offs, ctypes.byref(self.desc),
                                _lib.ptr(self.panels), h))
-        kmax = max(sizes) if sizes else 0
-        self.rtab = torch.as_tensor(reciprocal_table(kmax), device=self.device)
 
     def _desc(self, missing):
         d = _lib.PanelDesc()
@@ -346,10 +353,12 @@ class LDPanels:
 
     def repack(self):
         """Re-run the pack kernel into the existing panel buffer (no allocation)."""
+        if self.panels is None:
+            raise ValueError("these tallies were built without panels (pack=False)")
         offs = _lib.int32_array(self.row_offsets)
         _lib.check(_lib.lib().pgld_pack(_lib.ptr(self.mat), self.n_rows, self.V, self.ld,
                                         _lib.ptr(self.d_rows), offs, ctypes.byref(self.desc),
-                                        _lib.ptr(self.panels), _lib.stream_handle(self.stream)))
+                                        _lib.ptr(self.panels), _lib.stream_handle()))
 
     def plan(self, positions, bins, max_dist=None, coords=None, tiles=None, square=None):
         """Device-side plan of one binning job (positions, edges, band tiles)."""
@@ -377,5 +386,5 @@ class LDPanels:
         out = torch.zeros((q, ni, nj), dtype=torch.int32, device=self.device)
         _lib.check(_lib.lib().pgld_count_block(ctypes.byref(self.desc), _lib.ptr(self.panels),
                                                 _lib.ptr(self.site_stats), pop, i0, ni, j0, nj,
-                                                _lib.ptr(out), _lib.stream_handle(self.stream)))
+                                                _lib.ptr(out), _lib.stream_handle()))
         return out

@@ -128,42 +128,30 @@ class GenotypeMatrix:
     @classmethod
     def from_haplotype_matrix(cls, hap_matrix):
         """Pair haplotype rows (0,1), (2,3), ...; a genotype is missing if
-        either haplotype is (genotype_matrix.py:247-305)."""
-        h = dv.to_numpy(hap_matrix.haplotypes)
-        if h.shape[0] % 2:
-            raise ValueError(
-                f"Need even number of haplotypes for diploid conversion, got {h.shape[0]}")
-        h1, h2 = h[0::2], h[1::2]
-        miss = (h1 < 0) | (h2 < 0)
-        g = (np.maximum(h1, 0).astype(np.int16) + np.maximum(h2, 0)).astype(np.int8)
-        g[miss] = -1
+        either haplotype is (genotype_matrix.py:247-305).  The conversion runs
+        on the GPU (pgld_hap_to_geno); the result stays on the input's device."""
+        from . import filters
+
+        g = filters.hap_to_geno(hap_matrix.haplotypes)
         sets = None
         if hap_matrix._sample_sets is not None:
-            sets = {k: sorted(set(i // 2 for i in v)) for k, v in hap_matrix._sample_sets.items()}
-        gm = cls(g, dv.to_numpy(hap_matrix.positions), hap_matrix.chrom_start,
-                 hap_matrix.chrom_end, sample_sets=sets, n_total_sites=hap_matrix.n_total_sites)
-        if hap_matrix.device == "GPU":
-            gm.transfer_to_gpu()
-        return gm
+            sets = {k: sorted(set(int(i) // 2 for i in v))
+                    for k, v in hap_matrix._sample_sets.items()}
+        return cls(g, hap_matrix.positions, hap_matrix.chrom_start, hap_matrix.chrom_end,
+                   sample_sets=sets, n_total_sites=hap_matrix.n_total_sites,
+                   accessible_mask=hap_matrix.accessible_mask)
 
     def to_haplotype_matrix(self):
-        """Each individual becomes two haplotypes; hets become (1, 0)
-        (genotype_matrix.py:307-340)."""
+        """Each individual becomes two haplotypes: rows 2i = [g >= 1] and
+        2i+1 = [g >= 2], so hets become (1, 0); missing stays -1 in both
+        (genotype_matrix.py:307-340; on the GPU by pgld_geno_to_hap)."""
+        from . import filters
         from .haplotype_matrix import HaplotypeMatrix
 
-        g = dv.to_numpy(self.genotypes)
-        miss = g < 0
-        gg = np.maximum(g, 0)
-        hap = np.zeros((2 * g.shape[0], g.shape[1]), np.int8)
-        hap[0::2] = (gg >= 1).astype(np.int8)
-        hap[1::2] = (gg >= 2).astype(np.int8)
-        hap[0::2][miss] = -1
-        hap[1::2][miss] = -1
-        hm = HaplotypeMatrix(hap, dv.to_numpy(self.positions), self.chrom_start, self.chrom_end,
-                             n_total_sites=self.n_total_sites)
-        if self._device == "GPU":
-            hm.transfer_to_gpu()
-        return hm
+        hap = filters.geno_to_hap(self.genotypes)
+        return HaplotypeMatrix(hap, self.positions, self.chrom_start, self.chrom_end,
+                               n_total_sites=self.n_total_sites,
+                               accessible_mask=self.accessible_mask)
 
     @classmethod
     def from_vcf(cls, path, include_invariant=False, accessible_bed=None):
@@ -206,15 +194,14 @@ class GenotypeMatrix:
 
     def apply_biallelic_filter(self):
         """Keep sites with 0 < alt count < 2 * n_valid and n_valid >= 2
-        (genotype_matrix.py:506-539)."""
-        g = dv.to_numpy(self.genotypes)
-        valid = g >= 0
-        alt = np.where(valid, g, 0).sum(axis=0)
-        nv = valid.sum(axis=0)
-        keep = (alt > 0) & (alt < 2 * nv) & (nv >= 2)
-        gm = GenotypeMatrix(g[:, keep], dv.to_numpy(self.positions)[keep], self.chrom_start,
-                            self.chrom_end, sample_sets=self._sample_sets,
-                            n_total_sites=self.n_total_sites, samples=self.samples)
-        if self._device == "GPU":
-            gm.transfer_to_gpu()
-        return gm
+        (genotype_matrix.py:506-539); the site test runs on the GPU
+        (pgld_biallelic_mask)."""
+        from . import _lib, filters
+
+        g = self.genotypes
+        idx = filters.keep_index(filters.biallelic_keep(g, _lib.FILTER_GENO_BIALLELIC))
+        return GenotypeMatrix(filters.take_columns(g, idx),
+                              filters.take_columns(self.positions, idx), self.chrom_start,
+                              self.chrom_end, sample_sets=self._sample_sets,
+                              n_total_sites=self.n_total_sites, samples=self.samples,
+                              accessible_mask=self.accessible_mask)

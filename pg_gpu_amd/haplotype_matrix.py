@@ -14,6 +14,7 @@ from collections import OrderedDict
 import numpy as np
 
 from . import _device as dv
+from . import _lib
 from .accessible import read_bed_mask
 
 
@@ -149,23 +150,30 @@ class HaplotypeMatrix:
     def from_vcf(cls, path, region=None, samples=None, include_invariant=False,
                  accessible_bed=None):
         """Phased VCF -> haplotypes; allele-0 rows of every sample first, then
-        allele-1 rows (haplotype_matrix.py:292-359)."""
-        from .vcf import read_vcf_gt
+        allele-1 rows (haplotype_matrix.py:292-359).  ``region``
+        ('chrom[:begin-end]', 1-based inclusive) keeps the records inside it
+        and sets chrom_start / chrom_end to its bounds; ``samples`` keeps the
+        named samples, in the file's order."""
+        from .vcf import parse_region, read_vcf_gt
 
-        if region is not None or samples is not None:
-            raise NotImplementedError("region / samples subsetting is not supported")
-        d = read_vcf_gt(path)
+        d = read_vcf_gt(path, region=region, samples=samples)
         gt = d["gt"]
         n = gt.shape[1]
         hap = np.empty((2 * n, gt.shape[0]), dtype=np.int8)
         hap[:n] = gt[:, :, 0].T
         hap[n:] = gt[:, :, 1].T
         pos = d["pos"]
-        hm = cls(hap, pos, int(pos[0]), int(pos[-1]),
+        chrom_start, chrom_end = int(pos[0]), int(pos[-1])
+        chrom = d["chrom"][0]
+        if region is not None:
+            chrom, begin, end = parse_region(region)
+            chrom_start = begin if begin is not None else chrom_start
+            chrom_end = end if end is not None else chrom_end
+        hm = cls(hap, pos, chrom_start, chrom_end,
                  n_total_sites=gt.shape[0] if include_invariant else None,
                  samples=list(d["samples"]))
         if accessible_bed is not None:
-            hm.set_accessible_mask(accessible_bed, chrom=d["chrom"][0])
+            hm.set_accessible_mask(accessible_bed, chrom=chrom)
         return hm
 
     def load_pop_file(self, pop_file, pops=None):
@@ -189,21 +197,108 @@ class HaplotypeMatrix:
     def apply_biallelic_filter(self):
         """Keep variants with a 0 and a 1 and no allele >= 2 across all
         haplotypes (haplotype_matrix.py:724-789)."""
-        h = dv.to_numpy(self.haplotypes)
-        keep = (np.sum(h >= 2, axis=0) == 0) & (np.sum(h == 0, axis=0) > 0) & \
-            (np.sum(h == 1, axis=0) > 0)
-        idx = np.nonzero(keep)[0]
-        if len(idx) == 0:
+        from . import filters
+
+        h = self.haplotypes
+        idx = filters.keep_index(filters.biallelic_keep(h, _lib.FILTER_HAP_BIALLELIC))
+        if idx.numel() == 0:
             # the reference cannot represent an empty matrix either; callers get
             # a placeholder whose binned statistics are all zero
-            return _EmptyHaplotypes(h.shape[0], self._sample_sets)
-        pos = dv.to_numpy(self.positions)[idx]
-        out = HaplotypeMatrix(h[:, idx], pos, int(pos[0]), int(pos[-1]),
-                              sample_sets=self._sample_sets, n_total_sites=self.n_total_sites,
-                              samples=self.samples)
-        if self._device == "GPU":
-            out.transfer_to_gpu()
-        return out
+            return _EmptyHaplotypes(int(h.shape[0]), self._sample_sets)
+        hk = filters.take_columns(h, idx)
+        pos = filters.take_columns(self.positions, idx)
+        p_np = dv.to_numpy(pos[[0, -1]]) if not isinstance(pos, np.ndarray) else pos[[0, -1]]
+        return HaplotypeMatrix(hk, pos, int(p_np[0]), int(p_np[1]),
+                               sample_sets=self._sample_sets, n_total_sites=self.n_total_sites,
+                               samples=self.samples, accessible_mask=self.accessible_mask)
+
+    # ------------------------------------------------------------ pair tallies
+    def _pop_rows(self, pop):
+        if pop is None:
+            return list(range(self.num_haplotypes))
+        if self._sample_sets is None:
+            raise ValueError("sample_sets must be defined to use pop parameter")
+        if pop not in self._sample_sets:
+            raise KeyError(f"Population key {pop} must exist in sample_sets")
+        return [int(i) for i in self._sample_sets[pop]]
+
+    def _upper_counts(self, pan, p):
+        """int32 [V(V-1)/2, 4] (n11, n10, n01, n00) of population p over all
+        pairs i < j in np.triu_indices(V, 1) order, from MFMA counts of
+        32-row stripes of the upper triangle (pgld_count_block)."""
+        t = dv.torch()
+        V = pan.V
+        parts = []
+        stripe = max(32, min(4096, ((1 << 24) // max(V, 1)) // 32 * 32))
+        for i0 in range(0, max(V - 1, 0), stripe):
+            ni = min(stripe, V - i0)
+            blk = pan.count_block(p, i0, ni, i0, V - i0)          # [4, ni, V - i0]
+            upper = t.ones((ni, V - i0), dtype=t.bool, device=pan.device).triu(1)
+            parts.append(blk[:, upper].t())
+        if not parts:
+            return t.zeros((0, 4), dtype=t.int32, device=pan.device)
+        return t.cat(parts).contiguous()
+
+    def tally_gpu_haplotypes(self, pop=None):
+        """(counts int32 [V(V-1)/2, 4] = n11, n10, n01, n00 for every pair of
+        variants i < j, n_valid int32 [V(V-1)/2] or None without missing
+        calls) over the haplotypes of ``pop`` (all if None); pairs in
+        triu_indices(V, 1) order (haplotype_matrix.py:1320-1375).  Counts
+        come from the MFMA counting path and are exact; the reference's
+        no-missing branch multiplies int8 matrices, whose n11 wraps above 127
+        carriers, and that wrap is not reproduced."""
+        from .engine import LDPanels
+
+        if self._device == "CPU":
+            self.transfer_to_gpu()
+        rows = self._pop_rows(pop)
+        pan = LDPanels(self.haplotypes, [rows], "hap")
+        counts = self._upper_counts(pan, 0)
+        n_valid = counts.sum(dim=1, dtype=dv.torch().int32) if pan.has_missing[0] else None
+        return counts, n_valid
+
+    def _tally_gpu_haplotypes_with_missing_impl(self, X):
+        """(counts, n_valid) of all variant pairs over the rows of X, counting
+        only haplotypes valid at both variants (haplotype_matrix.py:1377-1433)."""
+        from .engine import LDPanels
+
+        t = dv.torch()
+        X = dv.as_tensor(X, t.int8)
+        pan = LDPanels(X, [list(range(X.shape[0]))], "hap")
+        counts = self._upper_counts(pan, 0)
+        return counts, counts.sum(dim=1, dtype=t.int32)
+
+    def tally_gpu_haplotypes_two_pops(self, pop1, pop2):
+        """(counts int32 [V(V-1)/2, 8] = (n11, n10, n01, n00) of pop1 then of
+        pop2, n_valid1, n_valid2), the n_valid arrays None when the matrix has
+        no missing call (haplotype_matrix.py:1511-1589)."""
+        from .engine import LDPanels
+
+        if self._device == "CPU":
+            self.transfer_to_gpu()
+        if self._sample_sets is None:
+            raise ValueError("sample_sets must be defined to use this function")
+        if pop1 not in self._sample_sets or pop2 not in self._sample_sets:
+            raise KeyError(f"Population keys {pop1} and {pop2} must exist in sample_sets")
+        t = dv.torch()
+        h = self.haplotypes
+        pan = LDPanels(h, [self._pop_rows(pop1), self._pop_rows(pop2)], "hap")
+        c1, c2 = self._upper_counts(pan, 0), self._upper_counts(pan, 1)
+        counts = t.cat([c1, c2], dim=1)
+        whole = LDPanels(h, [list(range(h.shape[0]))], "hap", pack=False)
+        if not whole.has_missing[0]:
+            return counts, None, None
+        return counts, c1.sum(dim=1, dtype=t.int32), c2.sum(dim=1, dtype=t.int32)
+
+    def tally_gpu_haplotypes_two_pops_with_missing(self, pop1, pop2):
+        """As tally_gpu_haplotypes_two_pops, always returning the valid-pair
+        counts (haplotype_matrix.py:1435-1509)."""
+        counts, n1, n2 = self.tally_gpu_haplotypes_two_pops(pop1, pop2)
+        if n1 is None:
+            t = dv.torch()
+            n1 = counts[:, 0:4].sum(dim=1, dtype=t.int32)
+            n2 = counts[:, 4:8].sum(dim=1, dtype=t.int32)
+        return counts, n1, n2
 
     # ------------------------------------------------------------ binned LD
     def _binned(self, bp_bins, pops, raw):

@@ -82,11 +82,14 @@ def iter_pairs_within_distance(positions, max_dist, chunk_size):
 # ---------------------------------------------------------------------------
 
 _PANEL_CACHE = weakref.WeakKeyDictionary()
+_PANELS_PER_MATRIX = 8   # populations kept packed per matrix (least recently used dropped)
 
 
 def _panels_for(mat, pop_indices, mode):
     """Packed panels of ``mat`` restricted to one population, cached per
-    (tensor, in-place version, rows) so repeated chunk calls pack only once."""
+    (tensor, in-place version, rows): the reference's chunk loop calls the
+    counting once per population and chunk (moments_ld.py:314-318), and each
+    population is packed only once."""
     t = dv.torch()
     if not isinstance(mat, t.Tensor) or not mat.is_cuda:
         mat = dv.as_tensor(mat, t.int8)
@@ -94,12 +97,19 @@ def _panels_for(mat, pop_indices, mode):
         mat = mat.to(t.int8)
     rows = list(range(mat.shape[0])) if pop_indices is None else [int(x) for x in
                                                                    dv.to_numpy(pop_indices)]
-    key = (mode, mat._version, tuple(rows))
+    key = (mode, tuple(rows))
     per = _PANEL_CACHE.get(mat)
-    if per is not None and key in per:
-        return per[key]
+    if per is None or per[0] != mat._version:   # written in place: repack everything
+        per = (mat._version, {})
+        _PANEL_CACHE[mat] = per
+    cache = per[1]
+    if key in cache:
+        cache[key] = cache.pop(key)   # most recently used last
+        return cache[key]
     pan = LDPanels(mat, [rows], mode)
-    _PANEL_CACHE[mat] = {key: pan}
+    cache[key] = pan
+    while len(cache) > _PANELS_PER_MATRIX:
+        cache.pop(next(iter(cache)))
     return pan
 
 

@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 
 from . import _device as dv
-from . import _lib
+from . import _lib, filters
 from .engine import LDPanels, decode_limbs
 from .genotype_matrix import GenotypeMatrix
 from .haplotype_matrix import HaplotypeMatrix
@@ -89,19 +89,25 @@ def compute_ld_statistics(vcf_file=None, rec_map_file=None, pop_file=None, pops=
         print(f"  Computing LD ({n_bins} bins, {num_pops} pops) ...")
 
     data = mat.genotypes if use_genotypes else mat.haplotypes
-    pop_rows = [list(mat.sample_sets[p]) for p in pops]
-    keep = None
+    pop_rows = [[int(i) for i in mat.sample_sets[p]] for p in pops]
+    mode = "geno" if use_genotypes else "hap"
     if use_genotypes:
-        keep = _union_biallelic(dv.to_numpy(data), pop_rows)
-        t = dv.torch()
-        data = data[:, dv.as_tensor(keep, t.int64, data.device)].contiguous()
-    pos = pos_all if keep is None else pos_all[keep]
-    coords = None if gen_all is None else (gen_all if keep is None else gen_all[keep])
-
-    pan = LDPanels(data, pop_rows, "geno" if use_genotypes else "hap")
+        # heterozygosity over every site of the matrix (moments_ld.py:346-388
+        # is called on the unfiltered genotypes): tallies only, no panels
+        het = heterozygosity(LDPanels(data, pop_rows, mode, pack=False))
+        # sites biallelic across the union of the populations' individuals
+        # (moments_ld.py:256-276), tested and compacted on the device
+        idx = _union_biallelic(data, pop_rows)
+        data = filters.take_columns(data, idx)
+        idx_np = idx.cpu().numpy()
+        pos = pos_all[idx_np]
+        coords = None if gen_all is None else gen_all[idx_np]
+        pan = LDPanels(data, pop_rows, mode)
+    else:
+        pos, coords = pos_all, gen_all
+        pan = LDPanels(data, pop_rows, mode)
+        het = heterozygosity(pan)   # the same site tallies
     res = pan.binned(pos, bins, max_bp, coords=coords)
-    het = heterozygosity(LDPanels(mat.genotypes if use_genotypes else mat.haplotypes, pop_rows,
-                                  "geno" if use_genotypes else "hap"))
     if report:
         print("  Done.")
     hn = het_names(num_pops)
@@ -122,7 +128,7 @@ def heterozygosity(pan):
     bad = t.zeros(n_het, dtype=t.int32, device=pan.device)
     _lib.check(_lib.lib().pgld_heterozygosity(ctypes.byref(pan.desc), _lib.ptr(pan.site_stats),
                                               _lib.ptr(out), _lib.ptr(bad),
-                                              _lib.stream_handle(pan.stream)))
+                                              _lib.stream_handle()))
     raw = out.cpu().numpy().reshape(n_het, 1, 3)
     vals = np.array([decode_limbs(np.concatenate([raw[e:e + 1], np.zeros((1, 1, 3), np.int64)],
                                                  axis=1), 1)[0][0, 0] for e in range(n_het)])
@@ -132,14 +138,11 @@ def heterozygosity(pan):
 
 
 def _union_biallelic(g, pop_rows):
-    """Sites biallelic across the union of the selected populations
+    """Indices (CUDA int64) of the sites biallelic across the union of the
+    selected populations' individuals, each individual counted once
     (moments_ld.py:256-279)."""
     rows = sorted(set(r for pr in pop_rows for r in pr))
-    sub = g[rows]
-    v = sub >= 0
-    alt = np.where(v, sub, 0).astype(np.int64).sum(axis=0)
-    nv = v.sum(axis=0)
-    return np.nonzero((alt > 0) & (alt < 2 * nv) & (nv >= 2))[0]
+    return filters.keep_index(filters.biallelic_keep(g, _lib.FILTER_GENO_BIALLELIC, rows))
 
 
 def _interpolate_genetic_distances(positions, rec_map_file):

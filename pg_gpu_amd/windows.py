@@ -116,7 +116,7 @@ class WindowPlan:
         # under which the items of one window already run side by side.
         sched = None
         fn = _lib.lib().pgld_windows_fused if kernel == "fused" else _lib.lib().pgld_windows_plain
-        _lib.check(fn(ctypes.byref(pan.desc), ctypes.byref(a), _lib.stream_handle(pan.stream)))
+        _lib.check(fn(ctypes.byref(pan.desc), ctypes.byref(a), _lib.stream_handle()))
         res = _PendingWindows(out, self.ns, nw)
         res._sched = sched
         return res

@@ -155,7 +155,7 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     from pg_gpu_amd import _lib
-    assert set(_lib.exported_symbols()) >= set(names) - {"pgld_last_error"} or True
+    assert set(_lib.exported_symbols()) == set(names)
 
 
 @pytest.mark.parametrize("square", [1, 3, 16, 64])
@@ -271,3 +271,70 @@ def test_generated_accumulation_reproduces_reference_stats(P):
     floor = 1e-4 * np.max(np.abs(ref), axis=0)
     scale = np.maximum(np.maximum(np.abs(ref), mag), np.maximum(floor, 1e-300))
     assert np.max(np.abs(got - ref) / scale) < 1e-12
+
+
+def header_status_codes():
+    text = (ROOT / "include/pgld.h").read_text()
+    return {k: int(v) for k, v in re.findall(r"\b(PGLD_E[A-Z]+|PGLD_OK)\s*=\s*(\d+)", text)}
+
+
+def test_status_codes_agree_across_header_binding_and_docs():
+    """include/pgld.h's status enum, the ctypes binding's constants and the
+    binding documented for pg_gpu maintainers name the same codes, and the
+    documented _check stub raises the exception the reference raises."""
+    from pg_gpu_amd import _lib
+
+    codes = header_status_codes()
+    assert codes == {"PGLD_OK": 0, "PGLD_EINVAL": 1, "PGLD_EDEVICE": 2, "PGLD_EUNSUPPORTED": 3}
+    for name, v in codes.items():
+        assert getattr(_lib, name) == v
+    doc = (ROOT / "INTEGRATION.md").read_text()
+    for name, v in codes.items():
+        assert f"{v} `{name}`" in doc
+    # the documented stub: run its _check against a stand-in library
+    block = re.search(r"```python\n(.*?)```", doc, flags=re.S).group(1)
+    src = block[block.index("PGLD_OK, PGLD_EINVAL"):]
+
+    class FakeLib:
+        @staticmethod
+        def pgld_last_error():
+            return b"msg"
+
+    ns = {"_lib": FakeLib()}
+    exec(src, ns)
+    ns["_check"](0)
+    for rc, exc in ((1, ValueError), (2, RuntimeError), (3, NotImplementedError)):
+        with pytest.raises(exc):
+            ns["_check"](rc)
+        with pytest.raises(exc):
+            _check_via_lib(rc)
+
+
+def _check_via_lib(rc):
+    from pg_gpu_amd import _lib
+
+    class L:
+        @staticmethod
+        def pgld_last_error():
+            return b"msg"
+
+    saved = _lib._lib
+    _lib._lib = L()
+    try:
+        _lib.check(rc)
+    finally:
+        _lib._lib = saved
+
+
+def test_estimate_ld_chunk_size_hand_values():
+    """Pairs per chunk = clamp(mem // ((4 H P + 150 P) * 3), 1e5, 1e7)
+    (ld_pipeline.py:16-49), worked by hand."""
+    from pg_gpu_amd.ld_pipeline import estimate_ld_chunk_size
+
+    # H = 500, P = 2: (4000 + 300) * 3 = 12,900 bytes per pair
+    assert estimate_ld_chunk_size(500, 12_900 * 2_000_000, 2) == 2_000_000
+    assert estimate_ld_chunk_size(500, 12_900 * 2_000_000 + 12_899, 2) == 2_000_000
+    assert estimate_ld_chunk_size(500, 10**9, 2) == 100_000          # 77,519 -> floor
+    assert estimate_ld_chunk_size(10, 10**12, 1) == 10_000_000       # ceiling
+    # H = 1000, P = 4: (16,000 + 600) * 3 = 49,800
+    assert estimate_ld_chunk_size(1000, 49_800 * 123_456, 4) == 123_456

@@ -155,7 +155,7 @@ def test_haplotype_r2_matches_brute_force_pearson():
 def test_centered_moment_forms_equal_reference_polynomials():
     """The device code evaluates the reference's three single-population
     sub-polynomials (genotype_kernels.py:27-71, 235-272, 364-405; the
-    coefficient tables in pg_gpu_amd/csrc/geno_kernel_polys.json) in centered
+    coefficient tables in oracle/geno_kernel_polys.json) in centered
     dosage moments.  Substituting the moments' definitions as linear forms of
     the nine class counts, each form expands to the reference polynomial
     exactly; so do the features D, pA, qA, pB, qB, A, B the other estimators
@@ -163,7 +163,7 @@ def test_centered_moment_forms_equal_reference_polynomials():
     import json
     from pathlib import Path
 
-    path = Path(__file__).resolve().parents[1] / "pg_gpu_amd" / "csrc" / "geno_kernel_polys.json"
+    path = Path(__file__).resolve().parents[1] / "oracle" / "geno_kernel_polys.json"
     data = json.loads(path.read_text())
     g = sp.symbols("g0:9")
     X = [2, 2, 2, 1, 1, 1, 0, 0, 0]   # left dosage of class m (moments order)
