@@ -138,7 +138,7 @@ def test_single_pop_values(cuda):
     hap, pos = founder_haps(11, 60, 400, 200_000, missing=0.02)
     hap[:, 5] = 1                        # removed by the biallelic filter
     hm = HaplotypeMatrix(hap, pos)
-    bins = [1, 10, 100, 1_000, 10_000, 60_000]
+    bins = [0.5, 1, 100, 1_000, 10_000, 60_000]   # distances are >= 1: the first bin is empty
     keep = o.biallelic_filter_hap(hap)
     ref, cnt, _, absr = o.binned_sums_hap(hap[:, keep], pos[keep], [list(range(60))], bins,
                                           with_abs=True)
@@ -214,7 +214,8 @@ def test_genotype_from_vcf_and_haplotype_subsets(cuda):
     path = GOLD / "im-parsing-example.vcf.gz"
     d = read_vcf_gt(path)
     gt = d["gt"]
-    keep = (np.sum(gt == 0, axis=(1, 2)) > 0) & (np.sum(gt == 1, axis=(1, 2)) > 0)
+    keep = (np.sum(gt == 0, axis=(1, 2)) > 0) & (np.sum(gt == 1, axis=(1, 2)) > 0) & \
+        (np.sum(gt >= 2, axis=(1, 2)) == 0)     # is_biallelic_01
     g = np.where((gt[keep] >= 0).all(axis=2), gt[keep].sum(axis=2), -1).astype(np.int8).T
     gm = GenotypeMatrix.from_vcf(str(path))
     np.testing.assert_array_equal(gm.genotypes, g)
