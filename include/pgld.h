@@ -10,7 +10,8 @@
  *
  * Conventions
  *   - Every pointer argument named d_* is device memory owned by the caller.
- *     The library allocates no device memory of its own.
+ *     The library allocates no device memory; its only device memory is the
+ *     static 1 KiB scheduler array of pgld_binned_fused_noalloc.
  *   - `stream` is a hipStream_t passed as void*; NULL means the null stream.
  *     Calls are asynchronous unless stated otherwise.
  *   - Return value: PGLD_OK or an error code; pgld_last_error() returns a
@@ -143,6 +144,19 @@ int pgld_n_stats(int32_t mode, int32_t n_pops);
  * more than ~50 bins) the bins are covered by consecutive launches over bin
  * ranges, with the static tile schedule. */
 int pgld_binned_fused(const pgld_panel_desc* desc, const pgld_binned_args* args, void* stream);
+
+/* pgld_binned_fused for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments and output as
+ * pgld_binned_fused, except: d_sched must be NULL -- the PGLD_SCHED_WORDS
+ * tile-scheduler words (1 KiB, fixed whatever the job size) live in a
+ * file-scope device array of the library, one copy per device held until
+ * the process ends -- and d_out need not be zeroed: this call clears it and
+ * the scheduler words with a fill kernel on `stream` (no memset node, no
+ * allocation, no host synchronisation).  Calls that overlap in time on one
+ * device share the scheduler words and race; serialise them (one stream). */
+int pgld_binned_fused_noalloc(const pgld_panel_desc* desc, const pgld_binned_args* args,
+                              void* stream);
 
 /* Arguments of the ragged-window kernels (one population). */
 typedef struct {

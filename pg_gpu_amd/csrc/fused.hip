@@ -25,6 +25,7 @@
 // wave's MFMA/loads overlap the other's FP64 epilogue; two-population
 // genotype sets need the full register file of a one-wave-per-SIMD launch.  No pair list and no per-pair statistic ever reaches HBM.
 #include <algorithm>
+#include <mutex>
 #include "pgld_internal.h"
 #include "stats_hap.inc"
 #include "stats_geno.inc"
@@ -1276,15 +1277,8 @@ extern "C" int pgld_n_stats(int32_t mode, int32_t n_pops) {
   return -1;
 }
 
-extern "C" int pgld_binned_fused(const pgld_panel_desc* desc, const pgld_binned_args* args,
-                                 void* stream) {
-  PanelLayout L;
-  int rc = validate_binned(desc, args, &L);
-  if (rc) return rc;
-  if (!args->d_panels || !args->d_site_stats || (!args->d_tiles && args->n_tiles))
-    return fail(PGLD_EINVAL, "fused kernel needs panels, site stats and a tile list");
-  if (args->n_tiles == 0) return PGLD_OK;
-  hipStream_t s = as_stream(stream);
+static int binned_fused_dispatch(const pgld_panel_desc* desc, const pgld_binned_args* args,
+                                 const PanelLayout& L, hipStream_t s) {
   if (args->with_r2) {  // one population (checked above)
     if (desc->mode == PGLD_MODE_HAP) return launch_fused<HapTraits<1, true>>(L, args, s);
     return L.pop_size[0] <= 1023 ? launch_fused<GenoTraits<1, 3, true>>(L, args, s)
@@ -1318,6 +1312,88 @@ extern "C" int pgld_binned_fused(const pgld_panel_desc* desc, const pgld_binned_
     }
   }
   return fail(PGLD_EUNSUPPORTED, "fused kernel: this (mode, n_pops) is not built yet");
+}
+
+extern "C" int pgld_binned_fused(const pgld_panel_desc* desc, const pgld_binned_args* args,
+                                 void* stream) {
+  PanelLayout L;
+  int rc = validate_binned(desc, args, &L);
+  if (rc) return rc;
+  if (!args->d_panels || !args->d_site_stats || (!args->d_tiles && args->n_tiles))
+    return fail(PGLD_EINVAL, "fused kernel needs panels, site stats and a tile list");
+  if (args->n_tiles == 0) return PGLD_OK;
+  return binned_fused_dispatch(desc, args, L, as_stream(stream));
+}
+
+// ---------------------------------------------------------------------------
+// pgld_binned_fused_noalloc: the same launch with the tile-scheduler words in
+// a file-scope device array instead of a caller buffer, and the output and
+// scheduler cleared by a fill kernel, so a caller that cannot allocate (code
+// under graph capture, serving loops) needs no device memory of its own
+// beyond inputs and outputs.  Each device holds its own copy of `scratch`
+// for the life of the process; its address is looked up once per device.
+// Calls that overlap in time on one device share it and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) uint32_t scratch[PGLD_SCHED_WORDS];
+constexpr int kScratchMaxDevices = 64;
+static uint32_t* g_scratch_ptr[kScratchMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_words(uint32_t** out) {
+  int dev = 0;
+  PGLD_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchMaxDevices) return fail(PGLD_EDEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    PGLD_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PGLD_OK;
+}
+
+__global__ void __launch_bounds__(256) clear_words_kernel(uint32_t* __restrict__ a, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    a[e] = 0u;
+}
+
+static int clear_words(void* p, int64_t n_words, hipStream_t s) {
+  if (n_words <= 0) return PGLD_OK;
+  const int64_t blocks = std::min<int64_t>((n_words + 255) / 256, 1024);
+  hipLaunchKernelGGL(clear_words_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
+                     static_cast<uint32_t*>(p), n_words);
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}
+
+extern "C" int pgld_binned_fused_noalloc(const pgld_panel_desc* desc,
+                                         const pgld_binned_args* args, void* stream) {
+  static_assert(sizeof(scratch) == PGLD_SCHED_WORDS * sizeof(uint32_t), "scheduler words");
+  PanelLayout L;
+  int rc = validate_binned(desc, args, &L);
+  if (rc) return rc;
+  if (!args->d_panels || !args->d_site_stats || (!args->d_tiles && args->n_tiles))
+    return fail(PGLD_EINVAL, "fused kernel needs panels, site stats and a tile list");
+  if (args->d_sched)
+    return fail(PGLD_EINVAL, "pgld_binned_fused_noalloc uses its own scheduler words: "
+                             "d_sched must be NULL");
+  const int n_out = pgld_n_stats(desc->mode, desc->n_pops) + (args->with_r2 ? 2 : 0);
+  if (n_out < 0) return fail(PGLD_EUNSUPPORTED, "statistic set not available");
+  hipStream_t s = as_stream(stream);
+  // the output: n_bins x (n_out + 1) x 3 int64 limbs = 6 words each
+  rc = clear_words(args->d_out, (int64_t)args->n_bins * (n_out + 1) * 6, s);
+  if (rc) return rc;
+  if (args->n_tiles == 0) return PGLD_OK;
+  uint32_t* sched = nullptr;
+  rc = scratch_words(&sched);
+  if (rc) return rc;
+  rc = clear_words(sched, PGLD_SCHED_WORDS, s);
+  if (rc) return rc;
+  pgld_binned_args a = *args;
+  a.d_sched = sched;
+  return binned_fused_dispatch(desc, &a, L, s);
 }
 
 extern "C" int pgld_windows_fused(const pgld_panel_desc* desc, const pgld_window_args* args,
