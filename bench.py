@@ -8,9 +8,10 @@ every site pair within 200 kb, the 15 two-population DD/Dz/pi2 statistics
 summed into 20 log-spaced distance bins (100 bp .. 200 kb).
 
 A step = one pass of the hot path over the whole input already resident in
-HBM: packing the int8 matrix into MFMA panels + the fused count/estimator/bin
-kernel.  The tile list, site tallies and reciprocal tables are a per-dataset
-plan built once, outside the timed region.  Timing uses HIP events on the
+HBM: the per-site tallies, packing the int8 matrix into MFMA panels, and the
+fused count/estimator/bin kernel.  The band's tile list and the reciprocal
+tables are a per-dataset plan built once, outside the timed region; the
+"end_to_end" object times the whole call including them.  Timing uses HIP events on the
 stream the kernels run on; the same run also times the plain per-pair FP64
 kernel, the CPU oracle on a bounded anchor prefix (before the GPU is touched,
 one process per core), and compares the fused result with the oracle on that
@@ -106,14 +107,26 @@ _W = None
 
 
 def _cpu_worker(block):
+    """The reference's form: chunked counts, FP64 estimators, np.add.at."""
     from oracle import ld_oracle as o
 
     w = _W
     t0 = time.perf_counter()
+    sums, counts, n = o.binned_sums(w.mat, w.positions, w.pop_rows, w.bins, w.max_dist,
+                                    chunk=100_000, exact=False, anchors=block, mode=w.mode,
+                                    r2=w.name in R2_WORKLOADS)
+    return sums, counts, n, None, time.perf_counter() - t0
+
+
+def _cpu_worker_exact(block):
+    """Correctly rounded bin sums and |term| sums (precision reference)."""
+    from oracle import ld_oracle as o
+
+    w = _W
     sums, counts, n, absr = o.binned_sums(w.mat, w.positions, w.pop_rows, w.bins, w.max_dist,
                                           chunk=100_000, with_abs=True, anchors=block,
                                           mode=w.mode, r2=w.name in R2_WORKLOADS)
-    return sums, counts, n, absr, time.perf_counter() - t0
+    return sums, counts, n, absr, 0.0
 
 
 def _cpu_worker_windows(block):
@@ -125,10 +138,16 @@ def _cpu_worker_windows(block):
     return sums, counts, int(counts.sum()), None, time.perf_counter() - t0
 
 
-def cpu_baseline(w, target_s=15.0, max_workers=16):
-    """Oracle (NumPy restatement of the reference estimators) on a prefix of
-    the work (anchor sites, or windows), one process per core over contiguous
-    blocks."""
+# worker processes: every host core, up to the 16-core CPU share a GPU of an
+# MI355X node gets (os.cpu_count() on a shared box reports the whole node)
+CPU_SHARE = 16
+
+
+def cpu_baseline(w, target_s=15.0, max_workers=CPU_SHARE):
+    """Oracle (NumPy restatement of the reference estimators, np.add.at
+    binning) timed on a prefix of the work (anchor sites, or windows), one
+    process per core over contiguous blocks; then, untimed, correctly rounded
+    sums over the first quarter of that prefix for the precision check."""
     global _W
     _W = w
     cores = max(1, min(os.cpu_count() or 1, max_workers))
@@ -169,21 +188,109 @@ def cpu_baseline(w, target_s=15.0, max_workers=16):
         parts = pool.map(worker, blocks)
     wall = time.perf_counter() - t0
     n_pairs = sum(p[2] for p in parts)
+    res = dict(value=n_pairs / wall, cores=cores, cpu_count=os.cpu_count(), X=X,
+               n_pairs=n_pairs, wall=wall)
     if w.windows is not None:
-        sums = np.concatenate([p[0] for p in parts])
-        counts = np.concatenate([p[1] for p in parts])
-        absr = None
-    else:
-        nb, ns = w.bins.size - 1, parts[0][0].shape[1]
-        sums = np.array([[math.fsum(p[0][b, s] for p in parts) for s in range(ns)]
-                         for b in range(nb)])
-        counts = sum(p[1] for p in parts)
-        absr = sum(p[3] for p in parts)
-    return dict(value=n_pairs / wall, cores=cores, X=X, n_pairs=n_pairs, wall=wall, sums=sums,
-                counts=counts, abs=absr)
+        res.update(sums=np.concatenate([p[0] for p in parts]),
+                   counts=np.concatenate([p[1] for p in parts]), abs=None, X_exact=X,
+                   n_pairs_exact=n_pairs)
+        return res
+    # precision reference: exact sums over the first quarter of the prefix
+    Xe = max(64 * cores, (X // 4) // (64 * cores) * 64 * cores)
+    edges = np.linspace(0, Xe, cores + 1).astype(int)
+    with ctx.Pool(cores) as pool:
+        parts = pool.map(_cpu_worker_exact, [(int(edges[q]), int(edges[q + 1]))
+                                             for q in range(cores)])
+    nb, ns = w.bins.size - 1, parts[0][0].shape[1]
+    res.update(sums=np.array([[math.fsum(p[0][b, s] for p in parts) for s in range(ns)]
+                              for b in range(nb)]),
+               counts=sum(p[1] for p in parts), abs=sum(p[3] for p in parts), X_exact=Xe,
+               n_pairs_exact=sum(p[2] for p in parts))
+    return res
 
 
 # --------------------------------------------------------------------------- GPU
+
+
+def counting_stage(pan, plan, launch, steps, n_pairs, ops_per_pair, nprod, w=None):
+    """The counting stage alone (pgld_count_band: the fused kernel's traits,
+    tiles and schedule with a checksum for an epilogue), timed with HIP
+    events on the launch stream, its checksum compared with the oracle's
+    closed form when the CPU leg runs."""
+    import ctypes
+
+    import torch
+
+    from pg_gpu_amd import _lib
+
+    stream = torch.cuda.current_stream()
+    dummy = torch.zeros((1, 1, 3), dtype=torch.int64, device=pan.device)
+    a = plan.args(dummy, launch["grid"])
+    sched = torch.zeros(_lib.SCHED_WORDS, dtype=torch.int32, device=pan.device)
+    sums = torch.zeros((pan.P, 4), dtype=torch.int64, device=pan.device)
+    if launch.get("dynamic", True):
+        a.d_sched = _lib.ptr(sched).value
+
+    def once(ev=None):
+        sched.zero_()
+        sums.zero_()
+        if ev is not None:
+            ev[0].record(stream)
+        _lib.check(_lib.lib().pgld_count_band(ctypes.byref(pan.desc), ctypes.byref(a),
+                                              _lib.ptr(sums), _lib.stream_handle()))
+        if ev is not None:
+            ev[1].record(stream)
+
+    for _ in range(2):
+        once()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    for ev in evs:
+        once(ev)
+    torch.cuda.synchronize()
+    ms = float(np.mean([x.elapsed_time(y) for x, y in evs]))
+    got = sums.cpu().numpy()
+    eq = None
+    if w is not None:
+        from oracle import ld_oracle as o
+
+        eq = bool(np.array_equal(got, o.band_count_checksum(w.mat, w.pop_rows, plan.tiles,
+                                                            marginals=pan.P > 1)))
+    kpad = [64 * ((k + 63) // 64) for k in pan.pop_sizes]
+    executed = 2 * sum(q * k for q, k in zip(nprod, kpad)) * 1024 * len(plan.tiles)
+    ach = ops_per_pair * n_pairs / (ms * 1e-3) / 1e12
+    return {"kernel": f"count_band_kernel<HapTraits<{pan.P}>>", "kernel_ms": ms,
+            "achieved": ach, "unit": "TOP/s (in-band pairs x 2 g sum K)",
+            "frac_of_fp4_peak": ach / PEAK_FP4_TOPS, "frac_of_i8_peak": ach / PEAK_I8_TOPS,
+            "executed_ops": executed,
+            "executed_frac_of_fp4_peak": executed / (ms * 1e-3) / 1e12 / PEAK_FP4_TOPS,
+            "checksum": got.tolist(), "checksum_equal": eq}
+
+
+def end_to_end(mat, w, r2, launch, n_pairs, reps=3):
+    """Wall time of the whole call from the resident int8 matrix: site
+    tallies (with the host read of the missing-data flags), packing, the
+    host-side band tile list and its upload, the kernel, and the download
+    and decoding of the limbs -- the median of ``reps``."""
+    import torch
+
+    from pg_gpu_amd.engine import LDPanels, band_tiles
+
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pan = LDPanels(mat, w.pop_rows, w.mode)
+        tiles = band_tiles(w.positions, w.max_dist, launch.get("square"))
+        plan = pan.plan(w.positions, w.bins, w.max_dist, tiles=tiles)
+        plan.run("fused", launch.get("grid", 0), r2, launch.get("dynamic", True)).finish()
+        times.append(time.perf_counter() - t0)
+        del pan, plan
+    ms = float(np.median(times)) * 1e3
+    return {"ms": ms, "value": n_pairs / (ms * 1e-3), "unit": "site-pairs/s", "reps": reps,
+            "includes": "tallies + flag read, pack, band tile list (host) + upload, kernel, "
+                        "limb download and decode"}
+
 
 def main():
     ap = argparse.ArgumentParser()
@@ -195,6 +302,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-count-stage", action="store_true",
+                    help="skip the counting-stage-only kernel leg")
     ap.add_argument("--no-tune", action="store_true",
                     help="use the default launch shape instead of the autotuned one")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -264,7 +373,14 @@ def main():
                                               launch["dynamic"])
         n_bins = int(len(w.bins) - 1)
 
-    def step(kernel="fused", ev=None, pev=None):
+    def step(kernel="fused", ev=None, pev=None, sev=None):
+        # one pass of the hot path over the resident int8 matrix: site
+        # tallies, MFMA panels, the fused count / estimator / bin kernel
+        if sev is not None:
+            sev[0].record(stream)
+        pan.rescan()
+        if sev is not None:
+            sev[1].record(stream)
         if pev is not None:
             pev[0].record(stream)
         pan.repack()
@@ -285,17 +401,22 @@ def main():
            for _ in range(args.steps)]
     pevs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
             for _ in range(args.steps)]
+    sevs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            for _ in range(args.steps)]
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     start.record(stream)
-    results = [step(ev=evs[k], pev=pevs[k]) for k in range(args.steps)]
+    results = [step(ev=evs[k], pev=pevs[k], sev=sevs[k]) for k in range(args.steps)]
     stop.record(stream)
     torch.cuda.synchronize()
     total_ms = start.elapsed_time(stop)
     kern_ms = [a.elapsed_time(b) for a, b in evs]
     pack_ms = float(np.mean([a.elapsed_time(b) for a, b in pevs]))
+    scan_ms = float(np.mean([a.elapsed_time(b) for a, b in sevs]))
     # pack: every (row, site) of each population read once, the panels written once
     pack_bytes = int(sum(pan.pop_sizes)) * V + int(pan.panels.numel())
+    # tallies: every (row, site) of each population read once, 3 int32 per site written
+    scan_bytes = int(sum(pan.pop_sizes)) * V + 12 * V * pan.P
     res = results[-1].finish()
     n_pairs = int(res.n_pairs.sum()) if windows else res.n_pairs
     ms_step = total_ms / args.steps
@@ -325,7 +446,7 @@ def main():
 
     prec = None
     if cpu:
-        X = cpu["X"]
+        X = cpu["X_exact"]
         if windows:
             rg = WindowPlan(pan, w.windows[:X]).run().finish()
             got = np.stack([rg.dd, rg.dz, rg.pi2], axis=1)
@@ -342,7 +463,7 @@ def main():
                         counts_equal=bool(np.array_equal(rg.n_pairs, cpu["counts"]) and
                                           np.array_equal(rg.r2_count,
                                                          np.rint(cpu["sums"][:, 4]).astype(int))),
-                        sample_pairs=int(cpu["n_pairs"]))
+                        sample_pairs=int(cpu["n_pairs_exact"]))
         else:
             sub = tiles[tiles[:, 0] < X // TILE]
             rg = pan.binned(w.positions, w.bins, w.max_dist, kernel="fused", tiles=sub, r2=r2)
@@ -359,22 +480,37 @@ def main():
                         per_stat_max_rel_err={n: float(e) for n, e in zip(names, err.max(axis=0))}
                         if len(names) == ns else None,
                         counts_equal=bool(np.array_equal(rg.counts, cpu["counts"])),
-                        sample_pairs=int(cpu["n_pairs"]))
+                        sample_pairs=int(cpu["n_pairs_exact"]))
             if r2:
                 prec["r2_sum_max_rel_err"] = float(np.max(
                     np.abs(rg.r2_sum - cpu["sums"][:, ns]) /
                     np.maximum(np.abs(cpu["sums"][:, ns]), 1e-300)))
                 prec["counts_equal"] = prec["counts_equal"] and bool(np.array_equal(
                     rg.r2_count, np.rint(cpu["sums"][:, ns + 1]).astype(np.int64)))
-        log(f"precision vs oracle on {cpu['n_pairs']:,} pairs: {prec}")
+        log(f"precision vs oracle on {cpu['n_pairs_exact']:,} pairs: {prec}")
 
     k_ms = float(np.mean(kern_ms))
+    geno = w.mode == "geno"
     # indicator products per pair and population: haplotypes H.H (+ H.V, V.H,
     # V.V with missing calls); genotypes the 2x2 (or, with missing, 3x3)
     # feature products.  2 ops (multiply + add) per sample and product.
-    geno = w.mode == "geno"
     nprod = [((9 if m else 4) if geno else (4 if m else 1)) for m in pan.has_missing]
     ops_per_pair = 2 * sum(q * k for q, k in zip(nprod, pan.pop_sizes))
+
+    counting = None
+    if not windows and not geno and not args.no_count_stage:
+        counting = counting_stage(pan, plan, launch, args.steps, n_pairs, ops_per_pair,
+                                  nprod, w if cpu is not None else None)
+        log(f"counting stage: {counting['kernel_ms']:.3f} ms, "
+            f"{counting['frac_of_fp4_peak']:.3f} of the FP4 peak, "
+            f"{counting['frac_of_i8_peak']:.3f} of the int8 peak, "
+            f"checksum equal: {counting['checksum_equal']}")
+
+    e2e = None
+    if not windows:
+        e2e = end_to_end(mat, w, r2, launch, n_pairs)
+        log(f"end to end: {e2e['ms']:.2f} ms ({e2e['value']:.3e} pairs/s)")
+
     achieved = ops_per_pair * n_pairs / (k_ms * 1e-3) / 1e12
     kname = (f"window_kernel<{'Geno' if geno else 'Hap'}Traits<1, r2>>" if windows else
              f"fused_kernel<{'Geno' if geno else 'Hap'}Traits<{pan.P}{', r2' if r2 else ''}>>")
@@ -411,6 +547,9 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP4_TOPS,
                      "unit": "TOP/s (FP4 MFMA on 0/1 indicators)",
                      "frac": achieved / PEAK_FP4_TOPS, "frac_of_i8_peak": achieved / PEAK_I8_TOPS,
+                     # what limits the fused kernel: the vector-ALU issue of its FP64
+                     # estimator epilogue, not the matrix cores (see fp64_valu, counters)
+                     "limiter": "valu issue (fp64 estimator epilogue)",
                      "traffic": traffic,
                      "traffic_profile": f"profiles/{prof}_hbm_traffic.md" if traffic else None,
                      "kernel": kname, "ops_per_pair": ops_per_pair, "kernel_ms": k_ms,
@@ -421,13 +560,22 @@ def main():
         "pack": {"ms": pack_ms, "bytes": pack_bytes,
                  "GB/s": pack_bytes / (pack_ms * 1e-3) / 1e9,
                  "frac_of_hbm_peak": pack_bytes / (pack_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
+        "scan": {"ms": scan_ms, "bytes": scan_bytes,
+                 "GB/s": scan_bytes / (scan_ms * 1e-3) / 1e9,
+                 "frac_of_hbm_peak": scan_bytes / (scan_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
+        "counting_stage": counting,
+        "end_to_end": e2e,
         "plain_fp64": plain,
         "speedup_vs_plain": (plain["ms"] / k_ms) if plain else None,
         "precision": prec,
         "cpu_baseline": None if cpu is None else {
             "value": cpu["value"], "unit": "site-pairs/s", "cores": cpu["cores"], "kind": "port",
-            "sample": (f"oracle (NumPy restatement) on {'windows' if windows else 'anchors'} "
-                       f"[0, {cpu['X']}): {cpu['n_pairs']:,} pairs, {cpu['wall']:.1f}s wall")},
+            "cpu_count": cpu["cpu_count"],
+            "core_cap": f"{CPU_SHARE} worker processes at most: one GPU's CPU share of the node",
+            "sample": (f"oracle (NumPy restatement of the reference's chunked counts, FP64 "
+                       f"estimators and np.add.at binning) on "
+                       f"{'windows' if windows else 'anchors'} [0, {cpu['X']}): "
+                       f"{cpu['n_pairs']:,} pairs, {cpu['wall']:.1f}s wall")},
     }
     print(json.dumps(line))
 

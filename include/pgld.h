@@ -158,6 +158,19 @@ int pgld_binned_fused(const pgld_panel_desc* desc, const pgld_binned_args* args,
 int pgld_binned_fused_noalloc(const pgld_panel_desc* desc, const pgld_binned_args* args,
                               void* stream);
 
+/* The counting stage of pgld_binned_fused by itself (haplotypes): the same
+ * tiles (d_tiles, n_tiles), schedule (d_sched, grid) and MFMA counting, with
+ * the estimator epilogue replaced by a checksum.  d_sums (uint64 [n_pops][4],
+ * zeroed) receives per population the sums over every slot of every listed
+ * 32 x 32 tile of its four count fields: (n11, pA, pB, n) for two to four
+ * populations (pA, pB, n the valid-both marginals, or with no missing call
+ * the site tallies and the population size), (n11, n10, n01, n00) for one.
+ * Replaces, as a measurement of its arithmetic, the counting of
+ * compute_counts_for_pairs over the band (pg_gpu/ld_pipeline.py:130-175);
+ * the other fields of args are ignored. */
+int pgld_count_band(const pgld_panel_desc* desc, const pgld_binned_args* args,
+                    unsigned long long* d_sums, void* stream);
+
 /* Arguments of the ragged-window kernels (one population). */
 typedef struct {
   const int8_t* d_panels;      /* from pgld_pack (fused kernel) */

@@ -1026,3 +1026,55 @@ def tally_upper(hap, rows=None):
         h = h[np.asarray(rows, np.int64)]
     ii, jj = np.triu_indices(h.shape[1], 1)
     return hap_counts(h, ii, jj)
+
+
+# ---------------------------------------------------------------------------
+# Counting-stage checksum over whole 32 x 32 site tiles
+# ---------------------------------------------------------------------------
+
+
+def band_count_checksum(hap, pop_rows, tiles, marginals):
+    """Per population, the sums over every pair (i, j) of every listed tile
+    (rows 32 rb .. 32 rb + 31, columns 32 cb .. 32 cb + 31, sites past the end
+    contributing zero samples) of the four count fields the counting stage
+    produces: with ``marginals`` (n11, pA, pB, n), else (n11, n10, n01, n00);
+    counts over samples valid at both sites when the population has a missing
+    call, otherwise pA / pB are the sites' allele tallies and n the population
+    size for every slot.  Evaluated as dot products of per-block sample
+    tallies: sum_{i in R, j in C} sum_s a_s(i) b_s(j) = sum_s A_s(R) B_s(C)."""
+    h = np.asarray(hap)
+    V = h.shape[1]
+    nb = (V + 31) // 32
+    tiles = np.asarray(tiles, np.int64).reshape(-1, 2)
+    out = np.zeros((len(pop_rows), 4), np.int64)
+
+    def blocks(x):   # [K, V] 0/1 -> [nb, K] per-block column sums
+        xp = np.zeros((x.shape[0], nb * 32), np.int64)
+        xp[:, :V] = x
+        return xp.reshape(x.shape[0], nb, 32).sum(axis=2).T.copy()
+
+    def tile_dots(A, B):
+        tot = 0
+        for lo in range(0, len(tiles), 20_000):
+            t = tiles[lo:lo + 20_000]
+            tot += int(np.einsum("ts,ts->", A[t[:, 0]], B[t[:, 1]]))
+        return tot
+
+    for p, rows in enumerate(pop_rows):
+        x = h[np.asarray(rows, np.int64)]
+        K = x.shape[0]
+        H = blocks(x == 1)
+        hh = tile_dots(H, H)
+        if np.any(x < 0):
+            Vb = blocks(x >= 0)
+            hv, vh, vv = tile_dots(H, Vb), tile_dots(Vb, H), tile_dots(Vb, Vb)
+        else:
+            ks = H.sum(axis=1)                 # per block: sum of the sites' tallies
+            hv = 32 * int(ks[tiles[:, 0]].sum())
+            vh = 32 * int(ks[tiles[:, 1]].sum())
+            vv = 1024 * K * len(tiles)
+        if marginals:
+            out[p] = (hh, hv, vh, vv)
+        else:
+            out[p] = (hh, hv - hh, vh - hh, vv - hv - vh + hh)
+    return out

@@ -91,6 +91,7 @@ _SIGS = {
     "pgld_binned_fused": (c_int32, [POINTER(PanelDesc), POINTER(BinnedArgs), c_void_p]),
     "pgld_binned_plain": (c_int32, [POINTER(PanelDesc), POINTER(BinnedArgs), c_void_p]),
     "pgld_binned_fused_noalloc": (c_int32, [POINTER(PanelDesc), POINTER(BinnedArgs), c_void_p]),
+    "pgld_count_band": (c_int32, [POINTER(PanelDesc), POINTER(BinnedArgs), c_void_p, c_void_p]),
     "pgld_windows_fused": (c_int32, [POINTER(PanelDesc), POINTER(WindowArgs), c_void_p]),
     "pgld_windows_plain": (c_int32, [POINTER(PanelDesc), POINTER(WindowArgs), c_void_p]),
     "pgld_count_block": (c_int32, [POINTER(PanelDesc), c_void_p, c_void_p, c_int32, c_int64,

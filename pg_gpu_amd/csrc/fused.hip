@@ -1098,6 +1098,113 @@ static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStre
 }
 
 // ---------------------------------------------------------------------------
+// The counting stage by itself: the same traits, tiles and schedule as
+// fused_kernel, with the epilogue replaced by a checksum of the tile's count
+// words -- per population the sums over all band tiles and slots of the two
+// 16-bit halves of both words (haplotype traits: n11, pA, pB, n with
+// marginal slot words; n11, n10, n01, n00 with class counts).  It times the
+// MFMA counting stage alone and pins its counts at full size (each sum is a
+// sum of dot products of per-block sample tallies).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256, T::MINB) count_band_kernel(FusedParams a,
+                                                                  unsigned long long* sums) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  v4i* ring = nullptr;
+  if (T::RING > 0 && a.use_ring) ring = reinterpret_cast<v4i*>(smem) + wave * (T::RING_BYTES / 16);
+  constexpr int P = T::NP;
+  unsigned long long acc[P][4];
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[p][k] = 0ull;
+  const int nx = 8;
+  const int xcd = blockIdx.x % nx;
+  const int waves_per_xcd = (gridDim.x / nx) * 4;
+  const int64_t seg_lo = a.n_tiles * xcd / nx, seg_hi = a.n_tiles * (xcd + 1) / nx;
+  uint32_t* ctr = a.sched ? a.sched + xcd * 32 : nullptr;
+  int64_t local = (blockIdx.x / nx) * 4 + wave;
+  int32_t nrb = 0, ncb = 0;
+  if (seg_lo + local < seg_hi) {
+    nrb = a.tiles[2 * (seg_lo + local)];
+    ncb = a.tiles[2 * (seg_lo + local) + 1];
+  }
+  while (seg_lo + local < seg_hi) {
+    const int64_t rb = __builtin_amdgcn_readfirstlane(nrb), cb = __builtin_amdgcn_readfirstlane(ncb);
+    uint32_t claim = 0;
+    if (ctr && lane == 0) claim = atomicAdd(ctr, 1u);
+    typename T::Counts cts;
+    T::counts(a, rb, cb, lane, cts, ring);
+    const int64_t next = ctr ? waves_per_xcd + (int64_t)__builtin_amdgcn_readfirstlane(__shfl(claim, 0, 64))
+                             : local + waves_per_xcd;
+    if (seg_lo + next < seg_hi) {
+      nrb = a.tiles[2 * (seg_lo + next)];
+      ncb = a.tiles[2 * (seg_lo + next) + 1];
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      if (a.K[p] < 4096) {   // 16 slots of 12-bit fields: the packed sums cannot carry
+        uint32_t s0 = 0, s1 = 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) { s0 += cts.w0[p][r]; s1 += cts.w1[p][r]; }
+        acc[p][0] += s0 & 0xffffu; acc[p][1] += s0 >> 16;
+        acc[p][2] += s1 & 0xffffu; acc[p][3] += s1 >> 16;
+      } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          acc[p][0] += cts.w0[p][r] & 0xffffu; acc[p][1] += cts.w0[p][r] >> 16;
+          acc[p][2] += cts.w1[p][r] & 0xffffu; acc[p][3] += cts.w1[p][r] >> 16;
+        }
+      }
+    }
+    local = next;
+  }
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      unsigned long long v = acc[p][k];
+      for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0 && v) atomicAdd(sums + 4 * p + k, v);
+    }
+}
+
+template <class T>
+static int launch_count_band(const PanelLayout& L, const pgld_binned_args* g,
+                             unsigned long long* sums, hipStream_t s) {
+  FusedParams a = FusedParams{};
+  a.panels = g->d_panels;
+  a.site_stats = g->d_site_stats;
+  a.tiles = g->d_tiles;
+  a.n_tiles = g->n_tiles;
+  a.sched = g->d_sched;
+  a.V = L.n_sites;
+  a.block_bytes = L.block_bytes;
+  for (int p = 0; p < PGLD_MAX_POPS; ++p) {
+    a.pop_off[p] = L.pop_off[p];
+    a.ksteps[p] = L.ksteps[p];
+    a.missing[p] = L.missing[p];
+    a.K[p] = L.pop_size[p];
+  }
+  const size_t lds = T::RING > 0 ? (size_t)4 * T::RING_BYTES : 0;
+  a.use_ring = T::RING > 0;
+  if (lds) {
+    PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(count_band_kernel<T>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
+  int grid = g->grid > 0 ? g->grid : T::MINB * num_cus();
+  grid = ((grid + 7) / 8) * 8;
+  const int64_t need = ((g->n_tiles + 31) / 32) * 8;
+  if (grid > need) grid = (int)need;
+  if (grid < 8) grid = 8;
+  hipLaunchKernelGGL(count_band_kernel<T>, dim3(grid), dim3(256), lds, s, a, sums);
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Ragged windows: every i < j pair inside each window [lo_w, hi_w) of sites,
 // summed per window.  The work list holds (window, row block, column block)
 // triples covering each window's upper triangle; a tile that straddles a
@@ -1394,6 +1501,29 @@ extern "C" int pgld_binned_fused_noalloc(const pgld_panel_desc* desc,
   pgld_binned_args a = *args;
   a.d_sched = sched;
   return binned_fused_dispatch(desc, &a, L, s);
+}
+
+extern "C" int pgld_count_band(const pgld_panel_desc* desc, const pgld_binned_args* args,
+                               unsigned long long* d_sums, void* stream) {
+  PanelLayout L;
+  int rc = make_layout(desc, &L);
+  if (rc) return rc;
+  if (!args) return fail(PGLD_EINVAL, "null binned args");
+  if (desc->mode != PGLD_MODE_HAP)
+    return fail(PGLD_EUNSUPPORTED, "the counting-stage kernel is built for haplotypes");
+  if (!d_sums || !args->d_panels || !args->d_site_stats || (!args->d_tiles && args->n_tiles))
+    return fail(PGLD_EINVAL, "count band needs panels, site stats, a tile list and d_sums");
+  if (args->n_tiles < 0) return fail(PGLD_EINVAL, "negative n_tiles");
+  if (args->n_tiles == 0) return PGLD_OK;
+  hipStream_t s = as_stream(stream);
+  switch (desc->n_pops) {
+    case 1: return launch_count_band<HapTraits<1>>(L, args, d_sums, s);
+    case 2: return launch_count_band<HapTraits<2>>(L, args, d_sums, s);
+    case 3: return launch_count_band<HapTraits<3>>(L, args, d_sums, s);
+    case 4: return launch_count_band<HapTraits<4>>(L, args, d_sums, s);
+    default: break;
+  }
+  return fail(PGLD_EINVAL, "1-4 populations supported");
 }
 
 extern "C" int pgld_windows_fused(const pgld_panel_desc* desc, const pgld_window_args* args,

@@ -55,31 +55,6 @@ int make_layout(const pgld_panel_desc* d, PanelLayout* L) {
 }
 
 // ---------------------------------------------------------------------------
-// Site tallies: one thread per (site, pop); rows of the pop streamed.
-// Consecutive threads read consecutive sites of the same row (coalesced).
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) scan_sites_kernel(
-    const int8_t* __restrict__ mat, int64_t n_sites, int64_t ld,
-    const int32_t* __restrict__ rows, int32_t row_begin, int32_t row_end, int32_t mode,
-    int32_t* __restrict__ stats /* [3][n_sites] for this pop */, int32_t* __restrict__ flags) {
-  const int64_t site = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (site >= n_sites) return;
-  int c1 = 0, c2 = 0, cm = 0, bad = 0;
-  const int maxv = mode == PGLD_MODE_HAP ? 1 : 2;
-  for (int r = row_begin; r < row_end; ++r) {
-    const int v = mat[(int64_t)rows[r] * ld + site];
-    c1 += (v == 1);
-    c2 += (v == 2);
-    cm += (v == -1);
-    bad |= (v < -1) | (v > maxv);
-  }
-  stats[site] = c1;
-  stats[n_sites + site] = c2;
-  stats[2 * n_sites + site] = cm;
-  if (bad) atomicOr(flags, 1);
-}
-
-// ---------------------------------------------------------------------------
 // Pack: one workgroup per (PACK_SB site blocks, pop, 64-sample k-step).
 // Phase 1: thread (row quad j, 16-site group g) reads 16 bytes of each of the
 // four sample rows 4j..4j+3 (rows not aligned to 16 bytes: two aligned loads
@@ -140,6 +115,106 @@ __device__ __forceinline__ uint4 pack_load16(const int8_t* mat, const int8_t* ma
     }
   }
   return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Site tallies.  A workgroup covers 512 consecutive sites (32 groups of 16)
+// of one population and splits the population's rows into 8 interleaved
+// slices: thread (slice, group) reads 16 bytes of each of its rows (one
+// coalesced 512-byte row segment per wave and load) and counts the bytes
+// equal to 1, 2 and -1 with bytewise (SWAR) tests into per-byte counters,
+// flushed to integers every 255 rows.  The slices meet in LDS; the tallies are
+// written once, coalesced.  A byte of a real site outside the mode's alphabet
+// sets flags[0].
+// ---------------------------------------------------------------------------
+constexpr int SCAN_SLICES = 8;
+constexpr int SCAN_GROUPS = 32;
+constexpr int SCAN_SITES = SCAN_GROUPS * 16;
+
+struct ScanParams {
+  const int8_t* mat;
+  const int8_t* mat_end;
+  int64_t n_sites, ld;
+  const int32_t* rows;
+  int32_t offs[PGLD_MAX_POPS + 1];
+  int32_t mode;
+  int32_t* stats;   // [P][3][n_sites]
+  int32_t* flags;
+};
+
+__global__ void __launch_bounds__(256) scan_sites_kernel(ScanParams a) {
+  __shared__ int part[3 * SCAN_SITES];
+  const int tid = threadIdx.x;
+  const int p = blockIdx.y;
+  for (int e = tid; e < 3 * SCAN_SITES; e += 256) part[e] = 0;
+  __syncthreads();
+  const int g = tid % SCAN_GROUPS, slice = tid / SCAN_GROUPS;
+  const int64_t base = (int64_t)blockIdx.x * SCAN_SITES;
+  const int64_t site0 = base + 16 * g;
+  const bool geno = a.mode == PGLD_MODE_GENO;
+  if (site0 < a.n_sites) {
+    // bit 8b of vm[e] set where site site0 + 4e + b is a real site
+    uint32_t vm[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      uint32_t m = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) m |= (site0 + 4 * e + b < a.n_sites ? 1u : 0u) << (8 * b);
+      vm[e] = m;
+    }
+    int c[16][3];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) c[k][0] = c[k][1] = c[k][2] = 0;
+    uint32_t a1[4] = {0, 0, 0, 0}, a2[4] = {0, 0, 0, 0}, am[4] = {0, 0, 0, 0};
+    uint32_t bad = 0;
+    int pending = 0;
+    const int r1 = a.offs[p + 1];
+    for (int r = a.offs[p] + slice; r < r1; r += SCAN_SLICES) {
+      const int8_t* src = a.mat + (int64_t)a.rows[r] * a.ld + site0;
+      const uint4 q = pack_load16(a.mat, a.mat_end, src, site0, a.n_sites);
+      const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const uint32_t e1 = bytes_eq(w[e], 0x01010101u), e2 = bytes_eq(w[e], 0x02020202u);
+        const uint32_t em = bytes_eq(w[e], 0xFFFFFFFFu), e0 = bytes_eq(w[e], 0u);
+        a1[e] += e1;
+        a2[e] += e2;
+        am[e] += em;
+        bad |= ~(e0 | e1 | em | (geno ? e2 : 0u)) & vm[e];
+      }
+      if (++pending == 255) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          const int e = k >> 2, sh = 8 * (k & 3);
+          c[k][0] += (a1[e] >> sh) & 0xFFu;
+          c[k][1] += (a2[e] >> sh) & 0xFFu;
+          c[k][2] += (am[e] >> sh) & 0xFFu;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) a1[e] = a2[e] = am[e] = 0u;
+        pending = 0;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int e = k >> 2, sh = 8 * (k & 3);
+      c[k][0] += (a1[e] >> sh) & 0xFFu;
+      c[k][1] += (a2[e] >> sh) & 0xFFu;
+      c[k][2] += (am[e] >> sh) & 0xFFu;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        if (c[k][q]) atomicAdd(&part[q * SCAN_SITES + 16 * g + k], c[k][q]);
+    if (bad) atomicOr(a.flags, 1);
+  }
+  __syncthreads();
+  int32_t* st = a.stats + (int64_t)p * 3 * a.n_sites;
+  for (int e = tid; e < 3 * SCAN_SITES; e += 256) {
+    const int q = e / SCAN_SITES, k = e - q * SCAN_SITES;
+    if (base + k < a.n_sites) st[(int64_t)q * a.n_sites + base + k] = part[e];
+  }
 }
 
 __global__ void __launch_bounds__(256) pack_kernel(
@@ -245,14 +320,20 @@ extern "C" int pgld_scan_sites(const int8_t* d_mat, int64_t n_rows, int64_t n_si
   if (n_sites != desc->n_sites || ld < n_sites || n_rows < 0)
     return fail(PGLD_EINVAL, "matrix shape disagrees with descriptor");
   if (n_sites == 0) return PGLD_OK;
-  hipStream_t s = as_stream(stream);
-  const unsigned grid = (unsigned)((n_sites + 255) / 256);
-  for (int p = 0; p < desc->n_pops; ++p) {
-    hipLaunchKernelGGL(scan_sites_kernel, dim3(grid), dim3(256), 0, s, d_mat, n_sites, ld, d_rows,
-                       pop_offsets[p], pop_offsets[p + 1], desc->mode,
-                       d_site_stats + (int64_t)p * 3 * n_sites, d_flags);
-    PGLD_LAUNCH_CHECK();
-  }
+  ScanParams a;
+  a.mat = d_mat;
+  a.mat_end = d_mat + (n_rows > 0 ? (n_rows - 1) * ld + n_sites : 0);
+  a.n_sites = n_sites;
+  a.ld = ld;
+  a.rows = d_rows;
+  for (int p = 0; p <= PGLD_MAX_POPS; ++p) a.offs[p] = p <= desc->n_pops ? pop_offsets[p] : 0;
+  a.mode = desc->mode;
+  a.stats = d_site_stats;
+  a.flags = d_flags;
+  const unsigned grid = (unsigned)((n_sites + SCAN_SITES - 1) / SCAN_SITES);
+  hipLaunchKernelGGL(scan_sites_kernel, dim3(grid, (unsigned)desc->n_pops), dim3(256), 0,
+                     as_stream(stream), a);
+  PGLD_LAUNCH_CHECK();
   return PGLD_OK;
 }
 

@@ -310,6 +310,7 @@ class LDPanels:
         offs = _lib.int32_array(self.row_offsets)
         self.site_stats = torch.zeros((self.P, 3, self.V), dtype=torch.int32, device=self.device)
         flags = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self._flags = flags
         h = _lib.stream_handle()
         _lib.check(L.pgld_scan_sites(_lib.ptr(self.mat), self.n_rows, self.V, self.ld,
                                      _lib.ptr(self.d_rows), offs, ctypes.byref(desc),
@@ -350,6 +351,19 @@ class LDPanels:
     @property
     def n_stats(self):
         return int(_lib.lib().pgld_n_stats(self.mode, self.P))
+
+    def rescan(self):
+        """Re-run the site-tally kernel into the existing tally buffer (no
+        allocation, no host synchronisation; the value check's flag word is
+        cleared by a fill kernel and not read back)."""
+        if self.V == 0:
+            return
+        self._flags.zero_()
+        offs = _lib.int32_array(self.row_offsets)
+        _lib.check(_lib.lib().pgld_scan_sites(_lib.ptr(self.mat), self.n_rows, self.V, self.ld,
+                                              _lib.ptr(self.d_rows), offs,
+                                              ctypes.byref(self.desc), _lib.ptr(self.site_stats),
+                                              _lib.ptr(self._flags), _lib.stream_handle()))
 
     def repack(self):
         """Re-run the pack kernel into the existing panel buffer (no allocation)."""

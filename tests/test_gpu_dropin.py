@@ -341,3 +341,32 @@ def test_launches_follow_the_current_stream(cuda):
         r1 = pan.binned(pos, bins)
     np.testing.assert_array_equal(r1.sums, r0.sums)
     np.testing.assert_array_equal(r1.counts, r0.counts)
+
+
+@pytest.mark.parametrize("mode,offset", [("hap", 0), ("hap", 3), ("geno", 0), ("geno", 7)])
+def test_scan_sites_tallies(cuda, mode, offset):
+    """pgld_scan_sites against NumPy: per population and site the counts of
+    1, 2 and -1, on a column view that starts off a 16-byte boundary, with
+    populations of more than 255 rows (the per-byte counters' flush) and a
+    site count that is not a multiple of 16."""
+    import torch
+
+    from pg_gpu_amd.engine import LDPanels
+
+    rng = np.random.default_rng(60 + offset)
+    hi = 2 if mode == "hap" else 3
+    full = rng.integers(-1, hi, (700, 1029 + offset)).astype(np.int8)
+    mat = full[:, offset:]
+    pops = [list(range(0, 600, 2)), list(range(1, 700, 2))[:290], [5, 699, 3]]
+    dev = torch.as_tensor(full, device=cuda)[:, offset:]
+    pan = LDPanels(dev, pops, mode, pack=False)
+    got = pan.site_stats.cpu().numpy()
+    for p, rows in enumerate(pops):
+        x = mat[rows]
+        np.testing.assert_array_equal(got[p, 0], (x == 1).sum(axis=0))
+        np.testing.assert_array_equal(got[p, 1], (x == 2).sum(axis=0))
+        np.testing.assert_array_equal(got[p, 2], (x == -1).sum(axis=0))
+    bad = full.copy()
+    bad[699, offset + 1028] = 2 if mode == "hap" else 3     # last site, last population
+    with pytest.raises(ValueError):
+        LDPanels(torch.as_tensor(bad, device=cuda)[:, offset:], pops, mode, pack=False)

@@ -147,3 +147,40 @@ def test_moments_driver_restatement_matches_golden_fixture():
     assert np.max(np.abs(got - ref) / np.abs(ref)) < 1e-12
     np.testing.assert_allclose(res["sums"][-1], fx["het"], rtol=1e-14)
     assert res["stats"] == (fx["ld_names"], fx["het_names"])
+
+
+@pytest.mark.parametrize("missing", [False, True])
+def test_band_count_checksum_brute_force(missing):
+    """The closed form of the counting-stage checksum against a pair-by-pair
+    loop over whole tiles (padded sites past the end included)."""
+    rng = np.random.default_rng(6 + missing)
+    K, V = 11, 70
+    h = rng.integers(0, 2, (K + 5, V)).astype(np.int8)
+    if missing:
+        h[rng.random(h.shape) < 0.2] = -1
+    rows = list(range(3, 3 + K))
+    tiles = np.array([[0, 0], [0, 1], [1, 2], [2, 2], [0, 2]])
+    x = h[rows]
+    miss = bool(np.any(x < 0))
+    tot = np.zeros((2, 4), np.int64)
+    for rb, cb in tiles:
+        for i in range(32 * rb, 32 * rb + 32):
+            for j in range(32 * cb, 32 * cb + 32):
+                a = x[:, i] if i < V else None
+                b = x[:, j] if j < V else None
+                if miss:
+                    if a is None or b is None:
+                        f = (0, 0, 0, 0)
+                    else:
+                        ok = (a >= 0) & (b >= 0)
+                        f = (np.sum(ok & (a == 1) & (b == 1)), np.sum(ok & (a == 1)),
+                             np.sum(ok & (b == 1)), np.sum(ok))
+                else:
+                    ki = int(np.sum(a == 1)) if a is not None else 0
+                    kj = int(np.sum(b == 1)) if b is not None else 0
+                    n11 = int(np.sum((a == 1) & (b == 1))) if a is not None and b is not None else 0
+                    f = (n11, ki, kj, K)
+                tot[0] += f
+                tot[1] += (f[0], f[1] - f[0], f[2] - f[0], f[3] - f[1] - f[2] + f[0])
+    np.testing.assert_array_equal(o.band_count_checksum(h, [rows], tiles, True)[0], tot[0])
+    np.testing.assert_array_equal(o.band_count_checksum(h, [rows], tiles, False)[0], tot[1])
