@@ -187,7 +187,8 @@ typedef struct {
   long long* d_out;            /* [n_windows][n_stats + 3][3] limbs, zeroed */
   int32_t grid;                /* workgroups (0 = auto) */
   int32_t n_lo[PGLD_MAX_POPS]; /* as in pgld_binned_args */
-  uint32_t* d_sched;           /* as in pgld_binned_args (fused kernel) */
+  uint32_t* d_sched;           /* not used: the window kernel splits the work list into
+                                  contiguous per-wave chunks */
 } pgld_window_args;
 
 /* Per-window sums over every i < j pair of sites inside each window (no

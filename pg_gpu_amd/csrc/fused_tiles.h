@@ -1,0 +1,955 @@
+// Tile machinery of the fused binned and window kernels (fused.hip):
+// kernel parameters, MFMA tile counting, the haplotype / genotype traits
+// (how a tile's counts are produced and how a pair's statistics follow), the
+// per-bin staging and reduction, and the per-tile statistic passes.
+#pragma once
+#include <algorithm>
+#include "pgld_internal.h"
+#include "stats_hap.inc"
+#include "stats_geno.inc"
+
+namespace pgld {
+
+struct FusedParams {
+  const int8_t* panels;
+  const int32_t* site_stats;  // [P][3][V]
+  const double* pos;
+  const double* coord;
+  int32_t coord_is_pos;
+  double max_dist;
+  const double* edges;
+  int32_t n_bins;
+  const double* rtab;          // 4 doubles per n in [0, max pop size]
+  const int32_t* tiles;        // (rb, cb) pairs of 32-site blocks
+  int64_t n_tiles;
+  unsigned long long* out;
+  int64_t V;
+  int64_t block_bytes;
+  int64_t pop_off[PGLD_MAX_POPS];
+  int32_t ksteps[PGLD_MAX_POPS];
+  int32_t missing[PGLD_MAX_POPS];
+  int32_t K[PGLD_MAX_POPS];
+  int32_t rt_lo[PGLD_MAX_POPS];   // smallest n kept in the LDS table of pop p
+  int32_t rt_off[PGLD_MAX_POPS];  // first entry of pop p in the LDS table
+  int32_t rt_len;                 // total LDS table entries
+  uint32_t* sched;                // per-XCD tile counters (null: static schedule)
+  int32_t use_ring;               // stream operands through the LDS ring (traits with RING)
+};
+
+// 16 packed count words per lane (one per MFMA output slot).  Kept as a
+// vector value so that the epilogue's wave-uniform slot index r selects a
+// register (index mode) instead of spilling the tile's counts to scratch.
+typedef uint32_t v16u __attribute__((ext_vector_type(16)));
+
+// Row (within a 32x32 MFMA output block) of accumulator register r of `lane`;
+// the column is lane & 31.
+__device__ __forceinline__ int mfma_row(int lane, int r) {
+  return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+}
+
+// acc1 += R.C1 and acc2 += R.C2 over the ks k-steps of one row operand R and
+// two column operands; operand loads run PF steps ahead in a register ring,
+// so the L2 latency of one step overlaps the multiplies of the previous ones.
+__device__ __forceinline__ void mfma_row_pass(const v4i* R, const v4i* C1, const v4i* C2, int ks,
+                                              v16f& acc1, v16f& acc2) {
+  constexpr int PF = 2;
+  v4i r[PF], c1[PF], c2[PF];
+#pragma unroll
+  for (int d = 0; d < PF; ++d)
+    if (d < ks) { r[d] = R[d * 64]; c1[d] = C1[d * 64]; c2[d] = C2[d * 64]; }
+  for (int s0 = 0; s0 < ks; s0 += PF) {
+#pragma unroll
+    for (int d = 0; d < PF; ++d) {
+      const int s = s0 + d;
+      if (s < ks) {
+        acc1 = pgld_mfma_fp4(r[d], c1[d], acc1);
+        acc2 = pgld_mfma_fp4(r[d], c2[d], acc2);
+        if (s + PF < ks) {
+          const int n = (s + PF) * 64;
+          r[d] = R[n]; c1[d] = C1[n]; c2[d] = C2[n];
+        }
+      }
+    }
+  }
+}
+
+// k-steps of operand loads in flight in the missing-data counting loop
+#ifndef PGLD_PF_MISSING
+#define PGLD_PF_MISSING 3
+#endif
+
+// hh = A.B, hv = A.Bv, vh = Av.B, vv = Av.Bv over the ks k-steps.  Operand
+// loads run PF k-steps ahead in a register ring indexed by the unrolled
+// position only.  The main loop covers whole groups of PF k-steps with every
+// load issued unconditionally (a load past the last k-step re-reads k-step 0,
+// already in L1, and is never multiplied), so the loop body has no branch and
+// the ring needs no register copies; the remaining ks % PF k-steps follow.
+template <int PF>
+__device__ __forceinline__ void mfma_products4(const v4i* A, const v4i* Av, const v4i* B,
+                                               const v4i* Bv, int ks, v16f& hh, v16f& hv,
+                                               v16f& vh, v16f& vv) {
+  v4i ra[PF], rav[PF], rb[PF], rbv[PF];
+#pragma unroll
+  for (int d = 0; d < PF; ++d) {
+    const int n = d < ks ? d * 64 : 0;
+    ra[d] = A[n]; rav[d] = Av[n]; rb[d] = B[n]; rbv[d] = Bv[n];
+  }
+  const int full = ks / PF;
+  if (full > 0) {
+    // the first k-step starts from zero accumulators (an inline constant of
+    // the MFMA): the 64 accumulator registers need no clearing
+#pragma unroll
+    for (int d = 0; d < PF; ++d) {
+      hh = pgld_mfma_fp4(ra[d], rb[d], d == 0 ? zero16f() : hh);
+      hv = pgld_mfma_fp4(ra[d], rbv[d], d == 0 ? zero16f() : hv);
+      vh = pgld_mfma_fp4(rav[d], rb[d], d == 0 ? zero16f() : vh);
+      vv = pgld_mfma_fp4(rav[d], rbv[d], d == 0 ? zero16f() : vv);
+      const int s = PF + d;
+      const int n = s < ks ? s * 64 : 0;
+      ra[d] = A[n]; rav[d] = Av[n]; rb[d] = B[n]; rbv[d] = Bv[n];
+    }
+  } else {
+    hh = hv = vh = vv = zero16f();
+  }
+  for (int g = 1; g < full; ++g) {
+#pragma unroll
+    for (int d = 0; d < PF; ++d) {
+      hh = pgld_mfma_fp4(ra[d], rb[d], hh);
+      hv = pgld_mfma_fp4(ra[d], rbv[d], hv);
+      vh = pgld_mfma_fp4(rav[d], rb[d], vh);
+      vv = pgld_mfma_fp4(rav[d], rbv[d], vv);
+      const int s = (g + 1) * PF + d;
+      const int n = s < ks ? s * 64 : 0;
+      ra[d] = A[n]; rav[d] = Av[n]; rb[d] = B[n]; rbv[d] = Bv[n];
+    }
+  }
+  const int rem = ks - full * PF;
+#pragma unroll
+  for (int d = 0; d < PF - 1; ++d) {
+    if (d < rem) {
+      hh = pgld_mfma_fp4(ra[d], rb[d], hh);
+      hv = pgld_mfma_fp4(ra[d], rbv[d], hv);
+      vh = pgld_mfma_fp4(rav[d], rb[d], vh);
+      vv = pgld_mfma_fp4(rav[d], rbv[d], vv);
+    }
+  }
+}
+
+// Operand ring in LDS: each wave streams its tile's fragments through a
+// private ring of RING k-steps x 4 fragments (1 KiB each) with
+// global_load_lds (lane l's 16 bytes land at byte 16 l of the slot), so loads
+// run RING k-steps ahead without holding registers.  The LDS writes of these
+// loads are tracked by vmcnt only, hence the explicit waits.
+typedef __attribute__((address_space(1))) void gmem_void;
+typedef __attribute__((address_space(3))) void lmem_void;
+
+__device__ __forceinline__ void lds_dma16(const v4i* g_lane, v4i* lds_slot) {
+  __builtin_amdgcn_global_load_lds((gmem_void*)g_lane, (lmem_void*)lds_slot, 16, 0, 0);
+}
+// Wait until at most n (0, 4, 8, 12 or 16) vector-memory operations of this
+// wave are outstanding; also a compiler barrier for memory accesses.
+__device__ __forceinline__ void wait_vm(int n) {
+  if (n >= 16) asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+  else if (n >= 12) asm volatile("s_waitcnt vmcnt(12)" ::: "memory");
+  else if (n >= 8) asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+  else if (n >= 4) asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+  else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+// The four fragments of one ring slot (1 KiB apart).  Read in one asm block
+// that waits for its own results: the compiler cannot tell these reads from
+// the in-flight ring loads of later k-steps and would otherwise wait for all
+// of them (vmcnt(0)) before every read.
+__device__ __forceinline__ void ring_read4(const v4i* slot_lane, v4i& f0, v4i& f1, v4i& f2,
+                                           v4i& f3) {
+  typedef const __attribute__((address_space(3))) v4i lds_v4i;
+  const uint32_t addr = (uint32_t)(uintptr_t)(lds_v4i*)slot_lane;  // LDS byte address
+  asm volatile(
+      "ds_read_b128 %0, %4\n\t"
+      "ds_read_b128 %1, %4 offset:1024\n\t"
+      "ds_read_b128 %2, %4 offset:2048\n\t"
+      "ds_read_b128 %3, %4 offset:3072\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(f0), "=&v"(f1), "=&v"(f2), "=&v"(f3)
+      : "v"(addr)
+      : "memory");
+}
+
+// hh, hv, vh, vv over the ks k-steps, operands streamed through the ring
+template <int RING>
+__device__ __forceinline__ void ring_products(const v4i* A, const v4i* Av, const v4i* B,
+                                              const v4i* Bv, int ks, int lane, v4i* ring,
+                                              v16f& hh, v16f& hv, v16f& vh, v16f& vv) {
+  const v4i* src[4] = {A, Av, B, Bv};
+  const int pre = ks < RING ? ks : RING;
+  for (int d = 0; d < pre; ++d) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f) lds_dma16(src[f] + d * 64, ring + (d * 4 + f) * 64);
+  }
+  for (int s = 0; s < ks; ++s) {
+    const int ahead = ks - 1 - s < RING - 1 ? ks - 1 - s : RING - 1;  // later k-steps in flight
+    wait_vm(4 * ahead);
+    v4i ah, av, bh, bv;
+    ring_read4(ring + (s % RING) * 4 * 64 + lane, ah, av, bh, bv);
+    hh = pgld_mfma_fp4(ah, bh, hh);
+    hv = pgld_mfma_fp4(ah, bv, hv);
+    vh = pgld_mfma_fp4(av, bh, vh);
+    vv = pgld_mfma_fp4(av, bv, vv);
+    if (s + RING < ks) {
+      asm volatile("" ::: "memory");
+      v4i* dst = ring + (s % RING) * 4 * 64;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) lds_dma16(src[f] + (s + RING) * 64, dst + f * 64);
+    }
+  }
+}
+
+// Counts of population p for tile (rb, cb), packed per slot as
+// w0 = n11 | n10 << 16, w1 = n01 | n00 << 16.  With RING > 0 (and missing
+// calls) the operands stream through the wave's LDS ring.
+template <bool TWO_PASS, int RING = 0>
+__device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t rb, int64_t cb,
+                                            int lane, v16u& w0, v16u& w1, v4i* ring = nullptr) {
+  // MARG: slot words hold the products' marginals (n11 | pA << 16, pB | n << 16
+  // with pA = n11 + n10, pB = n11 + n01, n the valid pairs); otherwise the
+  // four class counts (n11 | n10 << 16, n01 | n00 << 16).  Measured per
+  // traits: marginals for two to four populations (2 x 500 haplotypes over
+  // 200k sites: 3.36 -> 3.19 ms; 4 populations of 40-800 over 500k sites:
+  // 169.3 -> 166.9 ms); class counts for one (300 haplotypes in 5,000 ragged
+  // windows: 3.17 -> 3.29 ms with marginals).
+  constexpr bool MARG = !TWO_PASS;
+  const int ks = a.ksteps[p];
+  const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[p]) + lane;
+  const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[p]) + lane;
+  if (RING > 0 && ring != nullptr && a.missing[p] && !TWO_PASS) {
+    v16f hh = zero16f(), hv = zero16f(), vh = zero16f(), vv = zero16f();
+    ring_products<(RING > 0 ? RING : 1)>(A, A + ks * 64, B, B + ks * 64, ks, lane, ring, hh, hv,
+                                         vh, vv);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int n11 = (int)hh[r], h_v = (int)hv[r], v_h = (int)vh[r], v_v = (int)vv[r];
+      w0[r] = (uint32_t)n11 | ((uint32_t)h_v << 16);
+      w1[r] = (uint32_t)v_h | ((uint32_t)v_v << 16);
+    }
+  } else if (a.missing[p] && !TWO_PASS) {
+    v16f hh, hv, vh, vv;
+    const v4i* Av = A + ks * 64;
+    const v4i* Bv = B + ks * 64;
+    if constexpr (RING == 0) {
+      mfma_products4<PGLD_PF_MISSING>(A, Av, B, Bv, ks, hh, hv, vh, vv);
+    } else {
+      // fallback of the LDS ring traits: the form their register allocation
+      // was measured with
+      hh = hv = vh = vv = zero16f();
+      v4i ah, av, bh, bv;
+      if (ks > 0) { ah = A[0]; av = Av[0]; bh = B[0]; bv = Bv[0]; }
+      for (int s = 0; s < ks; ++s) {
+        hh = pgld_mfma_fp4(ah, bh, hh);
+        hv = pgld_mfma_fp4(ah, bv, hv);
+        vh = pgld_mfma_fp4(av, bh, vh);
+        vv = pgld_mfma_fp4(av, bv, vv);
+        if (s + 1 < ks) {
+          const int n = (s + 1) * 64;
+          ah = A[n]; av = Av[n]; bh = B[n]; bv = Bv[n];
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int n11 = (int)hh[r], h_v = (int)hv[r], v_h = (int)vh[r], v_v = (int)vv[r];
+      if constexpr (MARG) {
+        w0[r] = (uint32_t)n11 | ((uint32_t)h_v << 16);
+        w1[r] = (uint32_t)v_h | ((uint32_t)v_v << 16);
+      } else {
+        w0[r] = (uint32_t)n11 | ((uint32_t)(h_v - n11) << 16);
+        w1[r] = (uint32_t)(v_h - n11) | ((uint32_t)(v_v - h_v - v_h + n11) << 16);
+      }
+    }
+  } else if (a.missing[p]) {
+    // Two passes over K with one row operand each (H, then V) keep two
+    // accumulators live instead of four (for three waves per SIMD); the
+    // column operands are re-read from L1/L2 in the second pass.
+    v16f hh = zero16f(), hv = zero16f();
+    mfma_row_pass(A, B, B + ks * 64, ks, hh, hv);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int n11 = (int)hh[r], h_v = (int)hv[r];
+      w0[r] = (uint32_t)n11 | ((uint32_t)(MARG ? h_v : h_v - n11) << 16);
+    }
+    v16f vh = zero16f(), vv = zero16f();
+    mfma_row_pass(A + ks * 64, B, B + ks * 64, ks, vh, vv);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int v_h = (int)vh[r], v_v = (int)vv[r];
+      if constexpr (MARG) {
+        w1[r] = (uint32_t)v_h | ((uint32_t)v_v << 16);
+      } else {
+        const int n11 = (int)(w0[r] & 0xffffu), h_v = n11 + (int)(w0[r] >> 16);
+        w1[r] = (uint32_t)(v_h - n11) | ((uint32_t)(v_v - h_v - v_h + n11) << 16);
+      }
+    }
+  } else {
+    v16f hh = zero16f();
+    constexpr int PF = 2;
+    v4i ah[PF], bh[PF];
+#pragma unroll
+    for (int d = 0; d < PF; ++d)
+      if (d < ks) { ah[d] = A[d * 64]; bh[d] = B[d * 64]; }
+    for (int s0 = 0; s0 < ks; s0 += PF) {
+#pragma unroll
+      for (int d = 0; d < PF; ++d) {
+        const int s = s0 + d;
+        if (s < ks) {
+          hh = pgld_mfma_fp4(ah[d], bh[d], hh);
+          if (s + PF < ks) { ah[d] = A[(s + PF) * 64]; bh[d] = B[(s + PF) * 64]; }
+        }
+      }
+    }
+    const int32_t* cnt1 = a.site_stats + (int64_t)p * 3 * a.V;
+    const int64_t j = cb * 32 + (lane & 31);
+    const int kj = j < a.V ? cnt1[j] : 0;
+    const int K = a.K[p];
+    // row i = ib + o, o = mfma_row(lane, r) - 4 (lane >> 5) a constant per r:
+    // immediate load offsets from one base, o < lim <=> i < V
+    const int64_t ib = rb * 32 + 4 * (lane >> 5);
+    const int32_t* ci = cnt1 + ib;
+    const int lim = (int)(a.V - ib < 32 ? a.V - ib : 32);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int o = 8 * (r >> 2) + (r & 3);
+      const int ki = o < lim ? ci[o] : 0;
+      const int n11 = (int)hh[r];
+      if constexpr (MARG) {
+        w0[r] = (uint32_t)n11 | ((uint32_t)ki << 16);
+        w1[r] = (uint32_t)kj | ((uint32_t)K << 16);
+      } else {
+        w0[r] = (uint32_t)n11 | ((uint32_t)(ki - n11) << 16);
+        w1[r] = (uint32_t)(kj - n11) | ((uint32_t)(K - ki - kj + n11) << 16);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Genotype tile counts.  Features per sample: b = [g==1], c = [g==2] and, with
+// missing calls, v = [g>=0].  The 9 products give the 9 class counts:
+//   n11 = b.b, n12 = b.c, n21 = c.b, n22 = c.c, n10 = b.v - n11 - n12,
+//   n20 = c.v - n21 - n22, n01 = v.b - n11 - n21, n02 = v.c - n12 - n22,
+//   n00 = v.v - (b.v + c.v) - n01 - n02.
+// Packed per slot in moments order g1..g9 (n22, n21, n20, n12, n11, n10, n02,
+// n01, n00) as five 16-bit pairs.
+// ---------------------------------------------------------------------------
+// Pack the nine class counts of slot r (moments order) into w[.][r].
+template <int W>
+__device__ __forceinline__ void geno_pack_slot(v16u (&w)[W], int r, const int (&c)[9]) {
+  if constexpr (W == 3) {  // 10-bit fields: every count <= population size <= 1023
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      w[q][r] = (uint32_t)c[3 * q] | ((uint32_t)c[3 * q + 1] << 10) |
+                ((uint32_t)c[3 * q + 2] << 20);
+  } else {  // 16-bit fields
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q][r] = (uint32_t)c[2 * q] | ((uint32_t)c[2 * q + 1] << 16);
+    w[4][r] = (uint32_t)c[8];
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, int64_t rb,
+                                                 int64_t cb, int lane, v16u (&w)[W]) {
+  const int ks = a.ksteps[p];
+  const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[p]) + lane;
+  const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[p]) + lane;
+  const int fs = ks * 64;  // feature stride in v4i
+  v16f bb, bc, cbm, cc;
+  if (a.missing[p]) {
+    v16f bv, cv, vb, vc, vv;
+    // one k-step ahead: nine accumulators leave no registers for more (two
+    // steps ahead spilled and was no faster).  The first k-step starts from
+    // zero accumulators (an inline constant of the MFMA), so the 144
+    // accumulator registers need no clearing; the next step's loads go to the
+    // operand registers once this step's MFMAs have read them.
+    if (ks > 0) {
+      v4i fa[3], fb[3];
+#pragma unroll
+      for (int f = 0; f < 3; ++f) { fa[f] = A[f * fs]; fb[f] = B[f * fs]; }
+      bb = pgld_mfma_fp4(fa[0], fb[0], zero16f());
+      bc = pgld_mfma_fp4(fa[0], fb[1], zero16f());
+      bv = pgld_mfma_fp4(fa[0], fb[2], zero16f());
+      cbm = pgld_mfma_fp4(fa[1], fb[0], zero16f());
+      cc = pgld_mfma_fp4(fa[1], fb[1], zero16f());
+      cv = pgld_mfma_fp4(fa[1], fb[2], zero16f());
+      vb = pgld_mfma_fp4(fa[2], fb[0], zero16f());
+      vc = pgld_mfma_fp4(fa[2], fb[1], zero16f());
+      vv = pgld_mfma_fp4(fa[2], fb[2], zero16f());
+      if (ks > 1)
+#pragma unroll
+        for (int f = 0; f < 3; ++f) { fa[f] = A[f * fs + 64]; fb[f] = B[f * fs + 64]; }
+      for (int s = 1; s < ks; ++s) {
+        bb = pgld_mfma_fp4(fa[0], fb[0], bb);
+        bc = pgld_mfma_fp4(fa[0], fb[1], bc);
+        bv = pgld_mfma_fp4(fa[0], fb[2], bv);
+        cbm = pgld_mfma_fp4(fa[1], fb[0], cbm);
+        cc = pgld_mfma_fp4(fa[1], fb[1], cc);
+        cv = pgld_mfma_fp4(fa[1], fb[2], cv);
+        vb = pgld_mfma_fp4(fa[2], fb[0], vb);
+        vc = pgld_mfma_fp4(fa[2], fb[1], vc);
+        vv = pgld_mfma_fp4(fa[2], fb[2], vv);
+        if (s + 1 < ks) {
+          const int n = (s + 1) * 64;
+#pragma unroll
+          for (int f = 0; f < 3; ++f) { fa[f] = A[f * fs + n]; fb[f] = B[f * fs + n]; }
+        }
+      }
+    } else {
+      bb = bc = cbm = cc = bv = cv = vb = vc = vv = zero16f();
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int n11 = (int)bb[r], n12 = (int)bc[r], n21 = (int)cbm[r], n22 = (int)cc[r];
+      const int b_v = (int)bv[r], c_v = (int)cv[r];
+      const int n10 = b_v - n11 - n12, n20 = c_v - n21 - n22;
+      const int n01 = (int)vb[r] - n11 - n21, n02 = (int)vc[r] - n12 - n22;
+      const int n00 = (int)vv[r] - b_v - c_v - n01 - n02;
+      const int c[9] = {n22, n21, n20, n12, n11, n10, n02, n01, n00};
+      geno_pack_slot(w, r, c);
+    }
+  } else {
+    bb = bc = cbm = cc = zero16f();
+    constexpr int PF = 2;
+    v4i fa[PF][2], fb[PF][2];
+#pragma unroll
+    for (int d = 0; d < PF; ++d)
+      if (d < ks)
+#pragma unroll
+        for (int f = 0; f < 2; ++f) { fa[d][f] = A[f * fs + d * 64]; fb[d][f] = B[f * fs + d * 64]; }
+    for (int s0 = 0; s0 < ks; s0 += PF) {
+#pragma unroll
+      for (int d = 0; d < PF; ++d) {
+        const int s = s0 + d;
+        if (s < ks) {
+          bb = pgld_mfma_fp4(fa[d][0], fb[d][0], bb);
+          bc = pgld_mfma_fp4(fa[d][0], fb[d][1], bc);
+          cbm = pgld_mfma_fp4(fa[d][1], fb[d][0], cbm);
+          cc = pgld_mfma_fp4(fa[d][1], fb[d][1], cc);
+          if (s + PF < ks) {
+            const int n = (s + PF) * 64;
+#pragma unroll
+            for (int f = 0; f < 2; ++f) { fa[d][f] = A[f * fs + n]; fb[d][f] = B[f * fs + n]; }
+          }
+        }
+      }
+    }
+    const int32_t* st = a.site_stats + (int64_t)p * 3 * a.V;
+    const int64_t j = cb * 32 + (lane & 31);
+    const int vbj = j < a.V ? st[j] : 0, vcj = j < a.V ? st[a.V + j] : 0;
+    const int K = a.K[p];
+    // rows as in tile_counts: i = ib + o with o constant per r
+    const int64_t ib = rb * 32 + 4 * (lane >> 5);
+    const int32_t* si = st + ib;
+    const int32_t* sci = st + a.V + ib;
+    const int lim = (int)(a.V - ib < 32 ? a.V - ib : 32);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int o = 8 * (r >> 2) + (r & 3);
+      const int bvi = o < lim ? si[o] : 0, cvi = o < lim ? sci[o] : 0;
+      const int n11 = (int)bb[r], n12 = (int)bc[r], n21 = (int)cbm[r], n22 = (int)cc[r];
+      const int n10 = bvi - n11 - n12, n20 = cvi - n21 - n22;
+      const int n01 = vbj - n11 - n21, n02 = vcj - n12 - n22;
+      const int n00 = K - bvi - cvi - n01 - n02;
+      const int c[9] = {n22, n21, n20, n12, n11, n10, n02, n01, n00};
+      geno_pack_slot(w, r, c);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Traits: what a tile's counts are and how a pair's statistics follow.
+// ---------------------------------------------------------------------------
+// Entry rt_off + max(n - rt_lo, 0): n below the table's range only occurs in
+// padded slots (n = 0), whose statistics are weighted out.  FOLD adds the
+// offset before the clamp (two instructions fewer per population and pair;
+// the four-population kernel keeps the other form, which it spills less with).
+template <bool FOLD = true>
+__device__ __forceinline__ const double* rt_entry(const FusedParams& a, const double* srt, int p,
+                                                  int n) {
+  if constexpr (FOLD) {
+    const int e = n + (a.rt_off[p] - a.rt_lo[p]);
+    return srt + 4 * (e < a.rt_off[p] ? a.rt_off[p] : e);
+  } else {
+    int e = n - a.rt_lo[p];
+    e = e < 0 ? 0 : e;
+    return srt + 4 * (a.rt_off[p] + e);
+  }
+}
+
+// Statistic groups of a traits class T: groups 0 .. NG-1 hold the LD
+// statistics (GS each, consecutive), and with R2 one more group holds
+// (sum of r^2 over pairs where it is defined, number of such pairs) of the
+// first population.  Output row per bin: NSO = NS (+2) sums, then the pair count.
+template <int P, bool R2 = false>
+struct HapTraits {
+  static constexpr int NP = P;
+  static constexpr bool WITH_R2 = R2;
+  // Workgroups per CU (waves per SIMD): as many as the live state allows
+  // without spilling in the tile loop -- three for one population (168
+  // registers), two for two (256), one for three or four populations, which
+  // keep more counts and use the full register file.
+  static constexpr int MINB = P == 1 ? 3 : P == 2 ? 2 : 1;
+  static constexpr int NS = P == 1 ? HAP_NSTATS_P1 : P == 2 ? HAP_NSTATS_P2 : P == 3 ? HAP_NSTATS_P3
+                                                                                     : HAP_NSTATS_P4;
+  static constexpr int NG = P == 1 ? HAP_NGROUPS_P1 : P == 2 ? HAP_NGROUPS_P2 : P == 3 ? HAP_NGROUPS_P3
+                                                                                       : HAP_NGROUPS_P4;
+  // With one statistic group the r^2 pair joins it (statistics NS, NS+1),
+  // so the population bases are derived once per pair; otherwise it is a
+  // group of its own after the NG groups.
+  static constexpr bool MERGE_R2 = R2 && (P == 1 ? HAP_NGROUPS_P1 : 2) == 1;
+  static constexpr int GS = (P == 1 ? HAP_GS_P1 : P == 2 ? HAP_GS_P2 : P == 3 ? HAP_GS_P3
+                                                                          : HAP_GS_P4) +
+                            (MERGE_R2 ? 2 : 0);  // per group
+  static constexpr int NSO = NS + (R2 ? 2 : 0);
+  static constexpr int NGT = NG + (R2 && !MERGE_R2 ? 1 : 0);
+  // k-steps of the LDS operand ring per wave (0: register ring)
+  static constexpr int RING = P >= 4 ? 4 : 0;
+  static constexpr int RING_BYTES = RING * 4 * 1024;  // per wave
+  // the group's sums hold statistic q times 2^e (raw numerators; 0 past the
+  // statistics and for the r^2 slots)
+  template <int G>
+  __device__ __forceinline__ static int scale_exp(int q) {
+    if constexpr (G < NG) return hap_scale_group<P, G>(q);
+    else return 0;
+  }
+  // statistics of pass G, and the output column of its q-th (the r^2 pair
+  // after the group's statistics, or the r^2 group, lands at NS, NS + 1)
+  template <int G>
+  __device__ static constexpr int gsz() {
+    if constexpr (G < NG) return hap_group_size<P, G>() + (MERGE_R2 ? 2 : 0);
+    else return 2;
+  }
+  template <int G>
+  __device__ __forceinline__ static int col(int q) {
+    if constexpr (G < NG) return hap_stat_col<P, G>(q);
+    else return NS + q;
+  }
+  struct Counts { v16u w0[P], w1[P]; };  // slot words of tile_counts<., RING>
+  // population p's basis for slot r
+  __device__ __forceinline__ static HapPop pop(const FusedParams& a, const double* srt,
+                                               const Counts& c, int p, int r) {
+    const uint32_t w0 = c.w0[p][r], w1 = c.w1[p][r];
+    if constexpr (MINB < 3) {  // tile_counts' MARG
+      const int vv = (int)(w1 >> 16);
+      return hap_pop_marg((int)(w0 & 0xffff), (int)(w0 >> 16), (int)(w1 & 0xffff), vv,
+                          rt_entry(a, srt, p, vv));
+    } else {
+      const int n11 = w0 & 0xffff, n10 = w0 >> 16, n01 = w1 & 0xffff, n00 = w1 >> 16;
+      return hap_pop(n11, n10, n01, n00, 0,
+                     rt_entry<(P < 4)>(a, srt, p, n11 + n10 + n01 + n00));
+    }
+  }
+  __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane,
+                                                Counts& c, v4i* ring) {
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+      tile_counts<(MINB >= 3), RING>(a, p, rb, cb, lane, c.w0[p], c.w1[p], ring);
+  }
+  // statistics G*GS .. G*GS+GS-1 of the pair in accumulator slot r
+  template <int G>
+  __device__ __forceinline__ static void stats(const FusedParams& a, const double* srt, const Counts& c, int r,
+                               double* x) {
+    HapPop pp[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      pp[p] = pop(a, srt, c, p, r);
+    }
+    if constexpr (G < NG) {
+      hap_stats_group<P, G>(pp, x);
+      if constexpr (MERGE_R2) hap_r2(pp[0], x + GS - 2);
+    } else {
+      hap_r2(pp[0], x);
+    }
+  }
+  // acc[q] += wt * statistic G*GS+q (wt 0 or 1)
+  template <int G>
+  __device__ __forceinline__ static void accum(const FusedParams& a, const double* srt,
+                                               const Counts& c, int r, double* acc, double wt) {
+    if constexpr (G < NG) {
+      HapPop pp[P];
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        pp[p] = pop(a, srt, c, p, r);
+      }
+      hap_acc_group<P, G>(pp, acc, wt);
+      if constexpr (MERGE_R2) {
+        double x[2];
+        hap_r2(pp[0], x);
+        acc[GS - 2] = fma(x[0], wt, acc[GS - 2]);
+        acc[GS - 1] = fma(x[1], wt, acc[GS - 1]);
+      }
+    } else {
+      double x[2];
+      stats<G>(a, srt, c, r, x);
+      acc[0] = fma(x[0], wt, acc[0]);
+      acc[1] = fma(x[1], wt, acc[1]);
+    }
+  }
+};
+
+// W = 3: nine 10-bit counts in three words per slot (population sizes up to
+// 1023); W = 5: 16-bit counts in five words (up to 32767).
+template <int P, int W, bool R2 = false>
+struct GenoTraits {
+  static constexpr int NP = P;
+  static constexpr bool WITH_R2 = R2;
+  // One population fits two waves per SIMD; more need the full register file
+  // of a one-wave-per-SIMD launch.
+  static constexpr int MINB = P == 1 ? 2 : 1;
+  static constexpr int NS = P == 1 ? GENO_NSTATS_P1 : P == 2 ? GENO_NSTATS_P2 : P == 3 ? GENO_NSTATS_P3
+                                                                                        : GENO_NSTATS_P4;
+  static constexpr int NG = P == 1 ? GENO_NGROUPS_P1 : P == 2 ? GENO_NGROUPS_P2 : P == 3 ? GENO_NGROUPS_P3
+                                                                                          : GENO_NGROUPS_P4;
+  // the r^2 pair joins a single statistic group (as in HapTraits)
+  static constexpr bool MERGE_R2 = R2 && (P == 1 ? GENO_NGROUPS_P1 : 2) == 1;
+  static constexpr int GS = (P == 1 ? GENO_GS_P1 : P == 2 ? GENO_GS_P2 : P == 3 ? GENO_GS_P3
+                                                                           : GENO_GS_P4) +
+                            (MERGE_R2 ? 2 : 0);
+  static constexpr int NSO = NS + (R2 ? 2 : 0);
+  static constexpr int NGT = NG + (R2 && !MERGE_R2 ? 1 : 0);
+  static constexpr int RING = 0;
+  static constexpr int RING_BYTES = 0;
+  template <int G>
+  __device__ __forceinline__ static int scale_exp(int) { return 0; }
+  template <int G>
+  __device__ static constexpr int gsz() {
+    if constexpr (G < NG) return geno_group_size<P, G>() + (MERGE_R2 ? 2 : 0);
+    else return 2;
+  }
+  template <int G>
+  __device__ __forceinline__ static int col(int q) {
+    if constexpr (G < NG) return geno_stat_col<P, G>(q);
+    else return NS + q;
+  }
+  struct Counts { v16u w[P][W]; };
+  __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane,
+                                                Counts& c, v4i*) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) geno_tile_counts<W>(a, p, rb, cb, lane, c.w[p]);
+  }
+  template <int G>
+  __device__ __forceinline__ static void stats(const FusedParams& a, const double* srt, const Counts& c, int r,
+                               double* x) {
+    GenoPop pp[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      int g[9];
+      if constexpr (W == 3) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const uint32_t v = c.w[p][q][r];
+          g[3 * q] = v & 0x3ff;
+          g[3 * q + 1] = (v >> 10) & 0x3ff;
+          g[3 * q + 2] = v >> 20;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t v = c.w[p][q][r];
+          g[2 * q] = v & 0xffff;
+          g[2 * q + 1] = v >> 16;
+        }
+        g[8] = c.w[p][4][r];
+      }
+      int y[9];
+      geno_centered(g, y);  // y[0] = valid individuals
+      pp[p] = geno_pop(g, y, 0, rt_entry(a, srt, p, y[0]));
+    }
+    if constexpr (G < NG) {
+      geno_stats_group<P, G>(pp, x);
+      if constexpr (MERGE_R2) geno_r2(pp[0], x + GS - 2);
+    } else {
+      geno_r2(pp[0], x);
+    }
+  }
+  template <int G>
+  __device__ __forceinline__ static void accum(const FusedParams& a, const double* srt,
+                                               const Counts& c, int r, double* acc, double wt) {
+    constexpr int GSZ = gsz<G>();
+    double x[GSZ];
+    stats<G>(a, srt, c, r, x);
+#pragma unroll
+    for (int q = 0; q < GSZ; ++q) acc[q] = fma(x[q], wt, acc[q]);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Binning.  Each wave owns an FP64 staging array stage[bin][slot] for the GS
+// statistics of the group being evaluated, and a pair counter per bin, in
+// LDS.  Inside a tile, lanes add their running per-bin sums to the staging
+// array (LDS FP64 atomics, wave-private region); at the end of the group's
+// pass the wave converts the touched bins to 3-limb fixed point, adds them to
+// the workgroup's integer limbs and clears the staging entries.  The FP64
+// rounding is thus confined to the 1024 pairs of one tile, and the integer
+// limb sums across tiles, waves and workgroups are exact, so the result does
+// not depend on how tiles are dealt to waves.
+// ---------------------------------------------------------------------------
+template <int GS, bool COUNT>
+__device__ __forceinline__ void stage_add(double* stg, int* scnt, int bin, const double* acc,
+                                          int count) {
+  double* d = stg + bin * GS;
+#pragma unroll
+  for (int q = 0; q < GS; ++q) atomicAdd(d + q, acc[q]);
+  if constexpr (COUNT) atomicAdd(scnt + bin, count);
+}
+
+// Move staged bins [blo, bhi] of this wave into the workgroup limbs of
+// statistics s0 .. s0+GS-1 (and the pair counters).
+template <class T, int G, int NS, int GS>
+__device__ __forceinline__ void stage_drain(double* stg, int* scnt, unsigned long long* slimb,
+                                            int lane, int blo, int bhi) {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const int E = (bhi - blo + 1) * (GS + 1);
+  for (int e = lane; e < E; e += 64) {
+    const int b = blo + e / (GS + 1), q = e - (b - blo) * (GS + 1);
+    if (q < GS) {
+      const double v = stg[b * GS + q];
+      if (v != 0.0) {
+        stg[b * GS + q] = 0.0;
+        unsigned long long* dst = slimb + ((int64_t)b * (NS + 1) + T::template col<G>(q)) * 3;
+        long long x2, x1, x0;
+        pgld_to_limbs(v, x2, x1, x0, pgld_limb_scale(T::template scale_exp<G>(q)));
+        atomicAdd(dst + 0, (unsigned long long)x0);
+        atomicAdd(dst + 1, (unsigned long long)x1);
+        atomicAdd(dst + 2, (unsigned long long)x2);
+      }
+    } else {
+      const int c = scnt[b];
+      if (c) {
+        scnt[b] = 0;
+        atomicAdd(slimb + ((int64_t)b * (NS + 1) + NS) * 3, (unsigned long long)c);
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Adds the workgroup's LDS limbs to the global result (carry-normalised).
+template <int NS>
+__device__ __forceinline__ void flush_global(unsigned long long* slimb, unsigned long long* out,
+                                             int n_bins) {
+  for (int e = threadIdx.x; e < n_bins * (NS + 1); e += blockDim.x) {
+    unsigned long long* L = slimb + 3 * e;
+    if (e % (NS + 1) == NS) {
+      if (L[0]) atomicAdd(out + 3 * e, L[0]);
+      continue;
+    }
+    long long l0 = (long long)L[0], l1 = (long long)L[1], l2 = (long long)L[2];
+    pgld_normalize_limbs(l2, l1, l0);
+    if (l0) atomicAdd(out + 3 * e + 0, (unsigned long long)l0);
+    if (l1) atomicAdd(out + 3 * e + 1, (unsigned long long)l1);
+    if (l2) atomicAdd(out + 3 * e + 2, (unsigned long long)l2);
+  }
+}
+
+// Signed coordinate difference c[j] - c[i]; for i < j it is the pair's
+// distance (coordinates are non-decreasing along the sites), and over a tile
+// it is smallest at (i1, j0) and largest at (i0, j1).
+__device__ __forceinline__ double coord_diff(const FusedParams& a, int64_t i, int64_t j) {
+  return a.coord_is_pos ? (a.pos[j] - a.pos[i]) : (a.coord[j] - a.coord[i]);
+}
+
+struct TileCtx {
+  const double* sedges;
+  const double* srt;
+  double* stg;   // this wave's staging [nb][GS]
+  int* scw;      // this wave's pair counters [nb]
+  unsigned long long* slimb;
+  int lane, bmin, bmax;
+  bool fast;   // every pair valid and in bin bmin
+  bool dual;   // otherwise: every valid pair in bin blo or blo + 1
+  int blo;
+  int64_t i0, j0;
+};
+
+// Wave totals of a group's per-lane sums for one bin, added to the workgroup
+// limbs (lane q < GS: the group's statistic q, column T::col<G>(q)); with
+// COUNT, `pairs` joins the bin's pair counter.
+template <class T, int G, int NS, int GS, bool COUNT>
+__device__ __forceinline__ void reduce_to_limbs(const TileCtx& t, double (&acc)[GS], int bin,
+                                                unsigned long long pairs) {
+  const int lane = t.lane;
+  // one butterfly reduce-scatter of the GS sums (padded to NV): lane l ends
+  // with statistic bitreverse(l mod NV); the first padding slot carries the
+  // pair count
+  constexpr int NV = GS <= 2 ? 2 : GS <= 4 ? 4 : GS <= 8 ? 8 : GS <= 16 ? 16 : GS <= 32 ? 32 : 64;
+  constexpr int LOG = NV == 2 ? 1 : NV == 4 ? 2 : NV == 8 ? 3 : NV == 16 ? 4 : NV == 32 ? 5 : 6;
+  static_assert(GS <= 64 && (!COUNT || GS < NV), "group size and count slot");
+  double v[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = q < GS ? acc[q] : 0.0;
+  const double tot = pgld_wave_sums<NV>(v, lane);
+  const int q = pgld_bitrev(lane & (NV - 1), LOG);
+  if (lane < NV && q < GS) {
+    unsigned long long* dst = t.slimb + ((int64_t)bin * (NS + 1) + T::template col<G>(q)) * 3;
+    long long x2, x1, x0;
+    pgld_to_limbs(tot, x2, x1, x0, pgld_limb_scale(T::template scale_exp<G>(q)));
+    atomicAdd(dst + 0, (unsigned long long)x0);
+    atomicAdd(dst + 1, (unsigned long long)x1);
+    atomicAdd(dst + 2, (unsigned long long)x2);
+  } else if (COUNT && lane < NV && q == GS) {
+    atomicAdd(t.slimb + ((int64_t)bin * (NS + 1) + NS) * 3, pairs);
+  }
+}
+
+// Bins of the lane's 16 pairs of a tile that is not single-bin: 8 bits per
+// slot r in word r >> 2, 0xFF where the pair is not counted (i >= j, beyond
+// the cutoff, outside the bins).  All loads are issued before the epilogue.
+__device__ __forceinline__ void tile_bins(const FusedParams& a, const TileCtx& t,
+                                          uint32_t (&bw)[4]) {
+  const int nb = a.n_bins, lane = t.lane;
+  const int64_t j = t.j0 + (lane & 31);
+  const bool jok = j < a.V;
+  const double pos_j = jok ? a.pos[j] : 0.0;
+  const double co_j = (jok && !a.coord_is_pos) ? a.coord[j] : pos_j;
+  int guess = nb - 1;
+  // row i = ib + o with o = 8q + k a constant (mfma_row(lane, 4q + k)): the
+  // loads take immediate offsets from one base and i < j compares o with jr
+  const int64_t ib = t.i0 + 4 * (lane >> 5);
+  const double* pos_i = a.pos + ib;
+  const double* co_i = a.coord + ib;
+  const int jr = (int)(j - ib < 32 ? j - ib : 32);  // o < jr <=> i < j (o < 32)
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int o = 8 * q + k;
+      uint32_t b8 = 0xFF;
+      if (jok && o < jr) {
+        const double pi = pos_i[o];
+        const double d = a.coord_is_pos ? (pos_j - pi) : fabs(co_j - co_i[o]);
+        const int b = pgld_bin_scan(d, t.sedges, nb, guess, nb - 1);
+        if (pos_j <= pi + a.max_dist && b >= 0 && b < nb) {
+          b8 = (uint32_t)b;
+          guess = b;
+        }
+      }
+      w |= b8 << (8 * k);
+    }
+    bw[q] = w;
+  }
+}
+
+// One pass over the tile's 1024 pairs for statistic group G.  Single-bin
+// tiles are reduced to limbs at the tile end, so FP64 rounding never spans
+// more than one tile (and the result does not depend on the schedule).
+template <class T, int G>
+__device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& t,
+                                           const typename T::Counts& cts) {
+  // NS below is the output row width; GS the size of this group
+  constexpr int NS = T::NSO;
+  constexpr int GS = T::template gsz<G>();
+  // slot loops unrolled by two (two pairs' FP64 chains interleave) for the
+  // traits that do not spill with it: one population, or two haplotype
+  // populations at two waves per SIMD
+  constexpr bool U2 = T::NP == 1 || (T::NP == 2 && T::MINB == 2);
+  const int nb = a.n_bins, lane = t.lane;
+  double acc[GS];
+#pragma unroll
+  for (int q = 0; q < GS; ++q) acc[q] = 0.0;
+  if (t.fast) {
+    // every pair of the tile is valid and falls in bin bmin
+    if constexpr (U2) {  // 2 x 500 haplotypes, 200k sites: 3.21 -> 3.18 ms
+#pragma unroll 2
+      for (int r = 0; r < 16; ++r) T::template accum<G>(a, t.srt, cts, r, acc, 1.0);
+    } else {
+#pragma unroll 1
+      for (int r = 0; r < 16; ++r) T::template accum<G>(a, t.srt, cts, r, acc, 1.0);
+    }
+    reduce_to_limbs<T, G, NS, GS, G == 0>(t, acc, t.bmin, 1024ull);
+  } else if (t.dual) {
+    // Two bins: each pair's statistics go to both accumulators with 0/1
+    // weights (the estimators are finite for every slot: counts of padded
+    // sites are zero and the reciprocal table covers every n), so the wave
+    // does the fast path's work plus one FMA per statistic, and the per-bin
+    // pair counts are ballots.
+    uint32_t bw[4];
+    tile_bins(a, t, bw);
+    double acc1[GS];
+#pragma unroll
+    for (int q = 0; q < GS; ++q) acc1[q] = 0.0;
+    unsigned long long cnt0 = 0, cnt1 = 0;
+    if constexpr (U2) {
+#pragma unroll 2
+      for (int r = 0; r < 16; ++r) {
+        const int rq = r >> 2;
+        const uint32_t w = rq == 0 ? bw[0] : rq == 1 ? bw[1] : rq == 2 ? bw[2] : bw[3];
+        const int b = (int)((w >> (8 * (r & 3))) & 0xFFu);
+        const bool in0 = b == t.blo, in1 = b == t.blo + 1;
+        if constexpr (G == 0) {
+          cnt0 += (unsigned long long)__popcll(__ballot(in0));
+          cnt1 += (unsigned long long)__popcll(__ballot(in1));
+        }
+        T::template accum<G>(a, t.srt, cts, r, acc, in0 ? 1.0 : 0.0);
+        T::template accum<G>(a, t.srt, cts, r, acc1, in1 ? 1.0 : 0.0);
+      }
+    } else {
+#pragma unroll 1
+      for (int r = 0; r < 16; ++r) {
+        const int rq = r >> 2;
+        const uint32_t w = rq == 0 ? bw[0] : rq == 1 ? bw[1] : rq == 2 ? bw[2] : bw[3];
+        const int b = (int)((w >> (8 * (r & 3))) & 0xFFu);
+        const bool in0 = b == t.blo, in1 = b == t.blo + 1;
+        if constexpr (G == 0) {
+          cnt0 += (unsigned long long)__popcll(__ballot(in0));
+          cnt1 += (unsigned long long)__popcll(__ballot(in1));
+        }
+        T::template accum<G>(a, t.srt, cts, r, acc, in0 ? 1.0 : 0.0);
+        T::template accum<G>(a, t.srt, cts, r, acc1, in1 ? 1.0 : 0.0);
+      }
+    }
+    reduce_to_limbs<T, G, NS, GS, G == 0>(t, acc, t.blo, cnt0);
+    if (t.blo + 1 < nb) reduce_to_limbs<T, G, NS, GS, G == 0>(t, acc1, t.blo + 1, cnt1);
+  } else {
+    uint32_t bw[4];
+    tile_bins(a, t, bw);
+    int cur = -1, count = 0;
+#pragma unroll 1
+    for (int r = 0; r < 16; ++r) {
+      const int rq = r >> 2;
+      const uint32_t w = rq == 0 ? bw[0] : rq == 1 ? bw[1] : rq == 2 ? bw[2] : bw[3];
+      const int b = (int)((w >> (8 * (r & 3))) & 0xFFu);
+      if (b != 0xFF) {
+        double x[GS];
+#pragma unroll
+        for (int q = 0; q < GS; ++q) x[q] = 0.0;
+        T::template accum<G>(a, t.srt, cts, r, x, 1.0);
+        if (b != cur) {
+          if (cur >= 0) stage_add<GS, G == 0>(t.stg, t.scw, cur, acc, count);
+          cur = b;
+          count = 0;
+#pragma unroll
+          for (int q = 0; q < GS; ++q) acc[q] = 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < GS; ++q) acc[q] += x[q];
+        ++count;
+      }
+    }
+    if (cur >= 0) stage_add<GS, G == 0>(t.stg, t.scw, cur, acc, count);
+    stage_drain<T, G, NS, GS>(t.stg, t.scw, t.slimb, lane, t.bmin < 0 ? 0 : t.bmin,
+                              t.bmax >= nb ? nb - 1 : t.bmax);
+  }
+}
+
+template <class T, int G>
+__device__ __forceinline__ void tile_groups(const FusedParams& a, const TileCtx& t,
+                                            const typename T::Counts& cts) {
+  if constexpr (G < T::NGT) {
+    tile_group<T, G>(a, t, cts);
+    tile_groups<T, G + 1>(a, t, cts);
+  }
+}
+
+
+}  // namespace pgld

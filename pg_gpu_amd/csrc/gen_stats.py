@@ -122,14 +122,62 @@ def stat_set_plan(P, keyfn):
     return values, stats
 
 
-GROUP = 15  # statistics evaluated per pass of the fused epilogue
-# larger groups where the registers allow (one wave per SIMD): each pass
-# re-derives every population's basis from the counts
-GROUP_OF = {("hap", 4): 53}
+GROUP = 15  # statistics evaluated per pass of the fused epilogue (consecutive groups)
+
+# Passes by population set: each pass evaluates the statistics whose
+# populations form one of the listed sets, so a pass derives the bases of at
+# most three populations (a basis is ~23 FP64 values).  Used where evaluating
+# consecutive statistics keeps the bases of every population live at once and
+# spills: four haplotype populations, three and four genotype populations.
+_P4_PASSES = [
+    [{0}, {1}, {0, 1}],
+    [{2}, {3}, {2, 3}],
+    [{0, 2}, {0, 3}, {0, 2, 3}],
+    [{1, 2}, {1, 3}, {1, 2, 3}],
+    [{0, 1, 2}, {0, 1, 3}, {0, 1, 2, 3}],
+]
+_P3_PASSES = [
+    [{0}, {1}, {0, 1}],
+    [{2}, {0, 2}],
+    [{1, 2}, {0, 1, 2}],
+]
+POPSET_PASSES = {("hap", 4): _P4_PASSES, ("geno", 3): _P3_PASSES, ("geno", 4): _P4_PASSES}
 
 
-def group_size(prefix, P):
-    return GROUP_OF.get((prefix, P), GROUP)
+def stat_pops(name):
+    return frozenset(int(x) for x in name.split("_")[1:])
+
+
+def plan_groups(prefix, P):
+    """Statistic indices (ld_names order) of each pass of the fused epilogue."""
+    names = ld_names(P)
+    passes = POPSET_PASSES.get((prefix, P))
+    if passes is None:
+        return [list(range(k, min(k + GROUP, len(names)))) for k in range(0, len(names), GROUP)]
+    groups = []
+    for sets in passes:
+        want = set(frozenset(x) for x in sets)
+        groups.append([i for i, n in enumerate(names) if stat_pops(n) in want])
+    assert sorted(i for g in groups for i in g) == list(range(len(names)))
+    return groups
+
+
+def emit_group_info(prefix, P, groups, n_stats):
+    """Per-group size and statistic -> output column map (7-bit fields, 9 per
+    64-bit word, selected with immediates: no memory access)."""
+    lines = []
+    for k, grp in enumerate(groups):
+        lines.append(f"template <> __device__ constexpr int {prefix}_group_size<{P}, {k}>() "
+                     f"{{ return {len(grp)}; }}")
+        words = [sum(c << (7 * i) for i, c in enumerate(grp[w:w + 9]))
+                 for w in range(0, len(grp), 9)]
+        sel = "".join(f"q < {9 * (i + 1)} ? 0x{w:x}ull : " for i, w in enumerate(words)) + "0ull"
+        lines.append(f"template <> __device__ __forceinline__ int {prefix}_stat_col<{P}, {k}>(int q) {{")
+        lines.append(f"  if (q >= {len(grp)}) return {n_stats} + q - {len(grp)};")
+        lines.append(f"  const unsigned long long w = {sel};")
+        lines.append("  return (int)((w >> (7 * (q % 9))) & 127ull);")
+        lines.append("}")
+    return lines
 
 
 def emit_set(prefix, popty, P, keyfn):
@@ -138,15 +186,17 @@ def emit_set(prefix, popty, P, keyfn):
     re-evaluation.  Group k writes out[0 .. size-1] for statistics
     k*GROUP .. k*GROUP+size-1."""
     values, stats = stat_set_plan(P, keyfn)
-    gs = group_size(prefix, P)
-    ng = (len(stats) + gs - 1) // gs
+    groups = plan_groups(prefix, P)
+    ng = len(groups)
+    gs = max(len(g) for g in groups)
     lines = [f"// P = {P}: {len(stats)} statistics from {len(values)} distinct raw-call values, "
              f"{ng} group(s) of up to {gs}",
              f"#define {prefix.upper()}_NSTATS_P{P} {len(stats)}",
              f"#define {prefix.upper()}_NGROUPS_P{P} {ng}",
-             f"#define {prefix.upper()}_GS_P{P} {min(gs, len(stats))}"]
+             f"#define {prefix.upper()}_GS_P{P} {gs}"]
+    lines += emit_group_info(prefix, P, groups, len(stats))
     for k in range(ng):
-        grp = stats[k * gs:(k + 1) * gs]
+        grp = [stats[i] for i in groups[k]]
         used = sorted(set(v for terms in grp for v, _ in terms))
         lines.append(f"__device__ __forceinline__ void {prefix}_stats_P{P}_g{k}(const {popty}* p, "
                      "double* out) {")
@@ -163,10 +213,14 @@ def emit_set(prefix, popty, P, keyfn):
         lines.append("}")
         lines.append(f"template <> __device__ __forceinline__ void {prefix}_stats_group<{P}, {k}>("
                      f"const {popty}* p, double* out) {{ {prefix}_stats_P{P}_g{k}(p, out); }}")
-    # whole set (used by the per-pair and plain kernels)
+    # whole set in ld_names order (used by the per-pair and plain kernels)
     lines.append(f"__device__ __forceinline__ void {prefix}_stats_P{P}(const {popty}* p, double* out) {{")
     for k in range(ng):
-        lines.append(f"  {prefix}_stats_P{P}_g{k}(p, out + {k * gs});")
+        if groups[k] == list(range(groups[k][0], groups[k][0] + len(groups[k]))):
+            lines.append(f"  {prefix}_stats_P{P}_g{k}(p, out + {groups[k][0]});")
+        else:
+            lines.append(f"  {{ double t[{len(groups[k])}]; {prefix}_stats_P{P}_g{k}(p, t);")
+            lines.append("    " + " ".join(f"out[{c}] = t[{q}];" for q, c in enumerate(groups[k])) + " }")
     lines.append("}")
     return "\n".join(lines)
 
@@ -216,13 +270,13 @@ def emit_acc_set(P):
     constants); the sums are scaled by 2^-e_s where they become limbs
     (hap_scale_group)."""
     values, stats = stat_set_plan(P, hap_key)
-    gs = group_size("hap", P)
-    ng = (len(stats) + gs - 1) // gs
+    groups = plan_groups("hap", P)
+    ng = len(groups)
     fold = P in HAP_FOLD_P
     inner = HAP_INNER if fold else {}
     lines = []
     for k in range(ng):
-        grp = stats[k * gs:(k + 1) * gs]
+        grp = [stats[i] for i in groups[k]]
         used = sorted(set(v for terms in grp for v, _ in terms))
         lines.append(f"__device__ __forceinline__ void hap_acc_P{P}_g{k}(const HapPop* p, "
                      "double* acc, double wt) {")
@@ -340,6 +394,9 @@ def main():
                  "double wt);")
     parts.append("// exponent e of the 2^-e the fused kernel applies to statistic q of group G")
     parts.append("template <int P, int G> __device__ int hap_scale_group(int q);")
+    parts.append("// statistics in pass G and the output column of its q-th statistic")
+    parts.append("template <int P, int G> __device__ constexpr int hap_group_size();")
+    parts.append("template <int P, int G> __device__ int hap_stat_col(int q);")
     for P in range(1, 5):
         parts.append(emit_set("hap", "HapPop", P, hap_key))
         parts.append(emit_acc_set(P))
@@ -349,6 +406,8 @@ def main():
              "#pragma once",
              '#include "geno_estimators.h"', "",
              "template <int P, int G> __device__ void geno_stats_group(const GenoPop* p, double* out);",
+             "template <int P, int G> __device__ constexpr int geno_group_size();",
+             "template <int P, int G> __device__ int geno_stat_col(int q);",
              ""]
     for P in range(1, 5):
         parts.append(emit_set("geno", "GenoPop", P, geno_key))

@@ -234,7 +234,7 @@ def test_generated_accumulation_reproduces_reference_stats(P):
         return ref_call(fn, args) / r
 
     ref = o.multi_pop_stats_hap(counts, [None] * P)
-    got, mag = [], []
+    got, mag = np.full(ref.shape, np.nan), np.full(ref.shape, np.nan)
     for k in range(int(re.search(rf"#define HAP_NGROUPS_P{P} (\d+)", inc).group(1))):
         body = re.search(rf"void hap_acc_P{P}_g{k}\(.*?\n(.*?)\n\}}", inc, re.S).group(1)
         env = {}
@@ -258,12 +258,17 @@ def test_generated_accumulation_reproduces_reference_stats(P):
         words = re.search(rf"hap_scale_group<{P}, {k}>\(int q\) \{{\n  const unsigned long long w = "
                           r"(.*?);", inc).group(1)
         w = [int(x, 16) for x in re.findall(r"0x([0-9a-f]+)ull", words)]
+        # the emitted output-column table of the group (7-bit fields, 9 per word)
+        cwords = re.search(rf"hap_stat_col<{P}, {k}>\(int q\) \{{\n.*?\n"
+                           r"  const unsigned long long w = (.*?);", inc, re.S).group(1)
+        cw = [int(x, 16) for x in re.findall(r"0x([0-9a-f]+)ull", cwords)]
         for s in sorted(accs):
             e = (w[s // 16] >> (4 * (s % 16))) & 15
-            got.append(accs[s] * 2.0 ** -e)
-            mag.append(absl[s] * 2.0 ** -e)
-    got, mag = np.stack(got, axis=1), np.stack(mag, axis=1)
-    assert got.shape == ref.shape
+            c = (cw[s // 9] >> (7 * (s % 9))) & 127
+            assert np.isnan(got[0, c])           # every column written once
+            got[:, c] = accs[s] * 2.0 ** -e
+            mag[:, c] = absl[s] * 2.0 ** -e
+    assert not np.isnan(got).any()
     # relative to the terms' own magnitude where orientations cancel, with a
     # floor at 1e-4 of the statistic's largest value (the numerators here are
     # recovered from rounded reference values, which need not vanish exactly
