@@ -1,0 +1,37 @@
+// Genotype instantiations of the fused binned and window kernels: nine
+// 10-bit counts in three words per slot for populations of at most 1,023
+// individuals, 16-bit counts in five words above.
+#include "fused_kernels.h"
+
+namespace pgld {
+
+int launch_binned_geno(const PanelLayout& L, const pgld_binned_args* args, hipStream_t s) {
+  if (args->with_r2)  // one population
+    return L.pop_size[0] <= 1023 ? launch_fused<GenoTraits<1, 3, true>>(L, args, s)
+                                 : launch_fused<GenoTraits<1, 5, true>>(L, args, s);
+  bool narrow = true;
+  for (int p = 0; p < L.n_pops; ++p) narrow = narrow && L.pop_size[p] <= 1023;
+  switch (L.n_pops) {
+    case 1:
+      return narrow ? launch_fused<GenoTraits<1, 3>>(L, args, s)
+                    : launch_fused<GenoTraits<1, 5>>(L, args, s);
+    case 2:
+      return narrow ? launch_fused<GenoTraits<2, 3>>(L, args, s)
+                    : launch_fused<GenoTraits<2, 5>>(L, args, s);
+    case 3:
+      return narrow ? launch_fused<GenoTraits<3, 3>>(L, args, s)
+                    : launch_fused<GenoTraits<3, 5>>(L, args, s);
+    case 4:
+      return narrow ? launch_fused<GenoTraits<4, 3>>(L, args, s)
+                    : launch_fused<GenoTraits<4, 5>>(L, args, s);
+    default: break;
+  }
+  return fail(PGLD_EINVAL, "1-4 populations supported");
+}
+
+int launch_windows_geno(const PanelLayout& L, const pgld_window_args* args, hipStream_t s) {
+  return L.pop_size[0] <= 1023 ? launch_windows<GenoTraits<1, 3, true>>(L, args, s)
+                               : launch_windows<GenoTraits<1, 5, true>>(L, args, s);
+}
+
+}  // namespace pgld

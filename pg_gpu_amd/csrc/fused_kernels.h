@@ -1,0 +1,499 @@
+// Kernel templates and launchers of the fused binned LD kernel, its
+// counting-stage twin and the ragged-window kernel, instantiated per traits
+// class in fused_hap.hip and fused_geno.hip.
+#pragma once
+#include "fused_tiles.h"
+
+namespace pgld {
+
+// GTAB: the reciprocal table is read from global memory (a.rtab, indexed by
+// n) instead of LDS, for populations whose range of valid-sample counts
+// makes the LDS table too large; a.rt_len is then 0 and rt_lo / rt_off 0.
+template <class T, bool GTAB = false>
+__global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
+  constexpr int NS = T::NSO, GS = T::GS;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  double* sedges = reinterpret_cast<double*>(smem);
+  double* stab = sedges + PGLD_MAX_BINS + 8;
+  const double* srt = GTAB ? a.rtab : stab;
+  const int nb = a.n_bins;
+  double* sstage = stab + 4 * a.rt_len;                      // [4 waves][nb][GS]
+  unsigned long long* slimb =
+      reinterpret_cast<unsigned long long*>(sstage + 4 * nb * GS);  // [nb][NS+1][3]
+  int* scnt = reinterpret_cast<int*>(slimb + nb * (NS + 1) * 3);    // [4 waves][nb]
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // this wave's operand ring (16-byte aligned, after the tables)
+  v4i* ring = nullptr;
+  if (T::RING > 0 && a.use_ring) {
+    const uintptr_t end = reinterpret_cast<uintptr_t>(scnt + 4 * nb);
+    ring = reinterpret_cast<v4i*>((end + 15) & ~uintptr_t(15)) + wave * (T::RING_BYTES / 16);
+  }
+  for (int e = tid; e <= nb; e += blockDim.x) sedges[e] = a.edges[e];
+  if (!GTAB) {
+    for (int p = 0; p < T::NP; ++p) {
+      const int len = a.K[p] - a.rt_lo[p] + 1;
+      for (int e = tid; e < 4 * len; e += blockDim.x)
+        stab[4 * a.rt_off[p] + e] = a.rtab[4 * a.rt_lo[p] + e];
+    }
+  }
+  for (int e = tid; e < 4 * nb * GS; e += blockDim.x) sstage[e] = 0.0;
+  for (int e = tid; e < nb * (NS + 1) * 3; e += blockDim.x) slimb[e] = 0ull;
+  for (int e = tid; e < 4 * nb; e += blockDim.x) scnt[e] = 0;
+  __syncthreads();
+  TileCtx t;
+  t.sedges = sedges;
+  t.srt = srt;
+  t.stg = sstage + wave * nb * GS;
+  t.scw = scnt + wave * nb;
+  t.slimb = slimb;
+  t.lane = lane;
+
+  // XCD-aware split: workgroups are dealt round-robin to the 8 XCDs, so each
+  // XCD gets a contiguous stretch of the tile list (ordered in L2-sized
+  // squares of tiles); the waves of one XCD then share operand panels in
+  // that XCD's L2.  With a scheduler buffer, after its first tile a wave
+  // takes the next unclaimed tile of its XCD's stretch from an atomic
+  // counter, so the waves of an XCD stay within one square of tiles
+  // (a static stride lets fast and slow waves drift apart and the panels in
+  // flight outgrow the L2).  The bin sums do not depend on the assignment.
+  const int nx = 8;
+  const int xcd = blockIdx.x % nx;
+  const int waves_per_xcd = (gridDim.x / nx) * 4;
+  const int64_t seg_lo = a.n_tiles * xcd / nx, seg_hi = a.n_tiles * (xcd + 1) / nx;
+  uint32_t* ctr = a.sched ? a.sched + xcd * 32 : nullptr;
+
+  // The next tile is claimed at the start of the current one and its list
+  // entry loaded after the current counts: neither round trip is waited for
+  // at a tile start.
+  int64_t local = (blockIdx.x / nx) * 4 + wave;
+  int32_t nrb = 0, ncb = 0;
+  if (seg_lo + local < seg_hi) {
+    nrb = a.tiles[2 * (seg_lo + local)];
+    ncb = a.tiles[2 * (seg_lo + local) + 1];
+  }
+  while (seg_lo + local < seg_hi) {
+    const int64_t rb = __builtin_amdgcn_readfirstlane(nrb), cb = __builtin_amdgcn_readfirstlane(ncb);
+    uint32_t claim = 0;
+    if (ctr && lane == 0) claim = atomicAdd(ctr, 1u);  // the next tile
+    // corner coordinates for the classification, loaded before the counts
+    const int64_t i0 = rb * 32, j0 = cb * 32;
+    const int64_t i1 = min(i0 + 31, a.V - 1), j1 = min(j0 + 31, a.V - 1);
+    const double d_min = coord_diff(a, i1, j0), d_max = coord_diff(a, i0, j1);
+    const bool in_cut = a.pos[j1] <= a.pos[i0] + a.max_dist;
+    typename T::Counts cts;
+    T::counts(a, rb, cb, lane, cts, ring);
+    const int64_t next = ctr ? waves_per_xcd + (int64_t)__builtin_amdgcn_readfirstlane(__shfl(claim, 0, 64))
+                             : local + waves_per_xcd;
+    if (seg_lo + next < seg_hi) {
+      nrb = a.tiles[2 * (seg_lo + next)];
+      ncb = a.tiles[2 * (seg_lo + next) + 1];
+    }
+
+    // ---- tile classification (wave-uniform).  Coordinates are
+    // non-decreasing along the sites, so the corner pairs (i1, j0) and
+    // (i0, j1) bound the bins of every pair in the tile.
+    t.i0 = i0;
+    t.j0 = j0;
+    t.bmin = pgld_bin_scan(d_min, sedges, nb, 0, nb - 1);
+    t.bmax = pgld_bin_scan(d_max, sedges, nb, t.bmin < 0 ? 0 : t.bmin, nb - 1);
+    t.fast = (cb > rb) && (j1 == j0 + 31) && in_cut &&
+             t.bmin == t.bmax && t.bmin >= 0 && t.bmin < nb;
+    t.blo = t.bmin < 0 ? 0 : t.bmin;
+    t.dual = !t.fast && t.blo < nb && (t.bmax >= nb ? nb - 1 : t.bmax) - t.blo <= 1;
+    tile_groups<T, 0>(a, t, cts);
+    local = next;
+  }
+  // nothing may stay staged
+  stage_drain<T, 0, NS, T::template gsz<0>()>(t.stg, t.scw, slimb, lane, 0, nb - 1);
+  __syncthreads();
+  flush_global<NS>(slimb, a.out, nb);
+}
+
+static int g_num_cus = 0;
+
+static int num_cus() {
+  if (g_num_cus == 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+      g_num_cus = v;
+    if (g_num_cus <= 0) g_num_cus = 256;
+  }
+  return g_num_cus;
+}
+
+template <class T>
+static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStream_t s) {
+  constexpr int NS = T::NSO;
+  constexpr int P = T::NP;
+  FusedParams a;
+  a.panels = g->d_panels;
+  a.site_stats = g->d_site_stats;
+  a.pos = g->d_pos;
+  a.coord = g->d_coord ? g->d_coord : g->d_pos;
+  a.coord_is_pos = g->coord_is_pos || g->d_coord == nullptr || g->d_coord == g->d_pos;
+  a.max_dist = g->max_dist;
+  a.edges = g->d_edges;
+  a.n_bins = g->n_bins;
+  a.rtab = g->d_rtab;
+  a.tiles = g->d_tiles;
+  a.n_tiles = g->n_tiles;
+  a.out = reinterpret_cast<unsigned long long*>(g->d_out);
+  a.sched = g->d_sched;
+  a.V = L.n_sites;
+  a.block_bytes = L.block_bytes;
+  int off = 0;
+  for (int p = 0; p < PGLD_MAX_POPS; ++p) {
+    a.pop_off[p] = L.pop_off[p];
+    a.ksteps[p] = L.ksteps[p];
+    a.missing[p] = L.missing[p];
+    a.K[p] = L.pop_size[p];
+    int lo = 0;
+    if (p < P) {
+      lo = L.missing[p] ? g->n_lo[p] : L.pop_size[p];
+      if (lo < 0 || lo > L.pop_size[p]) return fail(PGLD_EINVAL, "n_lo out of range");
+    }
+    a.rt_lo[p] = lo;
+    a.rt_off[p] = off;
+    if (p < P) off += L.pop_size[p] - lo + 1;
+  }
+  a.rt_len = off;
+  constexpr size_t kLds = 160 * 1024;
+  size_t fixed = (PGLD_MAX_BINS + 8) * sizeof(double) + (size_t)off * 4 * sizeof(double);
+  const size_t per_bin = (size_t)4 * T::GS * sizeof(double) +
+                         (size_t)(NS + 1) * 3 * sizeof(unsigned long long) + 4 * sizeof(int);
+  // Populations whose valid-sample counts span too wide a range for the LDS
+  // table (large populations with heavily missing sites) read the global
+  // table instead: every entry of a.rtab, indexed by n directly.
+  const bool gtab = fixed + per_bin > kLds;
+  if (gtab) {
+    for (int p = 0; p < PGLD_MAX_POPS; ++p) a.rt_lo[p] = a.rt_off[p] = 0;
+    a.rt_len = 0;
+    fixed = (PGLD_MAX_BINS + 8) * sizeof(double);
+  }
+  // Bins whose per-workgroup limbs do not fit in LDS together are covered by
+  // consecutive launches over ranges of bins (each re-counts the band; pairs
+  // outside a launch's edges are skipped by its binning).  Those launches use
+  // the static tile schedule: the caller's scheduler words serve one launch.
+  const int bins_per = (int)std::min<size_t>((kLds - fixed) / per_bin, (size_t)g->n_bins);
+  const int n_launch = (g->n_bins + bins_per - 1) / bins_per;
+  int grid = g->grid > 0 ? g->grid : T::MINB * num_cus();
+  grid = ((grid + 7) / 8) * 8;
+  const int64_t need = ((g->n_tiles + 31) / 32) * 8;  // at least 4 tiles per wave
+  if (grid > need) grid = (int)need;
+  if (grid < 8) grid = 8;
+  for (int b0 = 0; b0 < g->n_bins; b0 += bins_per) {
+    FusedParams c = a;
+    c.n_bins = std::min(bins_per, g->n_bins - b0);
+    c.edges = g->d_edges + b0;
+    c.out = a.out + (int64_t)b0 * (NS + 1) * 3;
+    if (n_launch > 1) c.sched = nullptr;
+    size_t lds = fixed + (size_t)c.n_bins * per_bin;
+    // the operand ring only where the tables leave room for it
+    const size_t ring_bytes = T::RING > 0 ? 16 + (size_t)4 * T::RING_BYTES : 0;
+    c.use_ring = ring_bytes > 0 && lds + ring_bytes <= kLds;
+    if (c.use_ring) lds += ring_bytes;
+    const void* fn = gtab ? reinterpret_cast<const void*>(fused_kernel<T, true>)
+                          : reinterpret_cast<const void*>(fused_kernel<T, false>);
+    PGLD_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (gtab)
+      hipLaunchKernelGGL((fused_kernel<T, true>), dim3(grid), dim3(256), lds, s, c);
+    else
+      hipLaunchKernelGGL((fused_kernel<T, false>), dim3(grid), dim3(256), lds, s, c);
+    PGLD_LAUNCH_CHECK();
+  }
+  return PGLD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The counting stage by itself: the same traits, tiles and schedule as
+// fused_kernel, with the epilogue replaced by a checksum of the tile's count
+// words -- per population the sums over all band tiles and slots of the two
+// 16-bit halves of both words (haplotype traits: n11, pA, pB, n with
+// marginal slot words; n11, n10, n01, n00 with class counts).  It times the
+// MFMA counting stage alone and pins its counts at full size (each sum is a
+// sum of dot products of per-block sample tallies).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256, T::MINB) count_band_kernel(FusedParams a,
+                                                                  unsigned long long* sums) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  v4i* ring = nullptr;
+  if (T::RING > 0 && a.use_ring) ring = reinterpret_cast<v4i*>(smem) + wave * (T::RING_BYTES / 16);
+  constexpr int P = T::NP;
+  unsigned long long acc[P][4];
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[p][k] = 0ull;
+  const int nx = 8;
+  const int xcd = blockIdx.x % nx;
+  const int waves_per_xcd = (gridDim.x / nx) * 4;
+  const int64_t seg_lo = a.n_tiles * xcd / nx, seg_hi = a.n_tiles * (xcd + 1) / nx;
+  uint32_t* ctr = a.sched ? a.sched + xcd * 32 : nullptr;
+  int64_t local = (blockIdx.x / nx) * 4 + wave;
+  int32_t nrb = 0, ncb = 0;
+  if (seg_lo + local < seg_hi) {
+    nrb = a.tiles[2 * (seg_lo + local)];
+    ncb = a.tiles[2 * (seg_lo + local) + 1];
+  }
+  while (seg_lo + local < seg_hi) {
+    const int64_t rb = __builtin_amdgcn_readfirstlane(nrb), cb = __builtin_amdgcn_readfirstlane(ncb);
+    uint32_t claim = 0;
+    if (ctr && lane == 0) claim = atomicAdd(ctr, 1u);
+    typename T::Counts cts;
+    T::counts(a, rb, cb, lane, cts, ring);
+    const int64_t next = ctr ? waves_per_xcd + (int64_t)__builtin_amdgcn_readfirstlane(__shfl(claim, 0, 64))
+                             : local + waves_per_xcd;
+    if (seg_lo + next < seg_hi) {
+      nrb = a.tiles[2 * (seg_lo + next)];
+      ncb = a.tiles[2 * (seg_lo + next) + 1];
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      if (a.K[p] < 4096) {   // 16 slots of 12-bit fields: the packed sums cannot carry
+        uint32_t s0 = 0, s1 = 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) { s0 += cts.w0[p][r]; s1 += cts.w1[p][r]; }
+        acc[p][0] += s0 & 0xffffu; acc[p][1] += s0 >> 16;
+        acc[p][2] += s1 & 0xffffu; acc[p][3] += s1 >> 16;
+      } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          acc[p][0] += cts.w0[p][r] & 0xffffu; acc[p][1] += cts.w0[p][r] >> 16;
+          acc[p][2] += cts.w1[p][r] & 0xffffu; acc[p][3] += cts.w1[p][r] >> 16;
+        }
+      }
+    }
+    local = next;
+  }
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      unsigned long long v = acc[p][k];
+      for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0 && v) atomicAdd(sums + 4 * p + k, v);
+    }
+}
+
+template <class T>
+static int launch_count_band(const PanelLayout& L, const pgld_binned_args* g,
+                             unsigned long long* sums, hipStream_t s) {
+  FusedParams a = FusedParams{};
+  a.panels = g->d_panels;
+  a.site_stats = g->d_site_stats;
+  a.tiles = g->d_tiles;
+  a.n_tiles = g->n_tiles;
+  a.sched = g->d_sched;
+  a.V = L.n_sites;
+  a.block_bytes = L.block_bytes;
+  for (int p = 0; p < PGLD_MAX_POPS; ++p) {
+    a.pop_off[p] = L.pop_off[p];
+    a.ksteps[p] = L.ksteps[p];
+    a.missing[p] = L.missing[p];
+    a.K[p] = L.pop_size[p];
+  }
+  const size_t lds = T::RING > 0 ? (size_t)4 * T::RING_BYTES : 0;
+  a.use_ring = T::RING > 0;
+  if (lds) {
+    PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(count_band_kernel<T>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
+  int grid = g->grid > 0 ? g->grid : T::MINB * num_cus();
+  grid = ((grid + 7) / 8) * 8;
+  const int64_t need = ((g->n_tiles + 31) / 32) * 8;
+  if (grid > need) grid = (int)need;
+  if (grid < 8) grid = 8;
+  hipLaunchKernelGGL(count_band_kernel<T>, dim3(grid), dim3(256), lds, s, a, sums);
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ragged windows: every i < j pair inside each window [lo_w, hi_w) of sites,
+// summed per window.  The work list holds (window, row block, column block)
+// triples covering each window's upper triangle; a tile that straddles a
+// window edge appears once per window it touches, masked to that window.
+// ---------------------------------------------------------------------------
+struct WindowParams {
+  FusedParams f;           // panels, site stats, reciprocal table, V, layout
+  const int32_t* work;     // [n_work][3]
+  int64_t n_work;
+  const int64_t* win;      // [n_windows][2]
+  unsigned long long* out; // [n_windows][NSO + 1][3]
+};
+
+// One work item's sums over its pairs inside the window, reduced across the
+// wave and added as fixed-point limbs to the lane's running totals: lane l <
+// NV holds statistic q = bitreverse(l) (q < GS) or, at q == GS, the pair
+// count (in L0).  Integer limbs sum exactly, so the totals do not depend on
+// how items are grouped.
+template <class T>
+__device__ __forceinline__ void window_item(const WindowParams& w, const double* srt,
+                                            const typename T::Counts& cts, int lane, bool full,
+                                            int64_t i0, int64_t j0, int64_t lo, int64_t hi,
+                                            long long& L0, long long& L1, long long& L2) {
+  constexpr int GS = T::template gsz<0>();
+  double acc[GS];
+#pragma unroll
+  for (int q = 0; q < GS; ++q) acc[q] = 0.0;
+  int count = 0;
+  const int64_t j = j0 + (lane & 31);
+  // row i = ib + o with o = 8 (r >> 2) + (r & 3) wave-uniform: lo <= i < j
+  // becomes lo_o <= o < j_o on 32-bit bounds clamped to [-1, 32]
+  const int64_t ib = i0 + 4 * (lane >> 5);
+  const int lo_o = (int)(lo - ib < -1 ? -1 : lo - ib > 32 ? 32 : lo - ib);
+  const int j_o = (int)(j - ib < -1 ? -1 : j - ib > 32 ? 32 : j - ib);
+  const bool jin = j < hi;
+  // two slots per iteration: two pairs' FP64 chains interleave
+#pragma unroll 2
+  for (int r = 0; r < 16; ++r) {
+    const int o = 8 * (r >> 2) + (r & 3);
+    // every slot's statistics are finite (padded sites have zero counts), so
+    // slots outside the window take weight 0 instead of a divergent branch
+    const bool in = full || (o >= lo_o && o < j_o && jin);
+    T::template accum<0>(w.f, srt, cts, r, acc, in ? 1.0 : 0.0);
+    count += in;
+  }
+  // butterfly reduce-scatter (see reduce_to_limbs); the pair count rides in
+  // the first padding slot (an exact small integer)
+  constexpr int NV = GS <= 2 ? 2 : GS <= 4 ? 4 : GS <= 8 ? 8 : GS <= 16 ? 16 : GS <= 32 ? 32 : 64;
+  constexpr int LOG = NV == 2 ? 1 : NV == 4 ? 2 : NV == 8 ? 3 : NV == 16 ? 4 : NV == 32 ? 5 : 6;
+  static_assert(GS < NV, "count slot");
+  double v[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = q < GS ? acc[q] : (q == GS ? (double)count : 0.0);
+  const double tot = pgld_wave_sums<NV>(v, lane);
+  const int q = pgld_bitrev(lane & (NV - 1), LOG);
+  if (lane < NV && q < GS) {
+    long long x2, x1, x0;
+    pgld_to_limbs(tot, x2, x1, x0, pgld_limb_scale(T::template scale_exp<0>(q)));
+    L0 += x0;
+    L1 += x1;
+    L2 += x2;
+  } else if (lane < NV && q == GS) {
+    L0 += (long long)tot;
+  }
+}
+
+// Adds a lane's running totals of window `win` to the output.
+template <class T>
+__device__ __forceinline__ void window_flush(const WindowParams& w, int64_t win, int lane,
+                                             long long L0, long long L1, long long L2) {
+  constexpr int NS = T::NSO;
+  constexpr int GS = T::template gsz<0>();
+  constexpr int NV = GS <= 2 ? 2 : GS <= 4 ? 4 : GS <= 8 ? 8 : GS <= 16 ? 16 : GS <= 32 ? 32 : 64;
+  constexpr int LOG = NV == 2 ? 1 : NV == 4 ? 2 : NV == 8 ? 3 : NV == 16 ? 4 : NV == 32 ? 5 : 6;
+  const int q = pgld_bitrev(lane & (NV - 1), LOG);
+  unsigned long long* dst_w = w.out + win * (NS + 1) * 3;
+  if (lane < NV && q < GS) {
+    unsigned long long* dst = dst_w + (int64_t)T::template col<0>(q) * 3;
+    if (L0) atomicAdd(dst + 0, (unsigned long long)L0);
+    if (L1) atomicAdd(dst + 1, (unsigned long long)L1);
+    if (L2) atomicAdd(dst + 2, (unsigned long long)L2);
+  } else if (lane < NV && q == GS) {
+    if (L0) atomicAdd(dst_w + (int64_t)NS * 3, (unsigned long long)L0);
+  }
+}
+
+// Each wave takes a contiguous chunk of the work list (chunks of one XCD
+// adjacent, so its waves share windows' panels in that XCD's L2) and keeps
+// its window's totals in registers while consecutive items stay in that
+// window: global atomics once per (wave, window) instead of per item.
+template <class T, bool GTAB = false>
+__global__ void __launch_bounds__(256, T::MINB) window_kernel(WindowParams w) {
+  static_assert(T::NGT == 1, "window sums use one statistic pass");
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  double* stab = reinterpret_cast<double*>(smem);
+  const double* srt = GTAB ? w.f.rtab : stab;
+  const FusedParams& a = w.f;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if (!GTAB) {
+    for (int p = 0; p < T::NP; ++p) {
+      const int len = a.K[p] - a.rt_lo[p] + 1;
+      for (int e = tid; e < 4 * len; e += blockDim.x)
+        stab[4 * a.rt_off[p] + e] = a.rtab[4 * a.rt_lo[p] + e];
+    }
+    __syncthreads();
+  }
+  const int nx = 8;
+  const int64_t rank = ((int64_t)(blockIdx.x % nx) * (gridDim.x / nx) + blockIdx.x / nx) * 4 + wave;
+  const int64_t n_waves = (int64_t)gridDim.x * 4;
+  const int64_t chunk = (w.n_work + n_waves - 1) / n_waves;
+  const int64_t k_end = min(rank * chunk + chunk, w.n_work);
+  int64_t cur = -1;
+  long long L0 = 0, L1 = 0, L2 = 0;
+  for (int64_t k = rank * chunk; k < k_end; ++k) {
+    const int64_t win = __builtin_amdgcn_readfirstlane(w.work[3 * k]);
+    const int64_t rb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 1]);
+    const int64_t cb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 2]);
+    if (win != cur) {
+      if (cur >= 0) window_flush<T>(w, cur, lane, L0, L1, L2);
+      cur = win;
+      L0 = L1 = L2 = 0;
+    }
+    const int64_t lo = w.win[2 * win], hi = w.win[2 * win + 1];
+    typename T::Counts cts;
+    T::counts(a, rb, cb, lane, cts, nullptr);  // one population: no LDS ring
+    const int64_t i0 = rb * 32, j0 = cb * 32;
+    const bool full = cb > rb && i0 >= lo && j0 + 32 <= hi;
+    window_item<T>(w, srt, cts, lane, full, i0, j0, lo, hi, L0, L1, L2);
+  }
+  if (cur >= 0) window_flush<T>(w, cur, lane, L0, L1, L2);
+}
+
+template <class T>
+static int launch_windows(const PanelLayout& L, const pgld_window_args* g, hipStream_t s) {
+  WindowParams w;
+  FusedParams& a = w.f;
+  a = FusedParams{};
+  a.panels = g->d_panels;
+  a.site_stats = g->d_site_stats;
+  a.rtab = g->d_rtab;
+  a.V = L.n_sites;
+  a.block_bytes = L.block_bytes;
+  a.pop_off[0] = L.pop_off[0];
+  a.ksteps[0] = L.ksteps[0];
+  a.missing[0] = L.missing[0];
+  a.K[0] = L.pop_size[0];
+  const int lo = L.missing[0] ? g->n_lo[0] : L.pop_size[0];
+  if (lo < 0 || lo > L.pop_size[0]) return fail(PGLD_EINVAL, "n_lo out of range");
+  a.rt_lo[0] = lo;
+  a.rt_off[0] = 0;
+  a.rt_len = L.pop_size[0] - lo + 1;
+  a.sched = g->d_sched;
+  w.work = g->d_work;
+  w.n_work = g->n_work;
+  w.win = g->d_windows;
+  w.out = reinterpret_cast<unsigned long long*>(g->d_out);
+  size_t lds = (size_t)a.rt_len * 4 * sizeof(double);
+  // a table wider than the LDS (a large population with heavily missing
+  // sites) is read from global memory, indexed by n
+  const bool gtab = lds > 160 * 1024;
+  if (gtab) {
+    a.rt_lo[0] = 0;
+    a.rt_len = 0;
+    lds = 0;
+  } else {
+    PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(window_kernel<T, false>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
+  int grid = g->grid > 0 ? g->grid : T::MINB * num_cus();
+  grid = ((grid + 7) / 8) * 8;
+  const int64_t need = ((g->n_work + 31) / 32) * 8;
+  if (grid > need) grid = (int)need;
+  if (grid < 8) grid = 8;
+  if (gtab)
+    hipLaunchKernelGGL((window_kernel<T, true>), dim3(grid), dim3(256), 0, s, w);
+  else
+    hipLaunchKernelGGL((window_kernel<T, false>), dim3(grid), dim3(256), lds, s, w);
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}
+
+}  // namespace pgld

@@ -36,4 +36,13 @@ int make_layout(const pgld_panel_desc* d, PanelLayout* L);
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Launchers of the fused kernels per mode (fused_hap.hip, fused_geno.hip);
+// the arguments are validated by the entry points in fused.hip.
+int launch_binned_hap(const PanelLayout& L, const pgld_binned_args* args, hipStream_t s);
+int launch_binned_geno(const PanelLayout& L, const pgld_binned_args* args, hipStream_t s);
+int launch_count_band_hap(const PanelLayout& L, const pgld_binned_args* args,
+                          unsigned long long* sums, hipStream_t s);
+int launch_windows_hap(const PanelLayout& L, const pgld_window_args* args, hipStream_t s);
+int launch_windows_geno(const PanelLayout& L, const pgld_window_args* args, hipStream_t s);
+
 }  // namespace pgld
