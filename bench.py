@@ -352,11 +352,18 @@ def main():
     windows = w.windows is not None
     r2 = w.name in R2_WORKLOADS
     if windows:
-        plan = WindowPlan(pan, w.windows)
+        launch = {"tile": 32, "tuned": False}
+        if not args.no_tune:
+            from pg_gpu_amd import autotune
+
+            best, timings = autotune.tune_windows(pan, w.windows)
+            launch = dict(best, tuned=True, candidates=len(timings),
+                          candidate_ms={f"tile{c['tile']}": round(t, 4) for c, t in timings})
+            log(f"autotune: {best} over {len(timings)} tile shapes")
+        plan = WindowPlan(pan, w.windows, tile=launch["tile"])
         tiles = None
         run = plan.run
         n_bins = None
-        launch = {"tuned": False}
     else:
         launch = {"square": None, "dynamic": True, "grid": 0, "tuned": False}
         if not args.no_tune:
@@ -410,6 +417,26 @@ def main():
     stop.record(stream)
     torch.cuda.synchronize()
     total_ms = start.elapsed_time(stop)
+    graph = None
+    if windows:
+        # the whole job as one captured HIP graph (tallies, panels, clear, the
+        # persistent launch over every window's work items); its replays are
+        # the timed steps
+        g = plan.capture(repack=True)
+        for _ in range(max(args.warmup, 1)):
+            g.replay()
+        torch.cuda.synchronize()
+        start.record(stream)
+        for _ in range(args.steps):
+            g.replay()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        eager_ms = total_ms / args.steps
+        total_ms = start.elapsed_time(stop)
+        gres = g.replay().finish()
+        assert np.array_equal(gres.dd, results[-1].finish().dd)
+        graph = {"ms_per_replay": total_ms / args.steps, "eager_ms_per_step": eager_ms}
+        log(f"graph replays: {graph['ms_per_replay']:.3f} ms (eager step {eager_ms:.3f} ms)")
     kern_ms = [a.elapsed_time(b) for a, b in evs]
     pack_ms = float(np.mean([a.elapsed_time(b) for a, b in pevs]))
     scan_ms = float(np.mean([a.elapsed_time(b) for a, b in sevs]))
@@ -448,7 +475,7 @@ def main():
     if cpu:
         X = cpu["X_exact"]
         if windows:
-            rg = WindowPlan(pan, w.windows[:X]).run().finish()
+            rg = WindowPlan(pan, w.windows[:X], tile=launch["tile"]).run().finish()
             got = np.stack([rg.dd, rg.dz, rg.pi2], axis=1)
             ref = cpu["sums"][:, :3]
             scale = np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max(axis=0))
@@ -565,6 +592,7 @@ def main():
                  "frac_of_hbm_peak": scan_bytes / (scan_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
         "counting_stage": counting,
         "end_to_end": e2e,
+        "graph": graph,
         "plain_fp64": plain,
         "speedup_vs_plain": (plain["ms"] / k_ms) if plain else None,
         "precision": prec,

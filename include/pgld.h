@@ -182,13 +182,16 @@ typedef struct {
   const int64_t* d_windows;    /* [n_windows][2] site ranges [lo, hi), lo <= hi */
   int64_t n_windows;
   const int32_t* d_work;       /* fused: [n_work][3] (window, row block, col block) with
-                                  32-site blocks covering each window's upper triangle */
+                                  `tile`-site blocks covering each window's upper triangle */
   int64_t n_work;
   long long* d_out;            /* [n_windows][n_stats + 3][3] limbs, zeroed */
   int32_t grid;                /* workgroups (0 = auto) */
   int32_t n_lo[PGLD_MAX_POPS]; /* as in pgld_binned_args */
   uint32_t* d_sched;           /* not used: the window kernel splits the work list into
                                   contiguous per-wave chunks */
+  int32_t tile;                /* sites per side of the work items' blocks: 32 (or 0) for
+                                  32 x 32 tiles, 16 for 16 x 16 tiles (haplotypes; the
+                                  v_mfma_scale_f32_16x16x128_f8f6f4 path) */
 } pgld_window_args;
 
 /* Per-window sums over every i < j pair of sites inside each window (no

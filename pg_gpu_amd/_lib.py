@@ -73,6 +73,7 @@ class WindowArgs(ctypes.Structure):
         ("grid", c_int32),
         ("n_lo", c_int32 * MAX_POPS),
         ("d_sched", c_void_p),
+        ("tile", c_int32),
     ]
 
 

@@ -1,4 +1,4 @@
-"""Launch-shape autotuner of the fused binning kernel.
+"""Launch-shape autotuner of the fused binning kernel and the window kernel.
 
 The fused kernel's tile is fixed by the MFMA (32 x 32 sites per wave); what
 varies with the data is how the band's tiles are ordered and dealt out:
@@ -78,3 +78,50 @@ def tune_binned(pan, positions, bins, max_dist=None, coords=None, r2=False, reps
 
 def clear_cache():
     _cache.clear()
+    _win_cache.clear()
+
+
+# --------------------------------------------------------------------------- windows
+
+_win_cache = {}
+
+
+def window_job_key(pan, windows):
+    """Shape of a window job: mode, population size, missing data and the
+    quartiles of the window lengths (log2)."""
+    w = np.asarray(windows, np.int64).reshape(-1, 2)
+    k = np.maximum(w[:, 1] - w[:, 0], 1)
+    q = tuple(int(np.log2(x)) for x in np.percentile(k, [25, 50, 75])) if len(k) else ()
+    return (int(pan.mode), tuple(pan.pop_sizes), tuple(pan.has_missing), q)
+
+
+def tune_windows(pan, windows, reps=3, tiles=(32, 16), use_cache=True):
+    """-> (best dict(tile), [(config, ms)...]): times the window kernel with
+    32 x 32 and (haplotypes) 16 x 16 MFMA tiles on the job, cached per
+    ``window_job_key``."""
+    from .windows import WindowPlan
+
+    key = window_job_key(pan, windows)
+    if use_cache and key in _win_cache:
+        return dict(_win_cache[key]), []
+    torch = dv.torch()
+    stream = torch.cuda.current_stream()
+    timings = []
+    for tile in tiles:
+        if tile == 16 and pan.mode != 0:
+            continue
+        plan = WindowPlan(pan, windows, tile=tile)
+        plan.run().finish()                                   # warm-up
+        best = float("inf")
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            pend = plan.run()
+            b.record(stream)
+            b.synchronize()
+            pend.finish()
+            best = min(best, a.elapsed_time(b))
+        timings.append((dict(tile=int(tile)), best))
+    best = min(timings, key=lambda t: t[1])[0]
+    _win_cache[key] = dict(best)
+    return dict(best), timings

@@ -170,6 +170,8 @@ extern "C" int pgld_windows_fused(const pgld_panel_desc* desc, const pgld_window
   if (!args) return fail(PGLD_EINVAL, "null window args");
   if (desc->n_pops != 1) return fail(PGLD_EUNSUPPORTED, "window sums are built for one population");
   if (args->n_work < 0 || args->n_windows < 0) return fail(PGLD_EINVAL, "negative sizes");
+  if (args->tile != 0 && args->tile != 16 && args->tile != 32)
+    return fail(PGLD_EINVAL, "tile must be 16 or 32");
   if (args->n_work == 0) return PGLD_OK;
   if (!args->d_panels || !args->d_site_stats || !args->d_rtab || !args->d_work ||
       !args->d_windows || !args->d_out)

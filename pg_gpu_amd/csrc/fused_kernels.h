@@ -2,6 +2,7 @@
 // counting-stage twin and the ragged-window kernel, instantiated per traits
 // class in fused_hap.hip and fused_geno.hip.
 #pragma once
+#include <type_traits>
 #include "fused_tiles.h"
 
 namespace pgld {
@@ -447,6 +448,178 @@ __global__ void __launch_bounds__(256, T::MINB) window_kernel(WindowParams w) {
   if (cur >= 0) window_flush<T>(w, cur, lane, L0, L1, L2);
 }
 
+// ---------------------------------------------------------------------------
+// 16 x 16 site tiles for the window kernel (haplotypes, one population):
+// v_mfma_scale_f32_16x16x128_f8f6f4 reads the same packed panels.  Its
+// operand for 16 sites (the half h of a 32-site block) and 128 samples (two
+// 64-sample k-steps) gives lane l site 16 h + (l & 15) and samples
+// 128 t + 32 (l >> 4) + [0, 32): the 16 bytes that lane
+// 16 h + (l & 15) + 32 ((l >> 4) & 1) of k-step 2 t + (l >> 5) holds.  Its
+// 16 x 16 result gives lane l column l & 15 and rows 4 (l >> 4) + r, r < 4.
+// Quarter tiles cover a window's triangle with less padding where windows
+// are short; which side runs faster on a job is measured (autotune).
+// ---------------------------------------------------------------------------
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4f pgld_mfma16_fp4(v4i a, v4i b, v4f c) {
+  const v8i a8 = {a[0], a[1], a[2], a[3], 0, 0, 0, 0};
+  const v8i b8 = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
+  return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a8, b8, c, 4, 4, 0, 127, 0, 127);
+}
+
+// Lane's operand bytes of 128-sample step t for the 16 sites of half h; f
+// points at the population's feature start in the 32-site block (k-step 0);
+// k-steps past the population's last are zero.
+__device__ __forceinline__ v4i frag16(const int8_t* f, int t, int h, int ks, int lane) {
+  const int s = 2 * t + (lane >> 5);
+  const int lp = 16 * h + (lane & 15) + 32 * ((lane >> 4) & 1);
+  v4i v = {0, 0, 0, 0};
+  if (s < ks) v = *reinterpret_cast<const v4i*>(f + (int64_t)s * PGLD_FRAG_BYTES + lp * 16);
+  return v;
+}
+
+// Class counts of the population for the 16-site blocks (rb, cb), packed per
+// slot as w0 = n11 | n10 << 16, w1 = n01 | n00 << 16.
+__device__ __forceinline__ void tile16_counts(const FusedParams& a, int64_t rb, int64_t cb,
+                                              int lane, v4u& w0, v4u& w1) {
+  const int ks = a.ksteps[0];
+  const int nt = (ks + 1) / 2;
+  const int8_t* A = a.panels + (rb >> 1) * a.block_bytes + a.pop_off[0];
+  const int8_t* B = a.panels + (cb >> 1) * a.block_bytes + a.pop_off[0];
+  const int ha = (int)(rb & 1), hb = (int)(cb & 1);
+  const int64_t fs = (int64_t)ks * PGLD_FRAG_BYTES;   // feature stride
+  v4f hh = {0, 0, 0, 0};
+  if (a.missing[0]) {
+    v4f hv = hh, vh = hh, vv = hh;
+    for (int t = 0; t < nt; ++t) {
+      const v4i ah = frag16(A, t, ha, ks, lane), av = frag16(A + fs, t, ha, ks, lane);
+      const v4i bh = frag16(B, t, hb, ks, lane), bv = frag16(B + fs, t, hb, ks, lane);
+      hh = pgld_mfma16_fp4(ah, bh, hh);
+      hv = pgld_mfma16_fp4(ah, bv, hv);
+      vh = pgld_mfma16_fp4(av, bh, vh);
+      vv = pgld_mfma16_fp4(av, bv, vv);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n11 = (int)hh[r], h_v = (int)hv[r], v_h = (int)vh[r], v_v = (int)vv[r];
+      w0[r] = (uint32_t)n11 | ((uint32_t)(h_v - n11) << 16);
+      w1[r] = (uint32_t)(v_h - n11) | ((uint32_t)(v_v - h_v - v_h + n11) << 16);
+    }
+  } else {
+    for (int t = 0; t < nt; ++t)
+      hh = pgld_mfma16_fp4(frag16(A, t, ha, ks, lane), frag16(B, t, hb, ks, lane), hh);
+    const int32_t* cnt1 = a.site_stats;
+    const int64_t j = cb * 16 + (lane & 15);
+    const int kj = j < a.V ? cnt1[j] : 0;
+    const int K = a.K[0];
+    const int64_t ib = rb * 16 + 4 * (lane >> 4);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int ki = ib + r < a.V ? cnt1[ib + r] : 0;
+      const int n11 = (int)hh[r];
+      w0[r] = (uint32_t)n11 | ((uint32_t)(ki - n11) << 16);
+      w1[r] = (uint32_t)(kj - n11) | ((uint32_t)(K - ki - kj + n11) << 16);
+    }
+  }
+}
+
+// One population's haplotype statistics and r^2 per slot of a quarter tile
+// (HapTraits<1, true>'s statistic pass over 4 slots).
+struct Hap1Tile16 : HapTraits<1, true> {
+  struct Counts { v4u w0, w1; };
+  __device__ __forceinline__ static void accum(const FusedParams& a, const double* srt,
+                                               const Counts& c, int r, double* acc, double wt) {
+    const uint32_t w0 = c.w0[r], w1 = c.w1[r];
+    const int n11 = w0 & 0xffff, n10 = w0 >> 16, n01 = w1 & 0xffff, n00 = w1 >> 16;
+    HapPop pp[1];
+    pp[0] = hap_pop(n11, n10, n01, n00, 0, rt_entry(a, srt, 0, n11 + n10 + n01 + n00));
+    hap_acc_group<1, 0>(pp, acc, wt);
+    double x[2];
+    hap_r2(pp[0], x);
+    acc[GS - 2] = fma(x[0], wt, acc[GS - 2]);
+    acc[GS - 1] = fma(x[1], wt, acc[GS - 1]);
+  }
+};
+
+__device__ __forceinline__ void window_item16(const WindowParams& w, const double* srt,
+                                              const Hap1Tile16::Counts& cts, int lane, bool full,
+                                              int64_t i0, int64_t j0, int64_t lo, int64_t hi,
+                                              long long& L0, long long& L1, long long& L2) {
+  using T = Hap1Tile16;
+  constexpr int GS = T::template gsz<0>();
+  double acc[GS];
+#pragma unroll
+  for (int q = 0; q < GS; ++q) acc[q] = 0.0;
+  int count = 0;
+  const int64_t j = j0 + (lane & 15);
+  const int64_t ib = i0 + 4 * (lane >> 4);
+  const bool jin = j < hi;
+#pragma unroll 2
+  for (int r = 0; r < 4; ++r) {
+    const int64_t i = ib + r;
+    const bool in = full || (i >= lo && i < j && jin);
+    T::accum(w.f, srt, cts, r, acc, in ? 1.0 : 0.0);
+    count += in;
+  }
+  constexpr int NV = 8;
+  static_assert(GS < NV, "count slot");
+  double v[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = q < GS ? acc[q] : (q == GS ? (double)count : 0.0);
+  const double tot = pgld_wave_sums<NV>(v, lane);
+  const int q = pgld_bitrev(lane & (NV - 1), 3);
+  if (lane < NV && q < GS) {
+    long long x2, x1, x0;
+    pgld_to_limbs(tot, x2, x1, x0, pgld_limb_scale(T::template scale_exp<0>(q)));
+    L0 += x0;
+    L1 += x1;
+    L2 += x2;
+  } else if (lane < NV && q == GS) {
+    L0 += (long long)tot;
+  }
+}
+
+// The window kernel over 16-site work items (blocks of 16 sites in d_work).
+template <bool GTAB>
+__global__ void __launch_bounds__(256, 3) window16_kernel(WindowParams w) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  double* stab = reinterpret_cast<double*>(smem);
+  const double* srt = GTAB ? w.f.rtab : stab;
+  const FusedParams& a = w.f;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if (!GTAB) {
+    const int len = a.K[0] - a.rt_lo[0] + 1;
+    for (int e = tid; e < 4 * len; e += blockDim.x) stab[e] = a.rtab[4 * a.rt_lo[0] + e];
+    __syncthreads();
+  }
+  const int nx = 8;
+  const int64_t rank = ((int64_t)(blockIdx.x % nx) * (gridDim.x / nx) + blockIdx.x / nx) * 4 + wave;
+  const int64_t n_waves = (int64_t)gridDim.x * 4;
+  const int64_t chunk = (w.n_work + n_waves - 1) / n_waves;
+  const int64_t k_end = min(rank * chunk + chunk, w.n_work);
+  int64_t cur = -1;
+  long long L0 = 0, L1 = 0, L2 = 0;
+  for (int64_t k = rank * chunk; k < k_end; ++k) {
+    const int64_t win = __builtin_amdgcn_readfirstlane(w.work[3 * k]);
+    const int64_t rb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 1]);
+    const int64_t cb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 2]);
+    if (win != cur) {
+      if (cur >= 0) window_flush<Hap1Tile16>(w, cur, lane, L0, L1, L2);
+      cur = win;
+      L0 = L1 = L2 = 0;
+    }
+    const int64_t lo = w.win[2 * win], hi = w.win[2 * win + 1];
+    Hap1Tile16::Counts cts;
+    tile16_counts(a, rb, cb, lane, cts.w0, cts.w1);
+    const int64_t i0 = rb * 16, j0 = cb * 16;
+    const bool full = cb > rb && i0 >= lo && j0 + 16 <= hi;
+    window_item16(w, srt, cts, lane, full, i0, j0, lo, hi, L0, L1, L2);
+  }
+  if (cur >= 0) window_flush<Hap1Tile16>(w, cur, lane, L0, L1, L2);
+}
+
 template <class T>
 static int launch_windows(const PanelLayout& L, const pgld_window_args* g, hipStream_t s) {
   WindowParams w;
@@ -488,6 +661,21 @@ static int launch_windows(const PanelLayout& L, const pgld_window_args* g, hipSt
   const int64_t need = ((g->n_work + 31) / 32) * 8;
   if (grid > need) grid = (int)need;
   if (grid < 8) grid = 8;
+  if (g->tile == 16) {
+    if constexpr (std::is_same<T, HapTraits<1, true>>::value) {
+      if (!gtab)
+        PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(window16_kernel<false>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      if (gtab)
+        hipLaunchKernelGGL((window16_kernel<true>), dim3(grid), dim3(256), 0, s, w);
+      else
+        hipLaunchKernelGGL((window16_kernel<false>), dim3(grid), dim3(256), lds, s, w);
+      PGLD_LAUNCH_CHECK();
+      return PGLD_OK;
+    } else {
+      return fail(PGLD_EUNSUPPORTED, "16-site window tiles are built for haplotypes");
+    }
+  }
   if (gtab)
     hipLaunchKernelGGL((window_kernel<T, true>), dim3(grid), dim3(256), 0, s, w);
   else

@@ -32,15 +32,15 @@ def windows_from_positions(positions, window_len, start=0.0, end=None):
     return np.stack([idx[:-1], idx[1:]], axis=1).astype(np.int64)
 
 
-def window_work(windows):
+def window_work(windows, tile=TILE):
     """(window, row block, col block) triples covering each window's upper
-    triangle with TILE-site blocks, window by window."""
+    triangle with ``tile``-site blocks (32 or 16), window by window."""
     windows = np.asarray(windows, np.int64).reshape(-1, 2)
     parts = []
     for w, (lo, hi) in enumerate(windows):
         if hi - lo < 2:
             continue
-        b0, b1 = lo // TILE, (hi - 1) // TILE
+        b0, b1 = lo // tile, (hi - 1) // tile
         rb, cb = np.triu_indices(b1 - b0 + 1)
         parts.append(np.stack([np.full(len(rb), w), rb + b0, cb + b0], axis=1))
     if not parts:
@@ -64,9 +64,11 @@ class WindowResult:
 
 
 class WindowPlan:
-    """Device copies of the windows and work list of one job."""
+    """Device copies of the windows and work list of one job.  ``tile`` is
+    the side of the work items: 32 (32 x 32 MFMA tiles) or 16 (16 x 16,
+    haplotypes); autotune.tune_windows picks the faster for a job."""
 
-    def __init__(self, pan: LDPanels, windows):
+    def __init__(self, pan: LDPanels, windows, tile=TILE):
         torch = _torch()
         if pan.P != 1:
             raise NotImplementedError("window sums are built for one population")
@@ -76,9 +78,14 @@ class WindowPlan:
             raise ValueError("windows must be site ranges [lo, hi) inside the matrix")
         if len(win) > 1 and np.any(win[1:, 0] < win[:-1, 1]):
             raise ValueError("windows must be sorted and non-overlapping")
+        if tile not in (16, 32):
+            raise ValueError("tile must be 16 or 32")
+        if tile == 16 and pan.mode != _lib.MODE_HAP:
+            raise NotImplementedError("16-site window tiles are built for haplotypes")
         self.pan = pan
+        self.tile = int(tile)
         self.windows = win
-        self.work = window_work(win)
+        self.work = window_work(win, self.tile)
         k = win[:, 1] - win[:, 0]
         self.n_pairs = (k * (k - 1) // 2).astype(np.int64)
         self.d_win = torch.as_tensor(win.reshape(-1), device=pan.device)
@@ -102,6 +109,7 @@ class WindowPlan:
         a.d_out = _lib.ptr(out).value
         a.grid = int(grid)
         a.n_lo[0] = int(pan.n_lo[0])
+        a.tile = self.tile
         return a
 
     def run(self, kernel="fused", grid=0):
@@ -123,9 +131,10 @@ class WindowPlan:
 
 
     def capture(self, kernel="fused", grid=0, repack=False):
-        """Record the whole job (optionally re-packing the panels first) as a
-        HIP graph: one replay = clear + one persistent launch over every
-        window's work items.  The output is rewritten by each replay."""
+        """Record the whole job (optionally re-tallying and re-packing the
+        panels first) as a HIP graph: one replay = clear + one persistent
+        launch over every window's work items.  The output is rewritten by
+        each replay."""
         return WindowGraph(self, kernel, grid, repack)
 
 
@@ -151,6 +160,7 @@ class WindowGraph:
     def _body(self):
         pan = self.plan.pan
         if self.repack:
+            pan.rescan()
             pan.repack()
         self.out.zero_()
         _lib.check(self.fn(ctypes.byref(pan.desc), ctypes.byref(self.args), _lib.stream_handle()))

@@ -13,29 +13,31 @@ from pg_gpu_amd.windows import WindowPlan, windows_from_positions
 pytestmark = pytest.mark.gpu
 
 
-def run_windows(cuda, mat, windows, mode="hap", kernels=("fused", "plain")):
+def run_windows(cuda, mat, windows, mode="hap", kernels=("fused", "plain"), tiles=(32,)):
     import torch
 
-    print(f"case: windows {len(windows)} {mode} {mat.shape}", flush=True)
+    print(f"case: windows {len(windows)} {mode} {mat.shape} tiles {tiles}", flush=True)
     pan = LDPanels(torch.as_tensor(mat, device=cuda), [list(range(mat.shape[0]))], mode)
     ref, cnt = o.window_sums(mat, windows, mode=mode)
-    plan = WindowPlan(pan, windows)
-    for k in kernels:
+    plan = None
+    runs = [(k, 32) for k in kernels] + [("fused", t) for t in tiles if t != 32]
+    for k, tile in runs:
+        plan = WindowPlan(pan, windows, tile=tile)
         r = plan.run(k).finish()
         np.testing.assert_array_equal(r.n_pairs, cnt)
         np.testing.assert_array_equal(r.r2_count, np.rint(ref[:, 4]).astype(np.int64))
         np.testing.assert_allclose(r.r2_sum, ref[:, 3], rtol=2e-6, atol=1e-9)
         got = np.stack([r.dd, r.dz, r.pi2], axis=1)
         scale = np.maximum(np.abs(ref[:, :3]), 1e-3 * np.abs(ref[:, :3]).max(axis=0))
-        assert np.all(np.abs(got - ref[:, :3]) <= 1e-12 * scale + 1e-300), k
-    return plan
+        assert np.all(np.abs(got - ref[:, :3]) <= 1e-12 * scale + 1e-300), (k, tile)
+    return WindowPlan(pan, windows)
 
 
 def test_windows_c3_scaled(cuda):
     """Workload c3 shape: Gamma(0.8)-distributed SNP counts per 50 kb window."""
     w = synth.make("c3", scale=0.004)
     sel = [2, 3, 5, 6, 9, 13, 17, 18, 19]          # 20 .. 683 sites (a subset keeps the oracle fast)
-    run_windows(cuda, w.mat, w.windows[sel])
+    run_windows(cuda, w.mat, w.windows[sel], tiles=(32, 16))
 
 
 def test_windows_ragged_edges_and_empty(cuda):
@@ -45,7 +47,7 @@ def test_windows_ragged_edges_and_empty(cuda):
     hap[rng.random(hap.shape) < 0.04] = -1
     hap[:, 10] = 0
     windows = np.array([[0, 0], [0, 1], [1, 33], [33, 34], [34, 97], [97, 97], [97, 300]])
-    run_windows(cuda, hap, windows)
+    run_windows(cuda, hap, windows, tiles=(32, 16))
 
 
 def test_windows_genotypes(cuda):
@@ -76,3 +78,29 @@ def test_windows_graph_replays_match_eager(cuda):
         again = plan.run().finish()
         np.testing.assert_array_equal(again.dd, eager.dd)
         _ = (torch.arange(100, device=cuda) + rep).sum().item()
+
+
+@pytest.mark.parametrize("missing", [0.0, 0.05])
+def test_windows_16_site_tiles_odd_ksteps(cuda, missing):
+    """16 x 16 tiles with a sample count whose last 128-sample MFMA step is
+    half empty (300 haplotypes: five 64-sample k-steps) and windows starting
+    and ending inside 16-site blocks."""
+    rng = np.random.default_rng(43)
+    V = 200
+    hap = (rng.random((300, V)) < rng.beta(0.4, 0.4, V)).astype(np.int8)
+    if missing:
+        hap[rng.random(hap.shape) < missing] = -1
+    windows = np.array([[3, 9], [9, 40], [40, 41], [41, 200]])
+    run_windows(cuda, hap, windows, kernels=("fused",), tiles=(32, 16))
+
+
+def test_window_autotune_picks_a_measured_tile(cuda):
+    import torch
+
+    from pg_gpu_amd import autotune
+
+    w = synth.make("c3", scale=0.004)
+    pan = LDPanels(torch.as_tensor(w.mat, device=cuda), [list(range(w.mat.shape[0]))], "hap")
+    best, timings = autotune.tune_windows(pan, w.windows, use_cache=False)
+    assert best["tile"] in (16, 32) and len(timings) == 2
+    assert best == min(timings, key=lambda t: t[1])[0]
