@@ -352,15 +352,16 @@ def main():
     windows = w.windows is not None
     r2 = w.name in R2_WORKLOADS
     if windows:
-        launch = {"tile": 32, "tuned": False}
+        launch = {"tile": 32, "group": 0, "tuned": False}
         if not args.no_tune:
             from pg_gpu_amd import autotune
 
             best, timings = autotune.tune_windows(pan, w.windows)
             launch = dict(best, tuned=True, candidates=len(timings),
-                          candidate_ms={f"tile{c['tile']}": round(t, 4) for c, t in timings})
-            log(f"autotune: {best} over {len(timings)} tile shapes")
-        plan = WindowPlan(pan, w.windows, tile=launch["tile"])
+                          candidate_ms={f"tile{c['tile']}g{c['group']}": round(t, 4)
+                                        for c, t in timings})
+            log(f"autotune: {best} over {len(timings)} tile shapes and groups")
+        plan = WindowPlan(pan, w.windows, tile=launch["tile"], group=launch["group"])
         tiles = None
         run = plan.run
         n_bins = None
@@ -475,7 +476,8 @@ def main():
     if cpu:
         X = cpu["X_exact"]
         if windows:
-            rg = WindowPlan(pan, w.windows[:X], tile=launch["tile"]).run().finish()
+            rg = WindowPlan(pan, w.windows[:X], tile=launch["tile"],
+                            group=launch["group"]).run().finish()
             got = np.stack([rg.dd, rg.dz, rg.pi2], axis=1)
             ref = cpu["sums"][:, :3]
             scale = np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max(axis=0))

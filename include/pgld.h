@@ -125,6 +125,18 @@ typedef struct {
 /* Number of statistics produced per bin for (mode, n_pops). */
 int pgld_n_stats(int32_t mode, int32_t n_pops);
 
+/* Host-side planner (no device work, synchronous): the band tile list of a
+ * binned job.  Writes to `tiles` (int32 [n_tiles][2], host memory, `cap`
+ * entries) the 32 x 32 site tiles (r, c), c >= r, holding at least one pair
+ * i < j with pos[j] <= pos[i] + max_dist, in squares of `square` x `square`
+ * tiles (row-major squares, row-major inside), and to *n_tiles their number;
+ * with tiles == NULL only the counts are computed.  *n_pairs (optional)
+ * receives the number of such pairs.  pos: host float64, sorted.  Replaces the
+ * band enumeration of iter_pairs_within_distance (pg_gpu/ld_pipeline.py:85-88)
+ * at tile granularity, and its total pair count (:85-90). */
+int pgld_band_tiles(const double* pos, int64_t n_sites, double max_dist, int32_t square,
+                    int32_t* tiles, int64_t cap, int64_t* n_tiles, int64_t* n_pairs);
+
 /* Fused binned LD: banded MFMA counting (FP4 indicators, exact FP32 sums) +
  * estimator epilogue + bin
  * reduction in one kernel.  Replaces the chunk loop of
@@ -187,11 +199,14 @@ typedef struct {
   long long* d_out;            /* [n_windows][n_stats + 3][3] limbs, zeroed */
   int32_t grid;                /* workgroups (0 = auto) */
   int32_t n_lo[PGLD_MAX_POPS]; /* as in pgld_binned_args */
-  uint32_t* d_sched;           /* not used: the window kernel splits the work list into
-                                  contiguous per-wave chunks */
+  uint32_t* d_sched;           /* not used: the window kernel deals each XCD's eighth of
+                                  the work list to its waves in fixed groups */
   int32_t tile;                /* sites per side of the work items' blocks: 32 (or 0) for
                                   32 x 32 tiles, 16 for 16 x 16 tiles (haplotypes; the
                                   v_mfma_scale_f32_16x16x128_f8f6f4 path) */
+  int32_t group;               /* consecutive work items per wave turn, 1..4096 (0 = 16
+                                  for 32-site items, 32 for 16-site items); the result
+                                  does not depend on it */
 } pgld_window_args;
 
 /* Per-window sums over every i < j pair of sites inside each window (no

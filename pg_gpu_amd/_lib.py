@@ -74,6 +74,7 @@ class WindowArgs(ctypes.Structure):
         ("n_lo", c_int32 * MAX_POPS),
         ("d_sched", c_void_p),
         ("tile", c_int32),
+        ("group", c_int32),
     ]
 
 
@@ -98,6 +99,8 @@ _SIGS = {
     "pgld_count_block": (c_int32, [POINTER(PanelDesc), c_void_p, c_void_p, c_int32, c_int64,
                                    c_int64, c_int64, c_int64, c_void_p, c_void_p]),
     "pgld_pair_bounds": (c_int32, [c_void_p, c_int64, c_double, c_void_p, c_void_p]),
+    "pgld_band_tiles": (c_int32, [c_void_p, c_int64, c_double, c_int32, c_void_p, c_int64,
+                                  POINTER(c_int64), POINTER(c_int64)]),
     "pgld_emit_pairs": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p,
                                   c_void_p]),
     "pgld_count_pairs": (c_int32, [POINTER(PanelDesc), c_void_p, c_void_p, c_int32, c_void_p,

@@ -95,10 +95,10 @@ def window_job_key(pan, windows):
     return (int(pan.mode), tuple(pan.pop_sizes), tuple(pan.has_missing), q)
 
 
-def tune_windows(pan, windows, reps=3, tiles=(32, 16), use_cache=True):
-    """-> (best dict(tile), [(config, ms)...]): times the window kernel with
-    32 x 32 and (haplotypes) 16 x 16 MFMA tiles on the job, cached per
-    ``window_job_key``."""
+def tune_windows(pan, windows, reps=3, tiles=(32, 16), groups=(8, 16, 64), use_cache=True):
+    """-> (best dict(tile, group), [(config, ms)...]): times the window kernel
+    with 32 x 32 and (haplotypes) 16 x 16 MFMA tiles and several work-item
+    group sizes on the job, cached per ``window_job_key``."""
     from .windows import WindowPlan
 
     key = window_job_key(pan, windows)
@@ -110,18 +110,19 @@ def tune_windows(pan, windows, reps=3, tiles=(32, 16), use_cache=True):
     for tile in tiles:
         if tile == 16 and pan.mode != 0:
             continue
-        plan = WindowPlan(pan, windows, tile=tile)
-        plan.run().finish()                                   # warm-up
-        best = float("inf")
-        for _ in range(reps):
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record(stream)
-            pend = plan.run()
-            b.record(stream)
-            b.synchronize()
-            pend.finish()
-            best = min(best, a.elapsed_time(b))
-        timings.append((dict(tile=int(tile)), best))
+        for group in groups:
+            plan = WindowPlan(pan, windows, tile=tile, group=group)
+            plan.run().finish()                                   # warm-up
+            best = float("inf")
+            for _ in range(reps):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                pend = plan.run()
+                b.record(stream)
+                b.synchronize()
+                pend.finish()
+                best = min(best, a.elapsed_time(b))
+            timings.append((dict(tile=int(tile), group=int(group)), best))
     best = min(timings, key=lambda t: t[1])[0]
     _win_cache[key] = dict(best)
     return dict(best), timings

@@ -40,6 +40,8 @@ int validate_binned(const pgld_panel_desc* desc, const pgld_binned_args* g, Pane
   if (!g->d_pos || !g->d_edges || !g->d_rtab || !g->d_out)
     return fail(PGLD_EINVAL, "null device pointer in binned args");
   if (g->with_r2 != 0 && g->with_r2 != 1) return fail(PGLD_EINVAL, "with_r2 must be 0 or 1");
+  if (g->n_tiles < 0 || g->n_tiles > 0x7fffffffLL)
+    return fail(PGLD_EINVAL, "n_tiles must be in [0, 2^31)");
   if (g->with_r2 && desc->n_pops != 1)
     return fail(PGLD_EUNSUPPORTED, "with_r2 is built for one population");
   return PGLD_OK;
@@ -156,7 +158,8 @@ extern "C" int pgld_count_band(const pgld_panel_desc* desc, const pgld_binned_ar
     return fail(PGLD_EUNSUPPORTED, "the counting-stage kernel is built for haplotypes");
   if (!d_sums || !args->d_panels || !args->d_site_stats || (!args->d_tiles && args->n_tiles))
     return fail(PGLD_EINVAL, "count band needs panels, site stats, a tile list and d_sums");
-  if (args->n_tiles < 0) return fail(PGLD_EINVAL, "negative n_tiles");
+  if (args->n_tiles < 0 || args->n_tiles > 0x7fffffffLL)
+    return fail(PGLD_EINVAL, "n_tiles must be in [0, 2^31)");
   if (args->n_tiles == 0) return PGLD_OK;
   hipStream_t s = as_stream(stream);
   return launch_count_band_hap(L, args, d_sums, s);
@@ -172,6 +175,8 @@ extern "C" int pgld_windows_fused(const pgld_panel_desc* desc, const pgld_window
   if (args->n_work < 0 || args->n_windows < 0) return fail(PGLD_EINVAL, "negative sizes");
   if (args->tile != 0 && args->tile != 16 && args->tile != 32)
     return fail(PGLD_EINVAL, "tile must be 16 or 32");
+  if (args->group < 0 || args->group > 4096)
+    return fail(PGLD_EINVAL, "group must be in [0, 4096]");
   if (args->n_work == 0) return PGLD_OK;
   if (!args->d_panels || !args->d_site_stats || !args->d_rtab || !args->d_work ||
       !args->d_windows || !args->d_out)

@@ -326,7 +326,26 @@ struct WindowParams {
   int64_t n_work;
   const int64_t* win;      // [n_windows][2]
   unsigned long long* out; // [n_windows][NSO + 1][3]
+  int32_t group;           // consecutive work items per wave turn
 };
+
+// Adds one item's wave total `tot` to the lane's running limbs: statistic
+// lanes add its fixed-point limbs, the count lane the exact pair count.  One
+// unconditional add per limb (a conditional target would put the limbs in
+// scratch memory).
+template <class T, int NV, int LOG>
+__device__ __forceinline__ void window_add_limbs(double tot, int lane, long long& L0,
+                                                 long long& L1, long long& L2) {
+  constexpr int GS = T::template gsz<0>();
+  const int q = pgld_bitrev(lane & (NV - 1), LOG);
+  const bool stat = lane < NV && q < GS;
+  const bool cnt = lane < NV && q == GS;
+  long long x2 = 0, x1 = 0, x0 = 0;
+  if (stat) pgld_to_limbs(tot, x2, x1, x0, pgld_limb_scale(T::template scale_exp<0>(q)));
+  L0 += x0 + (long long)(cnt ? tot : 0.0);
+  L1 += x1;
+  L2 += x2;
+}
 
 // One work item's sums over its pairs inside the window, reduced across the
 // wave and added as fixed-point limbs to the lane's running totals: lane l <
@@ -368,17 +387,7 @@ __device__ __forceinline__ void window_item(const WindowParams& w, const double*
   double v[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) v[q] = q < GS ? acc[q] : (q == GS ? (double)count : 0.0);
-  const double tot = pgld_wave_sums<NV>(v, lane);
-  const int q = pgld_bitrev(lane & (NV - 1), LOG);
-  if (lane < NV && q < GS) {
-    long long x2, x1, x0;
-    pgld_to_limbs(tot, x2, x1, x0, pgld_limb_scale(T::template scale_exp<0>(q)));
-    L0 += x0;
-    L1 += x1;
-    L2 += x2;
-  } else if (lane < NV && q == GS) {
-    L0 += (long long)tot;
-  }
+  window_add_limbs<T, NV, LOG>(pgld_wave_sums<NV>(v, lane), lane, L0, L1, L2);
 }
 
 // Adds a lane's running totals of window `win` to the output.
@@ -401,10 +410,27 @@ __device__ __forceinline__ void window_flush(const WindowParams& w, int64_t win,
   }
 }
 
-// Each wave takes a contiguous chunk of the work list (chunks of one XCD
-// adjacent, so its waves share windows' panels in that XCD's L2) and keeps
-// its window's totals in registers while consecutive items stay in that
-// window: global atomics once per (wave, window) instead of per item.
+// Work items of one XCD (workgroup b runs on XCD b % 8) are a contiguous
+// eighth of the work list; inside it wave r of the XCD's R waves takes groups
+// r, r + R, r + 2R, ... of `group` consecutive items.  At any time the XCD's
+// waves work on neighbouring items (the same windows' panels, held in that
+// XCD's L2), and a wave keeps its window's totals in registers while its
+// items stay in that window: global atomics once per (wave, group, window)
+// instead of per item.  Every bound is wave-uniform.
+struct WinSched {
+  int64_t first, stride, end;
+  __device__ WinSched(int64_t n_work, int wave, int group) {
+    constexpr int nx = 8;  // the grid is a multiple of 8
+    const int64_t xcd = blockIdx.x % nx;
+    const int64_t R = (int64_t)(gridDim.x / nx) * 4;
+    const int64_t r = (int64_t)(blockIdx.x / nx) * 4 + wave;
+    const int64_t lo = n_work * xcd / nx;
+    end = n_work * (xcd + 1) / nx;
+    first = lo + r * group;
+    stride = R * group;
+  }
+};
+
 template <class T, bool GTAB = false>
 __global__ void __launch_bounds__(256, T::MINB) window_kernel(WindowParams w) {
   static_assert(T::NGT == 1, "window sums use one statistic pass");
@@ -422,14 +448,12 @@ __global__ void __launch_bounds__(256, T::MINB) window_kernel(WindowParams w) {
     }
     __syncthreads();
   }
-  const int nx = 8;
-  const int64_t rank = ((int64_t)(blockIdx.x % nx) * (gridDim.x / nx) + blockIdx.x / nx) * 4 + wave;
-  const int64_t n_waves = (int64_t)gridDim.x * 4;
-  const int64_t chunk = (w.n_work + n_waves - 1) / n_waves;
-  const int64_t k_end = min(rank * chunk + chunk, w.n_work);
+  const int group = w.group;
+  const WinSched ws(w.n_work, wave, group);
   int64_t cur = -1;
   long long L0 = 0, L1 = 0, L2 = 0;
-  for (int64_t k = rank * chunk; k < k_end; ++k) {
+  for (int64_t g0 = ws.first; g0 < ws.end; g0 += ws.stride)
+  for (int64_t k = g0, k1 = min(g0 + group, ws.end); k < k1; ++k) {
     const int64_t win = __builtin_amdgcn_readfirstlane(w.work[3 * k]);
     const int64_t rb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 1]);
     const int64_t cb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 2]);
@@ -567,17 +591,7 @@ __device__ __forceinline__ void window_item16(const WindowParams& w, const doubl
   double v[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) v[q] = q < GS ? acc[q] : (q == GS ? (double)count : 0.0);
-  const double tot = pgld_wave_sums<NV>(v, lane);
-  const int q = pgld_bitrev(lane & (NV - 1), 3);
-  if (lane < NV && q < GS) {
-    long long x2, x1, x0;
-    pgld_to_limbs(tot, x2, x1, x0, pgld_limb_scale(T::template scale_exp<0>(q)));
-    L0 += x0;
-    L1 += x1;
-    L2 += x2;
-  } else if (lane < NV && q == GS) {
-    L0 += (long long)tot;
-  }
+  window_add_limbs<T, NV, 3>(pgld_wave_sums<NV>(v, lane), lane, L0, L1, L2);
 }
 
 // The window kernel over 16-site work items (blocks of 16 sites in d_work).
@@ -594,14 +608,12 @@ __global__ void __launch_bounds__(256, 3) window16_kernel(WindowParams w) {
     for (int e = tid; e < 4 * len; e += blockDim.x) stab[e] = a.rtab[4 * a.rt_lo[0] + e];
     __syncthreads();
   }
-  const int nx = 8;
-  const int64_t rank = ((int64_t)(blockIdx.x % nx) * (gridDim.x / nx) + blockIdx.x / nx) * 4 + wave;
-  const int64_t n_waves = (int64_t)gridDim.x * 4;
-  const int64_t chunk = (w.n_work + n_waves - 1) / n_waves;
-  const int64_t k_end = min(rank * chunk + chunk, w.n_work);
+  const int group = w.group;
+  const WinSched ws(w.n_work, wave, group);
   int64_t cur = -1;
   long long L0 = 0, L1 = 0, L2 = 0;
-  for (int64_t k = rank * chunk; k < k_end; ++k) {
+  for (int64_t g0 = ws.first; g0 < ws.end; g0 += ws.stride)
+  for (int64_t k = g0, k1 = min(g0 + group, ws.end); k < k1; ++k) {
     const int64_t win = __builtin_amdgcn_readfirstlane(w.work[3 * k]);
     const int64_t rb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 1]);
     const int64_t cb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 2]);
@@ -644,6 +656,7 @@ static int launch_windows(const PanelLayout& L, const pgld_window_args* g, hipSt
   w.n_work = g->n_work;
   w.win = g->d_windows;
   w.out = reinterpret_cast<unsigned long long*>(g->d_out);
+  w.group = g->group > 0 ? g->group : (g->tile == 16 ? 32 : 16);
   size_t lds = (size_t)a.rt_len * 4 * sizeof(double);
   // a table wider than the LDS (a large population with heavily missing
   // sites) is read from global memory, indexed by n

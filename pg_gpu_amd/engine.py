@@ -39,33 +39,31 @@ def reciprocal_table(kmax):
     return tab.ravel()
 
 
-def band_tiles(pos, max_dist, square=None):
+def band_tiles(pos, max_dist, square=None, with_pairs=False):
     """(row tile, col tile) pairs of TILE sites covering every i<j pair with
     pos[j] <= pos[i] + max_dist (pg_gpu/ld_pipeline.py:85-88), in squares of
-    ``square`` x ``square`` tiles (default SUPER)."""
+    ``square`` x ``square`` tiles (default SUPER): squares in row-major order,
+    tiles row-major inside a square.  The waves running at the same time on
+    one XCD then share ~2 * square operand panels in that XCD's L2.  Built by
+    the library's host planner (pgld_band_tiles, one sweep over the sites);
+    ``with_pairs`` also returns the number of pairs within the cutoff."""
+    import ctypes
+
     sq = SUPER if square is None else int(square)
     if sq < 1:
         raise ValueError("square must be positive")
-    pos = np.asarray(pos, np.float64)
-    V = len(pos)
-    if V < 2:
-        return np.zeros((0, 2), np.int32)
-    jmax = np.searchsorted(pos, pos + float(max_dist), side="right")
-    n_rt = (V + TILE - 1) // TILE
-    last_row = np.minimum(np.arange(n_rt) * TILE + TILE - 1, V - 1)
-    last_col = jmax[last_row] - 1                       # largest partner of the tile's last row
-    cb_max = last_col // TILE
-    rb = np.arange(n_rt)
-    n_per = np.where(last_col > rb * TILE, cb_max - rb + 1, 0)
-    rows = np.repeat(rb, n_per)
-    start = np.repeat(np.cumsum(n_per) - n_per, n_per)
-    cols = rows + (np.arange(int(n_per.sum())) - start)
-    # Group the band into SUPER x SUPER squares of tiles, squares in row-major
-    # order and tiles row-major inside a square: the waves running at the same
-    # time on one XCD then share ~2 * SUPER operand panels in that XCD's L2.
-    key = (rows // sq) * (n_rt + sq) + (cols // sq)
-    order = np.lexsort((cols, rows, key))
-    return np.stack([rows[order], cols[order]], axis=1).astype(np.int32)
+    pos = np.ascontiguousarray(pos, np.float64)
+    L = _lib.lib()
+    n_tiles, n_pairs = ctypes.c_int64(0), ctypes.c_int64(0)
+    pp = pos.ctypes.data_as(ctypes.c_void_p)
+    _lib.check(L.pgld_band_tiles(pp, len(pos), float(max_dist), sq, None, 0,
+                                 ctypes.byref(n_tiles), ctypes.byref(n_pairs)))
+    out = np.empty((n_tiles.value, 2), np.int32)
+    if n_tiles.value:
+        _lib.check(L.pgld_band_tiles(pp, len(pos), float(max_dist), sq,
+                                     out.ctypes.data_as(ctypes.c_void_p), n_tiles.value,
+                                     ctypes.byref(n_tiles), ctypes.byref(n_pairs)))
+    return (out, n_pairs.value) if with_pairs else out
 
 
 def decode_limbs(raw, n_stats):
@@ -135,15 +133,16 @@ class BinnedPlan:
         self.d_coord = (torch.as_tensor(co_np, device=pan.device)
                         if coords is not None else self.d_pos)
         self.d_edges = torch.as_tensor(bins, device=pan.device)
-        self.tiles = (band_tiles(pos_np, self.max_dist, self.square) if tiles is None
-                      else np.asarray(tiles))
+        if tiles is None:
+            self.tiles, self.n_pairs = band_tiles(pos_np, self.max_dist, self.square,
+                                                  with_pairs=True)
+        else:
+            self.tiles = np.asarray(tiles)
         self.d_tiles = torch.as_tensor(self.tiles.reshape(-1), dtype=torch.int32,
                                        device=pan.device)
-        jmax = np.searchsorted(pos_np, pos_np + self.max_dist, side="right")
-        per = np.maximum(0, jmax - np.arange(pan.V) - 1)
-        if tiles is None:
-            self.n_pairs = int(per.sum())
-        else:  # pairs whose anchor lies in the listed row tiles
+        if tiles is not None:  # pairs whose anchor lies in the listed row tiles
+            jmax = np.searchsorted(pos_np, pos_np + self.max_dist, side="right")
+            per = np.maximum(0, jmax - np.arange(pan.V) - 1)
             rows = np.unique(self.tiles[:, 0]) if len(self.tiles) else np.zeros(0, int)
             sel = np.zeros(pan.V, bool)
             for r in rows:

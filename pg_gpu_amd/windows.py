@@ -68,7 +68,7 @@ class WindowPlan:
     the side of the work items: 32 (32 x 32 MFMA tiles) or 16 (16 x 16,
     haplotypes); autotune.tune_windows picks the faster for a job."""
 
-    def __init__(self, pan: LDPanels, windows, tile=TILE):
+    def __init__(self, pan: LDPanels, windows, tile=TILE, group=0):
         torch = _torch()
         if pan.P != 1:
             raise NotImplementedError("window sums are built for one population")
@@ -84,6 +84,9 @@ class WindowPlan:
             raise NotImplementedError("16-site window tiles are built for haplotypes")
         self.pan = pan
         self.tile = int(tile)
+        if not 0 <= int(group) <= 4096:
+            raise ValueError("group must be in [0, 4096]")
+        self.group = int(group)
         self.windows = win
         self.work = window_work(win, self.tile)
         k = win[:, 1] - win[:, 0]
@@ -110,6 +113,7 @@ class WindowPlan:
         a.grid = int(grid)
         a.n_lo[0] = int(pan.n_lo[0])
         a.tile = self.tile
+        a.group = self.group
         return a
 
     def run(self, kernel="fused", grid=0):

@@ -102,5 +102,24 @@ def test_window_autotune_picks_a_measured_tile(cuda):
     w = synth.make("c3", scale=0.004)
     pan = LDPanels(torch.as_tensor(w.mat, device=cuda), [list(range(w.mat.shape[0]))], "hap")
     best, timings = autotune.tune_windows(pan, w.windows, use_cache=False)
-    assert best["tile"] in (16, 32) and len(timings) == 2
+    assert best["tile"] in (16, 32) and len(timings) == 6
     assert best == min(timings, key=lambda t: t[1])[0]
+
+
+def test_window_groups_give_identical_sums(cuda):
+    """The work-item group size changes only which wave takes which items:
+    the integer limb sums are bit-identical for every group size and tile."""
+    import torch
+
+    w = synth.make("c3", scale=0.004)
+    sel = [2, 3, 5, 6, 9, 13, 17]
+    pan = LDPanels(torch.as_tensor(w.mat, device=cuda), [list(range(w.mat.shape[0]))], "hap")
+    for tile in (32, 16):
+        print(f"case: groups tile {tile}", flush=True)
+        base = WindowPlan(pan, w.windows[sel], tile=tile).run().finish()
+        for group in (1, 3, 64, 4096):
+            got = WindowPlan(pan, w.windows[sel], tile=tile, group=group).run().finish()
+            for f in ("dd", "dz", "pi2", "r2_sum", "r2_count", "n_pairs"):
+                np.testing.assert_array_equal(getattr(got, f), getattr(base, f), err_msg=f)
+    with pytest.raises(ValueError):
+        WindowPlan(pan, w.windows[sel], group=5000)

@@ -173,6 +173,42 @@ def test_band_tile_squares_are_permutations(square):
         np.testing.assert_array_equal(got, ref[np.lexsort((ref[:, 1], ref[:, 0]))])
 
 
+def _band_tiles_spec(pos, md, sq):
+    """The band tile list by definition: every 32-site tile pair (r, c >= r)
+    holding a pair within the cutoff, ordered by (square row, square column,
+    row, column)."""
+    V = len(pos)
+    T = engine.TILE
+    have = set()
+    ii, jj = o.pairs_within_distance(pos, md)
+    for i, j in zip(ii // T, jj // T):
+        have.add((int(i), int(j)))
+    return sorted(have, key=lambda t: (t[0] // sq, t[1] // sq, t[0], t[1])), len(ii)
+
+
+@pytest.mark.parametrize("seed,V,md,sq", [(0, 0, 5.0, 16), (1, 1, 5.0, 16), (2, 2, 5.0, 1),
+                                          (3, 700, 40.0, 3), (4, 999, 1e9, 16),
+                                          (5, 1500, 0.0, 8), (6, 2100, 200.0, 64)])
+def test_band_tiles_order_and_pair_count(seed, V, md, sq):
+    """pgld_band_tiles (host planner) against the definition: the set of
+    tiles, their square order and the number of pairs within the cutoff."""
+    rng = np.random.default_rng(seed)
+    pos = np.sort(rng.integers(0, 3 * V + 1, V)).astype(np.float64)
+    tiles, n_pairs = engine.band_tiles(pos, md, sq, with_pairs=True)
+    want, n_want = _band_tiles_spec(pos, md, sq)
+    assert [tuple(t) for t in tiles.tolist()] == want
+    assert n_pairs == n_want
+
+
+def test_band_tiles_rejects_bad_input():
+    with pytest.raises(ValueError):
+        engine.band_tiles(np.array([3.0, 1.0, 2.0]), 10.0)
+    with pytest.raises(ValueError):
+        engine.band_tiles(np.arange(100.0), -1.0)
+    with pytest.raises(ValueError):
+        engine.band_tiles(np.arange(100.0), 10.0, square=0)
+
+
 def test_smoke_case_runs_on_cpu():
     """The oracle side of __graft_entry__.smoke() (workload and expected sums)."""
     import importlib.util
