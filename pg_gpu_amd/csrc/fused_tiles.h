@@ -135,6 +135,65 @@ __device__ __forceinline__ void mfma_products4(const v4i* A, const v4i* Av, cons
   }
 }
 
+// acc[3 f + g] = A_f . B_g over the ks k-steps for three features f, g
+// (fragments of feature f at A + f fs): the genotype counting products.  The
+// same register ring as mfma_products4, PF k-steps of six operands deep (two
+// for one population; one where several populations fill the registers).
+template <int PF>
+__device__ __forceinline__ void mfma_products9(const v4i* A, const v4i* B, int fs, int ks,
+                                               v16f (&acc)[9]) {
+  v4i ra[PF][3], rb[PF][3];
+#pragma unroll
+  for (int d = 0; d < PF; ++d) {
+    const int n = d < ks ? d * 64 : 0;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) { ra[d][f] = A[f * fs + n]; rb[d][f] = B[f * fs + n]; }
+  }
+  const int full = ks / PF;
+  if (full > 0) {
+#pragma unroll
+    for (int d = 0; d < PF; ++d) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f)
+#pragma unroll
+        for (int g = 0; g < 3; ++g)
+          acc[3 * f + g] = pgld_mfma_fp4(ra[d][f], rb[d][g], d == 0 ? zero16f() : acc[3 * f + g]);
+      const int s = PF + d;
+      const int n = s < ks ? s * 64 : 0;
+#pragma unroll
+      for (int f = 0; f < 3; ++f) { ra[d][f] = A[f * fs + n]; rb[d][f] = B[f * fs + n]; }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] = zero16f();
+  }
+  for (int gi = 1; gi < full; ++gi) {
+#pragma unroll
+    for (int d = 0; d < PF; ++d) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f)
+#pragma unroll
+        for (int g = 0; g < 3; ++g)
+          acc[3 * f + g] = pgld_mfma_fp4(ra[d][f], rb[d][g], acc[3 * f + g]);
+      const int s = (gi + 1) * PF + d;
+      const int n = s < ks ? s * 64 : 0;
+#pragma unroll
+      for (int f = 0; f < 3; ++f) { ra[d][f] = A[f * fs + n]; rb[d][f] = B[f * fs + n]; }
+    }
+  }
+  const int rem = ks - full * PF;
+#pragma unroll
+  for (int d = 0; d < PF - 1; ++d) {
+    if (d < rem) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f)
+#pragma unroll
+        for (int g = 0; g < 3; ++g)
+          acc[3 * f + g] = pgld_mfma_fp4(ra[d][f], rb[d][g], acc[3 * f + g]);
+    }
+  }
+}
+
 // Operand ring in LDS: each wave streams its tile's fragments through a
 // private ring of RING k-steps x 4 fragments (1 KiB each) with
 // global_load_lds (lane l's 16 bytes land at byte 16 l of the slot), so loads
@@ -354,7 +413,7 @@ __device__ __forceinline__ void geno_pack_slot(v16u (&w)[W], int r, const int (&
   }
 }
 
-template <int W>
+template <int W, int PF>
 __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, int64_t rb,
                                                  int64_t cb, int lane, v16u (&w)[W]) {
   const int ks = a.ksteps[p];
@@ -363,47 +422,12 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
   const int fs = ks * 64;  // feature stride in v4i
   v16f bb, bc, cbm, cc;
   if (a.missing[p]) {
-    v16f bv, cv, vb, vc, vv;
-    // one k-step ahead: nine accumulators leave no registers for more (two
-    // steps ahead spilled and was no faster).  The first k-step starts from
-    // zero accumulators (an inline constant of the MFMA), so the 144
-    // accumulator registers need no clearing; the next step's loads go to the
-    // operand registers once this step's MFMAs have read them.
-    if (ks > 0) {
-      v4i fa[3], fb[3];
-#pragma unroll
-      for (int f = 0; f < 3; ++f) { fa[f] = A[f * fs]; fb[f] = B[f * fs]; }
-      bb = pgld_mfma_fp4(fa[0], fb[0], zero16f());
-      bc = pgld_mfma_fp4(fa[0], fb[1], zero16f());
-      bv = pgld_mfma_fp4(fa[0], fb[2], zero16f());
-      cbm = pgld_mfma_fp4(fa[1], fb[0], zero16f());
-      cc = pgld_mfma_fp4(fa[1], fb[1], zero16f());
-      cv = pgld_mfma_fp4(fa[1], fb[2], zero16f());
-      vb = pgld_mfma_fp4(fa[2], fb[0], zero16f());
-      vc = pgld_mfma_fp4(fa[2], fb[1], zero16f());
-      vv = pgld_mfma_fp4(fa[2], fb[2], zero16f());
-      if (ks > 1)
-#pragma unroll
-        for (int f = 0; f < 3; ++f) { fa[f] = A[f * fs + 64]; fb[f] = B[f * fs + 64]; }
-      for (int s = 1; s < ks; ++s) {
-        bb = pgld_mfma_fp4(fa[0], fb[0], bb);
-        bc = pgld_mfma_fp4(fa[0], fb[1], bc);
-        bv = pgld_mfma_fp4(fa[0], fb[2], bv);
-        cbm = pgld_mfma_fp4(fa[1], fb[0], cbm);
-        cc = pgld_mfma_fp4(fa[1], fb[1], cc);
-        cv = pgld_mfma_fp4(fa[1], fb[2], cv);
-        vb = pgld_mfma_fp4(fa[2], fb[0], vb);
-        vc = pgld_mfma_fp4(fa[2], fb[1], vc);
-        vv = pgld_mfma_fp4(fa[2], fb[2], vv);
-        if (s + 1 < ks) {
-          const int n = (s + 1) * 64;
-#pragma unroll
-          for (int f = 0; f < 3; ++f) { fa[f] = A[f * fs + n]; fb[f] = B[f * fs + n]; }
-        }
-      }
-    } else {
-      bb = bc = cbm = cc = bv = cv = vb = vc = vv = zero16f();
-    }
+    v16f acc[9];
+    mfma_products9<PF>(A, B, fs, ks, acc);
+    bb = acc[0]; bc = acc[1];
+    const v16f bv = acc[2];
+    cbm = acc[3]; cc = acc[4];
+    const v16f cv = acc[5], vb = acc[6], vc = acc[7], vv = acc[8];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int n11 = (int)bb[r], n12 = (int)bc[r], n21 = (int)cbm[r], n22 = (int)cc[r];
@@ -415,31 +439,7 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
       geno_pack_slot(w, r, c);
     }
   } else {
-    bb = bc = cbm = cc = zero16f();
-    constexpr int PF = 2;
-    v4i fa[PF][2], fb[PF][2];
-#pragma unroll
-    for (int d = 0; d < PF; ++d)
-      if (d < ks)
-#pragma unroll
-        for (int f = 0; f < 2; ++f) { fa[d][f] = A[f * fs + d * 64]; fb[d][f] = B[f * fs + d * 64]; }
-    for (int s0 = 0; s0 < ks; s0 += PF) {
-#pragma unroll
-      for (int d = 0; d < PF; ++d) {
-        const int s = s0 + d;
-        if (s < ks) {
-          bb = pgld_mfma_fp4(fa[d][0], fb[d][0], bb);
-          bc = pgld_mfma_fp4(fa[d][0], fb[d][1], bc);
-          cbm = pgld_mfma_fp4(fa[d][1], fb[d][0], cbm);
-          cc = pgld_mfma_fp4(fa[d][1], fb[d][1], cc);
-          if (s + PF < ks) {
-            const int n = (s + PF) * 64;
-#pragma unroll
-            for (int f = 0; f < 2; ++f) { fa[d][f] = A[f * fs + n]; fb[d][f] = B[f * fs + n]; }
-          }
-        }
-      }
-    }
+    mfma_products4<PGLD_PF_MISSING>(A, A + fs, B, B + fs, ks, bb, bc, cbm, cc);
     const int32_t* st = a.site_stats + (int64_t)p * 3 * a.V;
     const int64_t j = cb * 32 + (lane & 31);
     const int vbj = j < a.V ? st[j] : 0, vcj = j < a.V ? st[a.V + j] : 0;
@@ -632,7 +632,7 @@ struct GenoTraits {
   __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane,
                                                 Counts& c, v4i*) {
 #pragma unroll
-    for (int p = 0; p < P; ++p) geno_tile_counts<W>(a, p, rb, cb, lane, c.w[p]);
+    for (int p = 0; p < P; ++p) geno_tile_counts<W, P == 1 ? 2 : 1>(a, p, rb, cb, lane, c.w[p]);
   }
   template <int G>
   __device__ __forceinline__ static void stats(const FusedParams& a, const double* srt, const Counts& c, int r,
