@@ -274,15 +274,14 @@ def end_to_end(mat, w, r2, launch, n_pairs, reps=3):
     and decoding of the limbs -- the median of ``reps``."""
     import torch
 
-    from pg_gpu_amd.engine import LDPanels, band_tiles
+    from pg_gpu_amd.engine import LDPanels
 
     times = []
     for _ in range(reps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         pan = LDPanels(mat, w.pop_rows, w.mode)
-        tiles = band_tiles(w.positions, w.max_dist, launch.get("square"))
-        plan = pan.plan(w.positions, w.bins, w.max_dist, tiles=tiles)
+        plan = pan.plan(w.positions, w.bins, w.max_dist, square=launch.get("square"))
         plan.run("fused", launch.get("grid", 0), r2, launch.get("dynamic", True)).finish()
         times.append(time.perf_counter() - t0)
         del pan, plan

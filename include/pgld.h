@@ -131,7 +131,8 @@ int pgld_n_stats(int32_t mode, int32_t n_pops);
  * i < j with pos[j] <= pos[i] + max_dist, in squares of `square` x `square`
  * tiles (row-major squares, row-major inside), and to *n_tiles their number;
  * with tiles == NULL only the counts are computed.  *n_pairs (optional)
- * receives the number of such pairs.  pos: host float64, sorted.  Replaces the
+ * receives the number of such pairs (one pass over the sites; pass NULL to
+ * skip it).  pos: host float64, sorted.  Replaces the
  * band enumeration of iter_pairs_within_distance (pg_gpu/ld_pipeline.py:85-88)
  * at tile granularity, and its total pair count (:85-90). */
 int pgld_band_tiles(const double* pos, int64_t n_sites, double max_dist, int32_t square,

@@ -13,8 +13,9 @@ using namespace pgld;
 // partners, so block r's column run ends at the block of that row's last
 // partner.  Order: squares of `square` x `square` tiles in row-major order,
 // tiles row-major inside a square (the waves of one XCD then share about
-// 2 x square operand panels in its L2).  With `tiles` NULL only the counts
-// are returned.  n_pairs: sum over i of #{j > i : pos[j] <= pos[i] + max_dist}.
+// 2 x square operand panels in its L2).  With `tiles` NULL only the count is
+// returned.  n_pairs (optional): sum over i of #{j > i : pos[j] <= pos[i] +
+// max_dist}.
 extern "C" int pgld_band_tiles(const double* pos, int64_t n_sites, double max_dist,
                                int32_t square, int32_t* tiles, int64_t cap, int64_t* n_tiles,
                                int64_t* n_pairs) {
@@ -27,23 +28,26 @@ extern "C" int pgld_band_tiles(const double* pos, int64_t n_sites, double max_di
     if (pos[i] < pos[i - 1]) return fail(PGLD_EINVAL, "positions must be sorted");
   constexpr int64_t TILE = 32;
   const int64_t n_rt = n_sites < 2 ? 0 : (n_sites + TILE - 1) / TILE;
-  // column run end of each row block (-1: no pair), by one forward sweep
+  // column run end of each row block (-1: no pair): the block of the last
+  // partner of its last row (binary search, one per block)
   std::vector<int64_t> c_hi(n_rt, -1);
-  int64_t np = 0;
-  {
-    int64_t j = 0;
+  for (int64_t r = 0; r < n_rt; ++r) {
+    const int64_t i = std::min(r * TILE + TILE - 1, n_sites - 1);
+    const int64_t j = std::upper_bound(pos + i + 1, pos + n_sites, pos[i] + max_dist) - pos;
+    if (j - 1 > r * TILE) c_hi[r] = (j - 1) / TILE;
+  }
+  // the pair count, by one forward sweep (j = first partner past the cutoff,
+  // non-decreasing in i); only when asked for
+  if (n_pairs) {
+    int64_t np = 0, j = 0;
     for (int64_t i = 0; i < n_sites; ++i) {
       const double lim = pos[i] + max_dist;
       if (j < i + 1) j = i + 1;
-      while (j < n_sites && pos[j] <= lim) ++j;  // j: first partner past the cutoff
+      while (j < n_sites && pos[j] <= lim) ++j;
       np += j - i - 1;
-      if ((i % TILE) == TILE - 1 || i == n_sites - 1) {
-        const int64_t r = i / TILE;
-        if (j - 1 > r * TILE) c_hi[r] = (j - 1) / TILE;
-      }
     }
+    *n_pairs = np;
   }
-  if (n_pairs) *n_pairs = np;
   const int64_t sq = square;
   int64_t k = 0;
   for (int64_t R0 = 0; R0 < n_rt; R0 += sq) {

@@ -57,12 +57,12 @@ def band_tiles(pos, max_dist, square=None, with_pairs=False):
     n_tiles, n_pairs = ctypes.c_int64(0), ctypes.c_int64(0)
     pp = pos.ctypes.data_as(ctypes.c_void_p)
     _lib.check(L.pgld_band_tiles(pp, len(pos), float(max_dist), sq, None, 0,
-                                 ctypes.byref(n_tiles), ctypes.byref(n_pairs)))
+                                 ctypes.byref(n_tiles), None))
     out = np.empty((n_tiles.value, 2), np.int32)
-    if n_tiles.value:
-        _lib.check(L.pgld_band_tiles(pp, len(pos), float(max_dist), sq,
-                                     out.ctypes.data_as(ctypes.c_void_p), n_tiles.value,
-                                     ctypes.byref(n_tiles), ctypes.byref(n_pairs)))
+    _lib.check(L.pgld_band_tiles(pp, len(pos), float(max_dist), sq,
+                                 out.ctypes.data_as(ctypes.c_void_p), n_tiles.value,
+                                 ctypes.byref(n_tiles), ctypes.byref(n_pairs) if with_pairs
+                                 else None))
     return (out, n_pairs.value) if with_pairs else out
 
 
@@ -133,24 +133,40 @@ class BinnedPlan:
         self.d_coord = (torch.as_tensor(co_np, device=pan.device)
                         if coords is not None else self.d_pos)
         self.d_edges = torch.as_tensor(bins, device=pan.device)
-        if tiles is None:
-            self.tiles, self.n_pairs = band_tiles(pos_np, self.max_dist, self.square,
-                                                  with_pairs=True)
-        else:
-            self.tiles = np.asarray(tiles)
+        self._pos_np = np.ascontiguousarray(pos_np)
+        self._n_pairs = None
+        self._full_band = tiles is None
+        self.tiles = (band_tiles(pos_np, self.max_dist, self.square) if tiles is None
+                      else np.asarray(tiles))
         self.d_tiles = torch.as_tensor(self.tiles.reshape(-1), dtype=torch.int32,
                                        device=pan.device)
-        if tiles is not None:  # pairs whose anchor lies in the listed row tiles
-            jmax = np.searchsorted(pos_np, pos_np + self.max_dist, side="right")
-            per = np.maximum(0, jmax - np.arange(pan.V) - 1)
-            rows = np.unique(self.tiles[:, 0]) if len(self.tiles) else np.zeros(0, int)
-            sel = np.zeros(pan.V, bool)
-            for r in rows:
-                sel[r * TILE:(r + 1) * TILE] = True
-            self.n_pairs = int(per[sel].sum())
         self.ns = pan.n_stats
         if self.ns < 0:
             raise NotImplementedError("statistic set not available for this mode / n_pops")
+
+    @property
+    def n_pairs(self):
+        """Pairs within the cutoff (with an explicit tile list: those whose
+        anchor lies in one of its row tiles); counted on first use."""
+        if self._n_pairs is None:
+            pos_np, V = self._pos_np, self.pan.V
+            if self._full_band:
+                import ctypes
+
+                n_t, n_p = ctypes.c_int64(0), ctypes.c_int64(0)
+                _lib.check(_lib.lib().pgld_band_tiles(
+                    pos_np.ctypes.data_as(ctypes.c_void_p), V, self.max_dist, 1, None, 0,
+                    ctypes.byref(n_t), ctypes.byref(n_p)))
+                self._n_pairs = int(n_p.value)
+            else:
+                jmax = np.searchsorted(pos_np, pos_np + self.max_dist, side="right")
+                per = np.maximum(0, jmax - np.arange(V) - 1)
+                rows = (np.unique(self.tiles[:, 0]) if len(self.tiles)
+                        else np.zeros(0, np.int64))
+                sel = np.zeros((V + TILE - 1) // TILE, bool)
+                sel[rows] = True
+                self._n_pairs = int(per[np.repeat(sel, TILE)[:V]].sum())
+        return self._n_pairs
 
     def args(self, out, grid=0, r2=False):
         pan = self.pan
@@ -196,7 +212,7 @@ class BinnedPlan:
             a.d_sched = _lib.ptr(sched).value
         fn = _lib.lib().pgld_binned_fused if kernel == "fused" else _lib.lib().pgld_binned_plain
         _lib.check(fn(ctypes.byref(pan.desc), ctypes.byref(a), _lib.stream_handle()))
-        res = PendingResult(out, self.ns, self.n_pairs, len(self.tiles), r2)
+        res = PendingResult(out, self.ns, lambda: self.n_pairs, len(self.tiles), r2)
         res._sched = sched  # kept alive until the launch has been waited for
         return res
 
@@ -248,7 +264,8 @@ class GraphRun:
 
     def replay(self):
         self.graph.replay()
-        return PendingResult(self.out, self.plan.ns, self.plan.n_pairs, len(self.plan.tiles),
+        return PendingResult(self.out, self.plan.ns, lambda: self.plan.n_pairs,
+                             len(self.plan.tiles),
                              self.r2)
 
 
@@ -256,10 +273,17 @@ class GraphRun:
 class BinnedResult:
     sums: np.ndarray        # [n_bins, n_stats] exact-summed FP64
     counts: np.ndarray      # [n_bins] pairs per bin
-    n_pairs: int            # enumerated pairs (within the cutoff)
+    _n_pairs: object        # pairs within the cutoff: an int, or a callable giving it
     n_tiles: int
     r2_sum: np.ndarray = None    # [n_bins] sum of r^2 where defined (with r2=True)
     r2_count: np.ndarray = None  # [n_bins] pairs with r^2 defined
+
+    @property
+    def n_pairs(self):
+        """Enumerated pairs (within the cutoff), counted on first use."""
+        if callable(self._n_pairs):
+            self._n_pairs = int(self._n_pairs())
+        return self._n_pairs
 
     @property
     def mean_r2(self):
