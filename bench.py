@@ -50,6 +50,8 @@ WORKLOADS = {
           "20, 4000), all pairs per window, DD/Dz/pi2 + mean r^2 per window",
     "c4": "c4: 4 pops x 40/120/300/800 haplotypes, 500k SNPs / 50 Mb, 3% missing, 30% rare "
           "sites, pairs <= 500 kb, 25 log bins, 105 stats",
+    "g4": "g4: the c4 haplotypes paired into 4 pops x 20/60/150/400 diploid genotypes "
+          "(not a headline configuration: the four-population genotype kernel)",
 }
 
 

@@ -141,6 +141,11 @@ _P3_PASSES = [
     [{2}, {0, 2}],
     [{1, 2}, {0, 1, 2}],
 ]
+# Genotype sets use the same passes.  Smaller passes (at most three
+# populations, eight passes for four) remove the spills of the three-population
+# kernel and most of the four-population one, but re-derive more bases: they
+# measured slower (3 populations of 60/150/400 over 500k sites: 205 -> 216 ms;
+# 4 of 20/60/150/400: 447 -> 500 ms).
 POPSET_PASSES = {("hap", 4): _P4_PASSES, ("geno", 3): _P3_PASSES, ("geno", 4): _P4_PASSES}
 
 

@@ -17,6 +17,8 @@ Workloads (name: description):
   c4  4 pops of 40/120/300/800 haps, 500,000 SNPs on 50 Mb, 50 founders,
       BN F=0.1, 3% missing; at a random 30% of sites the allele is replaced by
       a rare allele carried by 1..12 haplotypes in total (MAF < 0.01)
+  g4  the c4 haplotypes paired into 4 pops of 20/60/150/400 diploid genotypes
+      (a genotype is missing where either haplotype is); same sites, bins
 """
 from __future__ import annotations
 
@@ -88,7 +90,7 @@ def log_bins(lo, hi, n):
 
 
 def make(name, scale=1.0, seed=None):
-    """Build workload ``name`` (c0..c4).  ``scale`` < 1 shrinks the number of
+    """Build workload ``name`` (c0..c4, g4).  ``scale`` < 1 shrinks the number of
     sites (and the span, keeping the SNP density) for quick tests."""
     idx = int(name[1])
     rng = np.random.default_rng(SEED_BASE + idx if seed is None else seed)
@@ -157,4 +159,16 @@ def make(name, scale=1.0, seed=None):
                         "hap", log_bins(100, 500_000, 25), 500_000.0,
                         params=dict(founders=50, V=V, span=span, missing=0.03, bn_F=0.1,
                                     rare_fraction=0.3))
+    if name == "g4":
+        h = make("c4", scale=scale, seed=seed)
+        parts, rows, off = [], [], 0
+        for r in h.pop_rows:
+            a, b = h.mat[r[0::2]], h.mat[r[1::2]]
+            g = (a.astype(np.int16) + b).astype(np.int8)
+            g[(a < 0) | (b < 0)] = -1
+            parts.append(g)
+            rows.append(list(range(off, off + len(g))))
+            off += len(g)
+        return Workload(name, np.concatenate(parts, axis=0), h.positions, rows, "geno", h.bins,
+                        h.max_dist, params=dict(h.params, diploid_of="c4"))
     raise ValueError(f"unknown workload {name}")

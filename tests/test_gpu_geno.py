@@ -100,6 +100,16 @@ def test_geno_workload_c2_scaled(cuda):
     run_geno(cuda, w.mat, w.positions, w.pop_rows, w.bins, w.max_dist)
 
 
+@pytest.mark.parametrize("sizes,missing_pops", [([20, 35, 12], None), ([9, 30, 17, 25], [0, 2]),
+                                                ([6, 70, 11, 40], None)])
+def test_geno_three_and_four_populations(cuda, sizes, missing_pops):
+    """Three and four genotype populations of uneven sizes (the population-set
+    passes of the fused epilogue), missing calls in all or some of them."""
+    rng = np.random.default_rng(sum(sizes))
+    geno, pos, pops = make_geno(rng, sizes, 300, 30_000, 0.04, missing_pops=missing_pops)
+    run_geno(cuda, geno, pos, pops, synth.log_bins(100, 15_000, 5), 15_000.0)
+
+
 def test_geno_fused_reproducible_across_grids(cuda):
     import torch
 
