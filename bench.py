@@ -284,7 +284,8 @@ def end_to_end(mat, w, r2, launch, n_pairs, reps=3):
         t0 = time.perf_counter()
         pan = LDPanels(mat, w.pop_rows, w.mode)
         plan = pan.plan(w.positions, w.bins, w.max_dist, square=launch.get("square"))
-        plan.run("fused", launch.get("grid", 0), r2, launch.get("dynamic", True)).finish()
+        plan.run("fused", launch.get("grid", 0), r2, launch.get("dynamic", True),
+                 launch.get("form", 0)).finish()
         times.append(time.perf_counter() - t0)
         del pan, plan
     ms = float(np.median(times)) * 1e3
@@ -307,6 +308,8 @@ def main():
                     help="skip the counting-stage-only kernel leg")
     ap.add_argument("--no-tune", action="store_true",
                     help="use the default launch shape instead of the autotuned one")
+    ap.add_argument("--form", type=int, default=0,
+                    help="fused kernel form with --no-tune: 0 per-wave, 1 producer/consumer")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB three times "
                          "(the known-size kernel that scales the HBM byte counters)")
@@ -367,19 +370,19 @@ def main():
         run = plan.run
         n_bins = None
     else:
-        launch = {"square": None, "dynamic": True, "grid": 0, "tuned": False}
+        launch = {"square": None, "dynamic": True, "grid": 0, "form": args.form, "tuned": False}
         if not args.no_tune:
             from pg_gpu_amd import autotune
 
             best, timings = autotune.tune_binned(pan, w.positions, w.bins, w.max_dist, r2=r2)
             launch = dict(best, tuned=True, candidates=len(timings),
-                          candidate_ms={f"sq{c['square']}{'d' if c['dynamic'] else 's'}": round(t, 4)
-                                        for c, t in timings})
+                          candidate_ms={f"sq{c['square']}{'d' if c['dynamic'] else 's'}"
+                                        f"f{c['form']}": round(t, 4) for c, t in timings})
             log(f"autotune: {best} over {len(timings)} launch shapes")
         tiles = band_tiles(w.positions, w.max_dist, launch["square"])
         plan = pan.plan(w.positions, w.bins, w.max_dist, tiles=tiles)
         run = lambda kernel="fused": plan.run(kernel, launch["grid"], r2,  # noqa: E731
-                                              launch["dynamic"])
+                                              launch["dynamic"], launch["form"])
         n_bins = int(len(w.bins) - 1)
 
     def step(kernel="fused", ev=None, pev=None, sev=None):

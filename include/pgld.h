@@ -120,6 +120,13 @@ typedef struct {
                                   for this launch (fused kernel), or NULL for a static
                                   tile schedule; launches sharing one buffer must not
                                   overlap.  The result does not depend on it. */
+  int32_t form;                /* fused kernel form: 0 = every wave counts its own
+                                  tiles and evaluates them; 1 = producer/consumer
+                                  (counting waves hand tiles to statistics waves
+                                  through LDS; one population, two haplotype
+                                  populations or one genotype population, when its
+                                  LDS plan fits -- otherwise form 0).  The result
+                                  does not depend on it. */
 } pgld_binned_args;
 
 /* Number of statistics produced per bin for (mode, n_pops). */

@@ -54,6 +54,7 @@ class BinnedArgs(ctypes.Structure):
         ("n_lo", c_int32 * MAX_POPS),
         ("with_r2", c_int32),
         ("d_sched", c_void_p),
+        ("form", c_int32),
     ]
 
 

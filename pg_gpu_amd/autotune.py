@@ -11,6 +11,9 @@ varies with the data is how the band's tiles are ordered and dealt out:
 - ``dynamic``: atomic-counter tile claiming per XCD, or a static stride.
 - ``grid``: resident workgroups (the kernel's default fills every CU at its
   occupancy).
+- ``form``: every wave counts and evaluates its own tiles (0), or counting
+  waves hand their tiles to statistics waves through LDS (1, for the traits
+  that run two or more waves per SIMD; the library uses form 0 elsewhere).
 
 ``tune_binned`` times every candidate on the job itself (HIP events on the
 launch stream, best of ``reps`` after one warm-up) and returns the fastest;
@@ -43,12 +46,12 @@ def job_key(pan, positions, bins, max_dist, r2):
 def _time(plan, cfg, r2, reps):
     torch = dv.torch()
     stream = torch.cuda.current_stream()
-    plan.run("fused", cfg["grid"], r2, cfg["dynamic"]).finish()   # warm-up
+    plan.run("fused", cfg["grid"], r2, cfg["dynamic"], cfg["form"]).finish()   # warm-up
     best = float("inf")
     for _ in range(reps):
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record(stream)
-        pend = plan.run("fused", cfg["grid"], r2, cfg["dynamic"])
+        pend = plan.run("fused", cfg["grid"], r2, cfg["dynamic"], cfg["form"])
         b.record(stream)
         b.synchronize()
         pend.finish()
@@ -57,8 +60,8 @@ def _time(plan, cfg, r2, reps):
 
 
 def tune_binned(pan, positions, bins, max_dist=None, coords=None, r2=False, reps=3,
-                squares=SQUARES, grids=(0,), use_cache=True):
-    """-> (best config dict(square, dynamic, grid), [(config, ms)...] measured).
+                squares=SQUARES, grids=(0,), forms=(0, 1), use_cache=True):
+    """-> (best config dict(square, dynamic, grid, form), [(config, ms)...] measured).
     The best config is cached per ``job_key``; a cached job returns it with an
     empty measurement list."""
     key = job_key(pan, positions, bins, max_dist, r2)
@@ -69,8 +72,9 @@ def tune_binned(pan, positions, bins, max_dist=None, coords=None, r2=False, reps
         plan = BinnedPlan(pan, positions, bins, max_dist, coords, square=sq)
         for dyn in (True, False):
             for g in grids:
-                cfg = dict(square=int(sq), dynamic=dyn, grid=int(g))
-                timings.append((cfg, _time(plan, cfg, r2, reps)))
+                for form in forms:
+                    cfg = dict(square=int(sq), dynamic=dyn, grid=int(g), form=int(form))
+                    timings.append((cfg, _time(plan, cfg, r2, reps)))
     best = min(timings, key=lambda t: t[1])[0]
     _cache[key] = dict(best)
     return dict(best), timings

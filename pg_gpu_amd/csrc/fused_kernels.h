@@ -102,7 +102,8 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
              t.bmin == t.bmax && t.bmin >= 0 && t.bmin < nb;
     t.blo = t.bmin < 0 ? 0 : t.bmin;
     t.dual = !t.fast && t.blo < nb && (t.bmax >= nb ? nb - 1 : t.bmax) - t.blo <= 1;
-    tile_groups<T, 0>(a, t, cts);
+    const uint32_t none[4] = {0u, 0u, 0u, 0u};
+    tile_groups<T, 0>(a, t, cts, none);
     local = next;
   }
   // nothing may stay staged
@@ -124,11 +125,16 @@ static int num_cus() {
   return g_num_cus;
 }
 
+// Kernel parameters and LDS plan of a binned job (shared by both forms of the
+// fused kernel).  fixed: bytes of the bin edges and reciprocal table;
+// per_bin: bytes per bin of the staging arrays (nst statistics waves), limbs
+// and pair counters; gtab: the table is read from global memory.
 template <class T>
-static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStream_t s) {
+static int fused_params(const PanelLayout& L, const pgld_binned_args* g, int nst, FusedParams& a,
+                        size_t& fixed, size_t& per_bin, bool& gtab) {
   constexpr int NS = T::NSO;
   constexpr int P = T::NP;
-  FusedParams a;
+  a = FusedParams{};
   a.panels = g->d_panels;
   a.site_stats = g->d_site_stats;
   a.pos = g->d_pos;
@@ -161,18 +167,30 @@ static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStre
   }
   a.rt_len = off;
   constexpr size_t kLds = 160 * 1024;
-  size_t fixed = (PGLD_MAX_BINS + 8) * sizeof(double) + (size_t)off * 4 * sizeof(double);
-  const size_t per_bin = (size_t)4 * T::GS * sizeof(double) +
-                         (size_t)(NS + 1) * 3 * sizeof(unsigned long long) + 4 * sizeof(int);
+  fixed = (PGLD_MAX_BINS + 8) * sizeof(double) + (size_t)off * 4 * sizeof(double);
+  per_bin = (size_t)nst * T::GS * sizeof(double) +
+            (size_t)(NS + 1) * 3 * sizeof(unsigned long long) + (size_t)nst * sizeof(int);
   // Populations whose valid-sample counts span too wide a range for the LDS
   // table (large populations with heavily missing sites) read the global
   // table instead: every entry of a.rtab, indexed by n directly.
-  const bool gtab = fixed + per_bin > kLds;
+  gtab = fixed + per_bin > kLds;
   if (gtab) {
     for (int p = 0; p < PGLD_MAX_POPS; ++p) a.rt_lo[p] = a.rt_off[p] = 0;
     a.rt_len = 0;
     fixed = (PGLD_MAX_BINS + 8) * sizeof(double);
   }
+  return PGLD_OK;
+}
+
+template <class T>
+static int launch_fused_waves(const PanelLayout& L, const pgld_binned_args* g, hipStream_t s) {
+  constexpr int NS = T::NSO;
+  constexpr size_t kLds = 160 * 1024;
+  FusedParams a;
+  size_t fixed = 0, per_bin = 0;
+  bool gtab = false;
+  int rc = fused_params<T>(L, g, 4, a, fixed, per_bin, gtab);
+  if (rc) return rc;
   // Bins whose per-workgroup limbs do not fit in LDS together are covered by
   // consecutive launches over ranges of bins (each re-counts the band; pairs
   // outside a launch's edges are skipped by its binning).  Those launches use
@@ -205,6 +223,252 @@ static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStre
     PGLD_LAUNCH_CHECK();
   }
   return PGLD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Producer/consumer form of the fused kernel: one workgroup of 4 * MINB waves
+// per CU.  Waves 0-3 ("counting waves") claim tiles, run the MFMA counting,
+// classify each tile and, for tiles that are not single-bin, compute its bin
+// words (tile_bins); they hand the tile to a statistics wave on the same SIMD
+// through an LDS mailbox and go on to the next tile.  Waves 4 .. 4 MINB - 1
+// ("statistics waves") only run the FP64 estimators and the binning of the
+// tile in their mailbox.  The operand-load latency of the counting then
+// overlaps the statistics of the other waves instead of stalling them, and a
+// statistics wave needs no MFMA accumulators or operand registers.
+//
+// Mailbox m (statistics wave 4 + m, fed by counting wave m % 4): a state word
+// (0 empty, 1 full, 2 closed), the tile descriptor, and the counts in the
+// traits' mailbox layout followed by 4 bin words per lane.  Waits are bounded:
+// a wave that gives up adds 2^62 to bin 0's pair count (the host rejects such
+// a result) and leaves its loop, so every wave reaches the final barrier.
+// ---------------------------------------------------------------------------
+struct PcDesc {
+  int32_t rb, cb, bmin, bmax, flags, blo, pad0, pad1;
+};
+
+template <class T>
+struct PcShape {
+  static constexpr int NCONS = 4 * (T::MINB - 1);                 // statistics waves
+  static constexpr int THREADS = 64 * 4 * T::MINB;
+  static constexpr int MB_WORDS = T::NWORDS * 16 * 64 + 4 * 64;   // per mailbox
+  static constexpr size_t BYTES = 256 + (size_t)NCONS * sizeof(PcDesc) +
+                                  (size_t)NCONS * MB_WORDS * sizeof(uint32_t);
+};
+
+// Polls mailbox state `st` until it is non-zero (for_full) or zero; returns
+// the state, or -1 after about 0.3 s of polling.
+__device__ __forceinline__ int pc_wait(int* st, bool for_full) {
+  for (int it = 0; it < (1 << 22); ++it) {
+    const int v = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(st, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+    if (for_full ? v != 0 : v == 0) return v;
+    __builtin_amdgcn_s_sleep(2);
+  }
+  return -1;
+}
+
+__device__ __forceinline__ void pc_set(int* st, int v) {
+  __hip_atomic_store(st, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <class T, bool GTAB = false>
+__global__ void __launch_bounds__(PcShape<T>::THREADS, 1) fused_pc_kernel(FusedParams a) {
+  using S = PcShape<T>;
+  constexpr int NS = T::NSO, GS = T::GS, NC = S::NCONS, KPER = T::MINB - 1;
+  static_assert(T::MINB >= 2, "a statistics wave per counting wave at least");
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  double* sedges = reinterpret_cast<double*>(smem);
+  double* stab = sedges + PGLD_MAX_BINS + 8;
+  const double* srt = GTAB ? a.rtab : stab;
+  const int nb = a.n_bins;
+  double* sstage = stab + 4 * a.rt_len;                      // [NC][nb][GS]
+  unsigned long long* slimb =
+      reinterpret_cast<unsigned long long*>(sstage + NC * nb * GS);  // [nb][NS+1][3]
+  int* scnt = reinterpret_cast<int*>(slimb + nb * (NS + 1) * 3);     // [NC][nb]
+  // mailboxes from the next 256-byte boundary (offsets from smem, so the
+  // compiler keeps LDS addressing)
+  const size_t mb_off =
+      ((size_t)(reinterpret_cast<unsigned char*>(scnt + NC * nb) - smem) + 255) & ~size_t(255);
+  int* mstate = reinterpret_cast<int*>(smem + mb_off);                // [NC] (256 B)
+  PcDesc* mdesc = reinterpret_cast<PcDesc*>(smem + mb_off + 256);    // [NC]
+  uint32_t* mbox = reinterpret_cast<uint32_t*>(mdesc + NC);         // [NC][MB_WORDS]
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int e = tid; e <= nb; e += blockDim.x) sedges[e] = a.edges[e];
+  if (!GTAB) {
+    for (int p = 0; p < T::NP; ++p) {
+      const int len = a.K[p] - a.rt_lo[p] + 1;
+      for (int e = tid; e < 4 * len; e += blockDim.x)
+        stab[4 * a.rt_off[p] + e] = a.rtab[4 * a.rt_lo[p] + e];
+    }
+  }
+  for (int e = tid; e < NC * nb * GS; e += blockDim.x) sstage[e] = 0.0;
+  for (int e = tid; e < nb * (NS + 1) * 3; e += blockDim.x) slimb[e] = 0ull;
+  for (int e = tid; e < NC * nb; e += blockDim.x) scnt[e] = 0;
+  for (int e = tid; e < NC; e += blockDim.x) mstate[e] = 0;
+  __syncthreads();
+  bool ok = true;
+  if (wave < 4) {
+    // ---- counting wave: tiles of this XCD's segment, as in fused_kernel
+    const int nx = 8;
+    const int xcd = blockIdx.x % nx;
+    const int waves_per_xcd = (gridDim.x / nx) * 4;
+    const int64_t seg_lo = a.n_tiles * xcd / nx, seg_hi = a.n_tiles * (xcd + 1) / nx;
+    uint32_t* ctr = a.sched ? a.sched + xcd * 32 : nullptr;
+    int64_t local = (blockIdx.x / nx) * 4 + wave;
+    int32_t nrb = 0, ncb = 0;
+    if (seg_lo + local < seg_hi) {
+      nrb = a.tiles[2 * (seg_lo + local)];
+      ncb = a.tiles[2 * (seg_lo + local) + 1];
+    }
+    TileCtx t;
+    t.sedges = sedges;
+    t.lane = lane;
+    int turn = 0;
+    while (seg_lo + local < seg_hi) {
+      const int64_t rb = __builtin_amdgcn_readfirstlane(nrb), cb = __builtin_amdgcn_readfirstlane(ncb);
+      uint32_t claim = 0;
+      if (ctr && lane == 0) claim = atomicAdd(ctr, 1u);
+      const int64_t i0 = rb * 32, j0 = cb * 32;
+      const int64_t i1 = min(i0 + 31, a.V - 1), j1 = min(j0 + 31, a.V - 1);
+      const double d_min = coord_diff(a, i1, j0), d_max = coord_diff(a, i0, j1);
+      const bool in_cut = a.pos[j1] <= a.pos[i0] + a.max_dist;
+      typename T::Counts cts;
+      T::counts_pc(a, rb, cb, lane, cts);
+      const int64_t next = ctr ? waves_per_xcd + (int64_t)__builtin_amdgcn_readfirstlane(__shfl(claim, 0, 64))
+                               : local + waves_per_xcd;
+      if (seg_lo + next < seg_hi) {
+        nrb = a.tiles[2 * (seg_lo + next)];
+        ncb = a.tiles[2 * (seg_lo + next) + 1];
+      }
+      const int bmin = pgld_bin_scan(d_min, sedges, nb, 0, nb - 1);
+      const int bmax = pgld_bin_scan(d_max, sedges, nb, bmin < 0 ? 0 : bmin, nb - 1);
+      const bool fast = (cb > rb) && (j1 == j0 + 31) && in_cut && bmin == bmax && bmin >= 0 &&
+                        bmin < nb;
+      const int blo = bmin < 0 ? 0 : bmin;
+      const bool dual = !fast && blo < nb && (bmax >= nb ? nb - 1 : bmax) - blo <= 1;
+      uint32_t bw[4] = {0u, 0u, 0u, 0u};
+      if (!fast) {
+        t.i0 = i0;
+        t.j0 = j0;
+        tile_bins(a, t, bw);
+      }
+      // the first empty mailbox of this wave's statistics waves
+      int m = -1;
+      for (int it = 0; it < (1 << 22) && m < 0; ++it) {
+#pragma unroll
+        for (int k = 0; k < KPER; ++k) {
+          const int mm = wave + 4 * ((turn + k) % KPER);
+          if (m < 0 && __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                           mstate + mm, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) == 0)
+            m = mm;
+        }
+        if (m < 0) __builtin_amdgcn_s_sleep(2);
+      }
+      if (m < 0) { ok = false; break; }
+      turn = (turn + 1) % KPER;
+      uint32_t* mb = mbox + (size_t)m * S::MB_WORDS;
+      T::put(cts, mb, lane);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) mb[T::NWORDS * 16 * 64 + q * 64 + lane] = bw[q];
+      if (lane == 0) {
+        PcDesc d;
+        d.rb = (int32_t)rb; d.cb = (int32_t)cb; d.bmin = bmin; d.bmax = bmax;
+        d.flags = (fast ? 1 : 0) | (dual ? 2 : 0); d.blo = blo; d.pad0 = d.pad1 = 0;
+        mdesc[m] = d;
+      }
+      pc_set(mstate + m, 1);
+      local = next;
+    }
+    // close this wave's mailboxes once they are empty
+#pragma unroll
+    for (int k = 0; k < KPER; ++k) {
+      const int mm = wave + 4 * k;
+      if (ok && pc_wait(mstate + mm, false) < 0) ok = false;
+      pc_set(mstate + mm, 2);
+    }
+  } else {
+    // ---- statistics wave m
+    const int m = wave - 4;
+    TileCtx t;
+    t.sedges = sedges;
+    t.srt = srt;
+    t.stg = sstage + m * nb * GS;
+    t.scw = scnt + m * nb;
+    t.slimb = slimb;
+    t.lane = lane;
+    const uint32_t* mb = mbox + (size_t)m * S::MB_WORDS;
+    for (;;) {
+      const int v = pc_wait(mstate + m, true);
+      if (v < 0) { ok = false; break; }
+      if (v == 2) break;
+      const int rb = __builtin_amdgcn_readfirstlane(mdesc[m].rb);
+      const int cb = __builtin_amdgcn_readfirstlane(mdesc[m].cb);
+      const int bmin = __builtin_amdgcn_readfirstlane(mdesc[m].bmin);
+      const int bmax = __builtin_amdgcn_readfirstlane(mdesc[m].bmax);
+      const int flags = __builtin_amdgcn_readfirstlane(mdesc[m].flags);
+      const int blo = __builtin_amdgcn_readfirstlane(mdesc[m].blo);
+      typename T::Counts cts;
+      T::get(cts, mb, lane);
+      uint32_t bw[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bw[q] = mb[T::NWORDS * 16 * 64 + q * 64 + lane];
+      pc_set(mstate + m, 0);
+      t.i0 = (int64_t)rb * 32;
+      t.j0 = (int64_t)cb * 32;
+      t.bmin = bmin;
+      t.bmax = bmax;
+      t.fast = flags & 1;
+      t.dual = (flags & 2) != 0;
+      t.blo = blo;
+      tile_groups<T, 0, true>(a, t, cts, bw);
+    }
+    stage_drain<T, 0, NS, T::template gsz<0>()>(t.stg, t.scw, slimb, lane, 0, nb - 1);
+  }
+  if (!ok && lane == 0) atomicAdd(a.out + (int64_t)NS * 3, 1ull << 62);
+  __syncthreads();
+  flush_global<NS>(slimb, a.out, nb);
+}
+
+// The producer/consumer form where its LDS plan fits in one launch (all bins
+// together); otherwise the per-wave form.
+template <class T>
+static int launch_fused_pc(const PanelLayout& L, const pgld_binned_args* g, hipStream_t s) {
+  using S = PcShape<T>;
+  constexpr size_t kLds = 160 * 1024;
+  FusedParams a;
+  size_t fixed = 0, per_bin = 0;
+  bool gtab = false;
+  int rc = fused_params<T>(L, g, S::NCONS, a, fixed, per_bin, gtab);
+  if (rc) return rc;
+  const size_t lds = fixed + (size_t)g->n_bins * per_bin + 256 + S::BYTES;
+  if (lds > kLds) return launch_fused_waves<T>(L, g, s);
+  a.use_ring = 0;
+  int grid = g->grid > 0 ? g->grid : num_cus();
+  grid = ((grid + 7) / 8) * 8;
+  // at most one workgroup per 16 tiles (4 per counting wave), a multiple of 8
+  const int64_t need = (((g->n_tiles + 15) / 16) + 7) / 8 * 8;
+  if (grid > need) grid = (int)need;
+  if (grid < 8) grid = 8;
+  const void* fn = gtab ? reinterpret_cast<const void*>(fused_pc_kernel<T, true>)
+                        : reinterpret_cast<const void*>(fused_pc_kernel<T, false>);
+  PGLD_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (gtab)
+    hipLaunchKernelGGL((fused_pc_kernel<T, true>), dim3(grid), dim3(S::THREADS), lds, s, a);
+  else
+    hipLaunchKernelGGL((fused_pc_kernel<T, false>), dim3(grid), dim3(S::THREADS), lds, s, a);
+  PGLD_LAUNCH_CHECK();
+  return PGLD_OK;
+}
+
+// form 1 (producer/consumer) for the traits that run two or more waves per
+// SIMD; form 0, or any other traits, the per-wave kernel.
+template <class T>
+static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStream_t s) {
+  if constexpr (T::MINB >= 2 && T::RING == 0) {
+    if (g->form == 1) return launch_fused_pc<T>(L, g, s);
+  }
+  return launch_fused_waves<T>(L, g, s);
 }
 
 // ---------------------------------------------------------------------------

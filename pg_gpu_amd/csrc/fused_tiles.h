@@ -266,7 +266,7 @@ __device__ __forceinline__ void ring_products(const v4i* A, const v4i* Av, const
 // Counts of population p for tile (rb, cb), packed per slot as
 // w0 = n11 | n10 << 16, w1 = n01 | n00 << 16.  With RING > 0 (and missing
 // calls) the operands stream through the wave's LDS ring.
-template <bool TWO_PASS, int RING = 0>
+template <bool TWO_PASS, int RING = 0, int PFM = PGLD_PF_MISSING>
 __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t rb, int64_t cb,
                                             int lane, v16u& w0, v16u& w1, v4i* ring = nullptr) {
   // MARG: slot words hold the products' marginals (n11 | pA << 16, pB | n << 16
@@ -295,7 +295,7 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
     const v4i* Av = A + ks * 64;
     const v4i* Bv = B + ks * 64;
     if constexpr (RING == 0) {
-      mfma_products4<PGLD_PF_MISSING>(A, Av, B, Bv, ks, hh, hv, vh, vv);
+      mfma_products4<PFM>(A, Av, B, Bv, ks, hh, hv, vh, vv);
     } else {
       // fallback of the LDS ring traits: the form their register allocation
       // was measured with
@@ -532,6 +532,28 @@ struct HapTraits {
     else return NS + q;
   }
   struct Counts { v16u w0[P], w1[P]; };  // slot words of tile_counts<., RING>
+  // Mailbox form of a tile's counts (the producer/consumer kernel): word k of
+  // slot r of lane l at mb[(16 k + r) * 64 + l], k = 2p (w0) or 2p + 1 (w1);
+  // every index a constant, so the counts never leave registers otherwise.
+  static constexpr int NWORDS = 2 * P;
+  __device__ __forceinline__ static void put(const Counts& c, uint32_t* mb, int lane) {
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        mb[((2 * p) * 16 + r) * 64 + lane] = c.w0[p][r];
+        mb[((2 * p + 1) * 16 + r) * 64 + lane] = c.w1[p][r];
+      }
+  }
+  __device__ __forceinline__ static void get(Counts& c, const uint32_t* mb, int lane) {
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        c.w0[p][r] = mb[((2 * p) * 16 + r) * 64 + lane];
+        c.w1[p][r] = mb[((2 * p + 1) * 16 + r) * 64 + lane];
+      }
+  }
   // population p's basis for slot r
   __device__ __forceinline__ static HapPop pop(const FusedParams& a, const double* srt,
                                                const Counts& c, int p, int r) {
@@ -551,6 +573,18 @@ struct HapTraits {
 #pragma unroll
     for (int p = 0; p < P; ++p)
       tile_counts<(MINB >= 3), RING>(a, p, rb, cb, lane, c.w0[p], c.w1[p], ring);
+  }
+  // the counting waves of the producer/consumer kernel hold no statistics
+  // state, so their operand ring runs PC_PF k-steps deep
+#ifndef PGLD_PC_PF_HAP
+#define PGLD_PC_PF_HAP 6
+#endif
+  static constexpr int PC_PF = PGLD_PC_PF_HAP;
+  __device__ __forceinline__ static void counts_pc(const FusedParams& a, int64_t rb, int64_t cb,
+                                                   int lane, Counts& c) {
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+      tile_counts<(MINB >= 3), 0, PC_PF>(a, p, rb, cb, lane, c.w0[p], c.w1[p], nullptr);
   }
   // statistics G*GS .. G*GS+GS-1 of the pair in accumulator slot r
   template <int G>
@@ -629,10 +663,38 @@ struct GenoTraits {
     else return NS + q;
   }
   struct Counts { v16u w[P][W]; };
+  // mailbox form (see HapTraits): word k = W p + q
+  static constexpr int NWORDS = P * W;
+  __device__ __forceinline__ static void put(const Counts& c, uint32_t* mb, int lane) {
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+      for (int q = 0; q < W; ++q)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) mb[((W * p + q) * 16 + r) * 64 + lane] = c.w[p][q][r];
+  }
+  __device__ __forceinline__ static void get(Counts& c, const uint32_t* mb, int lane) {
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+      for (int q = 0; q < W; ++q)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) c.w[p][q][r] = mb[((W * p + q) * 16 + r) * 64 + lane];
+  }
   __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane,
                                                 Counts& c, v4i*) {
 #pragma unroll
     for (int p = 0; p < P; ++p) geno_tile_counts<W, P == 1 ? 2 : 1>(a, p, rb, cb, lane, c.w[p]);
+  }
+  // counting waves of the producer/consumer kernel (see HapTraits)
+#ifndef PGLD_PC_PF_GENO
+#define PGLD_PC_PF_GENO 3
+#endif
+  static constexpr int PC_PF = PGLD_PC_PF_GENO;
+  __device__ __forceinline__ static void counts_pc(const FusedParams& a, int64_t rb, int64_t cb,
+                                                   int lane, Counts& c) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) geno_tile_counts<W, PC_PF>(a, p, rb, cb, lane, c.w[p]);
   }
   template <int G>
   __device__ __forceinline__ static void stats(const FusedParams& a, const double* srt, const Counts& c, int r,
@@ -843,9 +905,12 @@ __device__ __forceinline__ void tile_bins(const FusedParams& a, const TileCtx& t
 // One pass over the tile's 1024 pairs for statistic group G.  Single-bin
 // tiles are reduced to limbs at the tile end, so FP64 rounding never spans
 // more than one tile (and the result does not depend on the schedule).
-template <class T, int G>
+// PRE: the tile's bin words (tile_bins) were computed beforehand and are
+// passed in `pre` (the producer/consumer kernel: its counting wave does it).
+template <class T, int G, bool PRE = false>
 __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& t,
-                                           const typename T::Counts& cts) {
+                                           const typename T::Counts& cts,
+                                           const uint32_t (&pre)[4]) {
   // NS below is the output row width; GS the size of this group
   constexpr int NS = T::NSO;
   constexpr int GS = T::template gsz<G>();
@@ -874,7 +939,12 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
     // does the fast path's work plus one FMA per statistic, and the per-bin
     // pair counts are ballots.
     uint32_t bw[4];
-    tile_bins(a, t, bw);
+    if constexpr (PRE) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bw[q] = pre[q];
+    } else {
+      tile_bins(a, t, bw);
+    }
     double acc1[GS];
 #pragma unroll
     for (int q = 0; q < GS; ++q) acc1[q] = 0.0;
@@ -912,7 +982,12 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
     if (t.blo + 1 < nb) reduce_to_limbs<T, G, NS, GS, G == 0>(t, acc1, t.blo + 1, cnt1);
   } else {
     uint32_t bw[4];
-    tile_bins(a, t, bw);
+    if constexpr (PRE) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bw[q] = pre[q];
+    } else {
+      tile_bins(a, t, bw);
+    }
     int cur = -1, count = 0;
 #pragma unroll 1
     for (int r = 0; r < 16; ++r) {
@@ -942,12 +1017,13 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
   }
 }
 
-template <class T, int G>
+template <class T, int G, bool PRE = false>
 __device__ __forceinline__ void tile_groups(const FusedParams& a, const TileCtx& t,
-                                            const typename T::Counts& cts) {
+                                            const typename T::Counts& cts,
+                                            const uint32_t (&pre)[4]) {
   if constexpr (G < T::NGT) {
-    tile_group<T, G>(a, t, cts);
-    tile_groups<T, G + 1>(a, t, cts);
+    tile_group<T, G, PRE>(a, t, cts, pre);
+    tile_groups<T, G + 1, PRE>(a, t, cts, pre);
   }
 }
 

@@ -70,6 +70,10 @@ def decode_limbs(raw, n_stats):
     """int64 [n_bins, n_stats+1, 3] limbs -> (float64 sums [n_bins, n_stats], int64 counts)."""
     raw = np.asarray(raw, np.int64).reshape(-1, n_stats + 1, 3)
     nb = raw.shape[0]
+    # a producer/consumer launch whose mailbox handshake timed out adds 2^62
+    # to bin 0's pair count: no real count comes near it
+    if nb and raw[0, n_stats, 0] >= (1 << 60):
+        raise RuntimeError("fused kernel: a mailbox wait timed out; the result is invalid")
     sums = np.zeros((nb, n_stats))
     for b in range(nb):
         for s in range(n_stats):
@@ -168,7 +172,7 @@ class BinnedPlan:
                 self._n_pairs = int(per[np.repeat(sel, TILE)[:V]].sum())
         return self._n_pairs
 
-    def args(self, out, grid=0, r2=False):
+    def args(self, out, grid=0, r2=False, form=0):
         pan = self.pan
         a = _lib.BinnedArgs()
         a.d_panels = _lib.ptr(pan.panels).value if pan.panels is not None else None
@@ -192,20 +196,22 @@ class BinnedPlan:
         for p in range(pan.P):
             a.n_lo[p] = int(pan.n_lo[p])
         a.with_r2 = int(bool(r2))
+        a.form = int(form)
         return a
 
-    def run(self, kernel="fused", grid=0, r2=False, dynamic=True):
+    def run(self, kernel="fused", grid=0, r2=False, dynamic=True, form=0):
         """Launch one binning pass.  With ``r2`` (one population) the bins also
         receive the sum of r^2 and the number of pairs where r^2 is defined.
         ``dynamic`` selects the atomic-counter tile schedule of the fused
-        kernel (False: static stride); the result is the same either way."""
+        kernel (False: static stride) and ``form`` its producer/consumer form
+        (1) or per-wave form (0); the result is the same either way."""
         torch = _torch()
         pan = self.pan
         if r2 and pan.P != 1:
             raise NotImplementedError("r2 sums are built for one population")
         width = self.ns + (2 if r2 else 0)
         out = torch.zeros((self.nb, width + 1, 3), dtype=torch.int64, device=pan.device)
-        a = self.args(out, grid, r2)
+        a = self.args(out, grid, r2, form)
         sched = None
         if dynamic:
             sched = torch.zeros(_lib.SCHED_WORDS, dtype=torch.int32, device=pan.device)
@@ -216,11 +222,11 @@ class BinnedPlan:
         res._sched = sched  # kept alive until the launch has been waited for
         return res
 
-    def capture(self, kernel="fused", grid=0, r2=False, dynamic=True, repack=False):
+    def capture(self, kernel="fused", grid=0, r2=False, dynamic=True, repack=False, form=0):
         """Record one binning pass (optionally preceded by re-packing the
         panels) as a HIP graph; ``GraphRun.replay()`` then relaunches it with
         one call.  Buffers are cleared by fill kernels inside the graph."""
-        return GraphRun(self, kernel, grid, r2, dynamic, repack)
+        return GraphRun(self, kernel, grid, r2, dynamic, repack, form)
 
 
 class GraphRun:
@@ -228,7 +234,7 @@ class GraphRun:
     graph and rewritten by every replay, so a replay's result must be read
     (``finish()``) before the next replay."""
 
-    def __init__(self, plan, kernel, grid, r2, dynamic, repack):
+    def __init__(self, plan, kernel, grid, r2, dynamic, repack, form=0):
         torch = _torch()
         pan = plan.pan
         if r2 and pan.P != 1:
@@ -238,7 +244,7 @@ class GraphRun:
         self.out = torch.zeros((plan.nb, width + 1, 3), dtype=torch.int64, device=pan.device)
         self.sched = (torch.zeros(_lib.SCHED_WORDS, dtype=torch.int32, device=pan.device)
                       if dynamic else None)
-        self.args = plan.args(self.out, grid, r2)
+        self.args = plan.args(self.out, grid, r2, form)
         if self.sched is not None:
             self.args.d_sched = _lib.ptr(self.sched).value
         self.fn = _lib.lib().pgld_binned_fused if kernel == "fused" else \
@@ -402,7 +408,7 @@ class LDPanels:
         return BinnedPlan(self, positions, bins, max_dist, coords, tiles, square)
 
     def binned(self, positions, bins, max_dist=None, coords=None, kernel="fused", grid=0,
-               tiles=None, sync=True, r2=False):
+               tiles=None, sync=True, r2=False, form=0):
         """Bin sums of every LD statistic over pairs within ``max_dist``.
 
         positions: sorted site positions (the cutoff coordinate), numpy or tensor
@@ -413,7 +419,8 @@ class LDPanels:
         With ``sync=False`` the launch is left in flight and a PendingResult is
         returned; call ``.finish()`` on it to wait and decode.
         """
-        pend = self.plan(positions, bins, max_dist, coords, tiles).run(kernel, grid, r2)
+        pend = self.plan(positions, bins, max_dist, coords, tiles).run(kernel, grid, r2,
+                                                                       form=form)
         return pend.finish() if sync else pend
 
     def count_block(self, pop, i0, ni, j0, nj):

@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 REL = 1e-12
 
 
-def run_case(cuda, hap, pos, pops, bins, max_dist=None, coords=None, kernels=("fused", "plain")):
+def run_case(cuda, hap, pos, pops, bins, max_dist=None, coords=None, kernels=("fused", "fused_pc", "plain")):
     import torch
 
     print(f"case: hap {hap.shape} pops {[len(p) for p in pops]} bins {len(bins) - 1}", flush=True)
@@ -28,7 +28,9 @@ def run_case(cuda, hap, pos, pops, bins, max_dist=None, coords=None, kernels=("f
                                                coords=coords, with_abs=True)
     out = {}
     for k in kernels:
-        r = pan.binned(pos, bins, max_dist=max_dist, coords=coords, kernel=k)
+        # "fused_pc": the fused kernel's producer/consumer form
+        r = pan.binned(pos, bins, max_dist=max_dist, coords=coords,
+                       kernel="plain" if k == "plain" else "fused", form=int(k == "fused_pc"))
         torch.cuda.synchronize()
         np.testing.assert_array_equal(r.counts, cnt)
         assert r.n_pairs == n_enum
@@ -110,9 +112,10 @@ def test_bitwise_reproducible_across_grids(cuda):
     pan = LDPanels(torch.as_tensor(w.mat, device=cuda), w.pop_rows, "hap")
     base = pan.binned(w.positions, w.bins, w.max_dist, kernel="fused")
     for grid in (8, 64, 1024):
-        r = pan.binned(w.positions, w.bins, w.max_dist, kernel="fused", grid=grid)
-        np.testing.assert_array_equal(r.sums, base.sums)
-        np.testing.assert_array_equal(r.counts, base.counts)
+        for form in (0, 1):
+            r = pan.binned(w.positions, w.bins, w.max_dist, kernel="fused", grid=grid, form=form)
+            np.testing.assert_array_equal(r.sums, base.sums)
+            np.testing.assert_array_equal(r.counts, base.counts)
 
 
 @pytest.mark.parametrize("missing", [0.0, 0.07])
@@ -200,8 +203,8 @@ def check_r2(res, ref, n_ld, base):
     np.testing.assert_array_equal(res.counts, base.counts)
 
 
-@pytest.mark.parametrize("kernel", ["fused", "plain"])
-def test_single_pop_mean_r2(cuda, kernel):
+@pytest.mark.parametrize("kernel,form", [("fused", 0), ("fused", 1), ("plain", 0)])
+def test_single_pop_mean_r2(cuda, kernel, form):
     """Workload c0 shape: DD, Dz, pi2 and mean r^2 per bin."""
     import torch
 
@@ -211,8 +214,8 @@ def test_single_pop_mean_r2(cuda, kernel):
     pan = LDPanels(torch.as_tensor(hap, device=cuda), w.pop_rows, "hap")
     ref, cnt, _ = o.binned_sums_hap(hap, w.positions, w.pop_rows, w.bins, max_dist=w.max_dist,
                                     r2=True)
-    res = pan.binned(w.positions, w.bins, w.max_dist, kernel=kernel, r2=True)
-    base = pan.binned(w.positions, w.bins, w.max_dist, kernel=kernel)
+    res = pan.binned(w.positions, w.bins, w.max_dist, kernel=kernel, r2=True, form=form)
+    base = pan.binned(w.positions, w.bins, w.max_dist, kernel=kernel, form=form)
     np.testing.assert_array_equal(res.counts, cnt)
     check_r2(res, ref, 3, base)
     assert np.all(res.r2_count <= res.counts) and res.r2_count.sum() < res.counts.sum()
@@ -279,12 +282,15 @@ def test_graph_replays_single_pop_r2(cuda):
     pan = LDPanels(torch.as_tensor(hap, device=cuda), pops, "hap")
     plan = pan.plan(pos, bins, 50_000)
     eager = plan.run(r2=True).finish()
-    g = plan.capture(r2=True)
-    for _ in range(3):
-        got = g.replay().finish()
-        np.testing.assert_array_equal(got.sums, eager.sums)
-        np.testing.assert_array_equal(got.r2_sum, eager.r2_sum)
-        np.testing.assert_array_equal(got.r2_count, eager.r2_count)
+    for form in (0, 1):
+        g = plan.capture(r2=True, form=form)
+        for _ in range(3):
+            got = g.replay().finish()
+            np.testing.assert_array_equal(got.sums, eager.sums)
+            np.testing.assert_array_equal(got.r2_sum, eager.r2_sum)
+            np.testing.assert_array_equal(got.r2_count, eager.r2_count)
+            torch.cuda.synchronize()
+            _ = (torch.arange(1000, device=cuda) * 3).sum().item()
 
 
 def test_autotuned_launch_shape_gives_identical_sums(cuda):
@@ -298,12 +304,12 @@ def test_autotuned_launch_shape_gives_identical_sums(cuda):
     pan = LDPanels(torch.as_tensor(hap, device=cuda), pops, "hap")
     autotune.clear_cache()
     best, timings = autotune.tune_binned(pan, pos, bins, 80_000, reps=1, squares=(4, 16))
-    assert len(timings) == 4 and best in [t[0] for t in timings]
+    assert len(timings) == 8 and best in [t[0] for t in timings]
     assert all(ms > 0 for _, ms in timings)
     again, t2 = autotune.tune_binned(pan, pos, bins, 80_000)
     assert again == best and t2 == []              # cached per job shape
     ref = pan.plan(pos, bins, 80_000).run().finish()
     got = pan.plan(pos, bins, 80_000, square=best["square"]).run(
-        grid=best["grid"], dynamic=best["dynamic"]).finish()
+        grid=best["grid"], dynamic=best["dynamic"], form=best["form"]).finish()
     np.testing.assert_array_equal(got.sums, ref.sums)
     np.testing.assert_array_equal(got.counts, ref.counts)

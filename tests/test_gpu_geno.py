@@ -19,7 +19,7 @@ REL = 1e-12
 
 
 def run_geno(cuda, geno, pos, pops, bins, max_dist=None, coords=None,
-             kernels=("fused", "plain")):
+             kernels=("fused", "fused_pc", "plain")):
     import torch
 
     print(f"case: geno {geno.shape} pops {[len(p) for p in pops]} bins {len(bins) - 1}",
@@ -30,7 +30,9 @@ def run_geno(cuda, geno, pos, pops, bins, max_dist=None, coords=None,
                                                 coords=coords, with_abs=True)
     out = {}
     for k in kernels:
-        r = pan.binned(pos, bins, max_dist=max_dist, coords=coords, kernel=k)
+        # "fused_pc": the fused kernel's producer/consumer form
+        r = pan.binned(pos, bins, max_dist=max_dist, coords=coords,
+                       kernel="plain" if k == "plain" else "fused", form=int(k == "fused_pc"))
         torch.cuda.synchronize()
         np.testing.assert_array_equal(r.counts, cnt)
         assert r.n_pairs == n_enum
@@ -165,8 +167,8 @@ def test_geno_three_and_four_pops(cuda, sizes, missing):
     run_geno(cuda, geno, pos, pops, synth.log_bins(100, 8_000, 4), 8_000.0)
 
 
-@pytest.mark.parametrize("kernel", ["fused", "plain"])
-def test_geno_mean_r2_c2_scaled(cuda, kernel):
+@pytest.mark.parametrize("kernel,form", [("fused", 0), ("fused", 1), ("plain", 0)])
+def test_geno_mean_r2_c2_scaled(cuda, kernel, form):
     """Workload c2 shape: genotype DD, Dz, pi2 and mean dosage r^2 per bin."""
     import torch
 
@@ -176,6 +178,6 @@ def test_geno_mean_r2_c2_scaled(cuda, kernel):
     pan = LDPanels(torch.as_tensor(w.mat, device=cuda), w.pop_rows, "geno")
     ref, cnt, _ = o.binned_sums_geno(w.mat, w.positions, w.pop_rows, w.bins,
                                      max_dist=w.max_dist, r2=True)
-    res = pan.binned(w.positions, w.bins, w.max_dist, kernel=kernel, r2=True)
+    res = pan.binned(w.positions, w.bins, w.max_dist, kernel=kernel, r2=True, form=form)
     np.testing.assert_array_equal(res.counts, cnt)
-    check_r2(res, ref, 3, pan.binned(w.positions, w.bins, w.max_dist, kernel=kernel))
+    check_r2(res, ref, 3, pan.binned(w.positions, w.bins, w.max_dist, kernel=kernel, form=form))
