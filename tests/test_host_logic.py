@@ -379,3 +379,30 @@ def test_estimate_ld_chunk_size_hand_values():
     assert estimate_ld_chunk_size(10, 10**12, 1) == 10_000_000       # ceiling
     # H = 1000, P = 4: (16,000 + 600) * 3 = 49,800
     assert estimate_ld_chunk_size(1000, 49_800 * 123_456, 4) == 123_456
+
+
+def header_struct_fields(name):
+    """Field names of `typedef struct { ... } name;` in include/pgld.h."""
+    text = (ROOT / "include/pgld.h").read_text()
+    end = re.search(r"\}\s*" + name + ";", text).start()
+    body = text[text.rindex("typedef struct {", 0, end) + len("typedef struct {"):end]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    return re.findall(r"(\w+)\s*(?:\[[^\]]*\])?\s*;", body)
+
+
+@pytest.mark.parametrize("cname,pyname", [("pgld_binned_args", "BinnedArgs"),
+                                          ("pgld_window_args", "WindowArgs"),
+                                          ("pgld_panel_desc", "PanelDesc")])
+def test_ctypes_structs_match_the_header(cname, pyname):
+    """The ctypes mirrors of the argument structs list the header's fields in
+    the header's order (a missing or reordered field shifts every later one)."""
+    from pg_gpu_amd import _lib
+
+    want = header_struct_fields(cname)
+    got = [f[0] for f in getattr(_lib, pyname)._fields_]
+    assert got == want
+    if pyname in ("BinnedArgs", "PanelDesc"):
+        doc = (ROOT / "INTEGRATION.md").read_text()
+        stub = re.search(r"class " + pyname + r"\(ctypes.Structure\):\n\s*_fields_ = \[(.*?)\]\n",
+                         doc, flags=re.S).group(1)
+        assert re.findall(r'\("(\w+)"', stub) == want
