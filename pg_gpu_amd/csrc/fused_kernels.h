@@ -24,12 +24,6 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
   int* scnt = reinterpret_cast<int*>(slimb + nb * (NS + 1) * 3);    // [4 waves][nb]
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  // this wave's operand ring (16-byte aligned, after the tables)
-  v4i* ring = nullptr;
-  if (T::RING > 0 && a.use_ring) {
-    const uintptr_t end = reinterpret_cast<uintptr_t>(scnt + 4 * nb);
-    ring = reinterpret_cast<v4i*>((end + 15) & ~uintptr_t(15)) + wave * (T::RING_BYTES / 16);
-  }
   for (int e = tid; e <= nb; e += blockDim.x) sedges[e] = a.edges[e];
   if (!GTAB) {
     for (int p = 0; p < T::NP; ++p) {
@@ -83,7 +77,7 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
     const double d_min = coord_diff(a, i1, j0), d_max = coord_diff(a, i0, j1);
     const bool in_cut = a.pos[j1] <= a.pos[i0] + a.max_dist;
     typename T::Counts cts;
-    T::counts(a, rb, cb, lane, cts, ring);
+    T::counts(a, rb, cb, lane, cts);
     const int64_t next = ctr ? waves_per_xcd + (int64_t)__builtin_amdgcn_readfirstlane(__shfl(claim, 0, 64))
                              : local + waves_per_xcd;
     if (seg_lo + next < seg_hi) {
@@ -208,11 +202,7 @@ static int launch_fused_waves(const PanelLayout& L, const pgld_binned_args* g, h
     c.edges = g->d_edges + b0;
     c.out = a.out + (int64_t)b0 * (NS + 1) * 3;
     if (n_launch > 1) c.sched = nullptr;
-    size_t lds = fixed + (size_t)c.n_bins * per_bin;
-    // the operand ring only where the tables leave room for it
-    const size_t ring_bytes = T::RING > 0 ? 16 + (size_t)4 * T::RING_BYTES : 0;
-    c.use_ring = ring_bytes > 0 && lds + ring_bytes <= kLds;
-    if (c.use_ring) lds += ring_bytes;
+    const size_t lds = fixed + (size_t)c.n_bins * per_bin;
     const void* fn = gtab ? reinterpret_cast<const void*>(fused_kernel<T, true>)
                           : reinterpret_cast<const void*>(fused_kernel<T, false>);
     PGLD_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -443,7 +433,6 @@ static int launch_fused_pc(const PanelLayout& L, const pgld_binned_args* g, hipS
   if (rc) return rc;
   const size_t lds = fixed + (size_t)g->n_bins * per_bin + 256 + S::BYTES;
   if (lds > kLds) return launch_fused_waves<T>(L, g, s);
-  a.use_ring = 0;
   int grid = g->grid > 0 ? g->grid : num_cus();
   grid = ((grid + 7) / 8) * 8;
   // at most one workgroup per 16 tiles (4 per counting wave), a multiple of 8
@@ -465,7 +454,7 @@ static int launch_fused_pc(const PanelLayout& L, const pgld_binned_args* g, hipS
 // SIMD; form 0, or any other traits, the per-wave kernel.
 template <class T>
 static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStream_t s) {
-  if constexpr (T::MINB >= 2 && T::RING == 0) {
+  if constexpr (T::MINB >= 2) {
     if (g->form == 1) return launch_fused_pc<T>(L, g, s);
   }
   return launch_fused_waves<T>(L, g, s);
@@ -483,11 +472,8 @@ static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStre
 template <class T>
 __global__ void __launch_bounds__(256, T::MINB) count_band_kernel(FusedParams a,
                                                                   unsigned long long* sums) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  v4i* ring = nullptr;
-  if (T::RING > 0 && a.use_ring) ring = reinterpret_cast<v4i*>(smem) + wave * (T::RING_BYTES / 16);
   constexpr int P = T::NP;
   unsigned long long acc[P][4];
 #pragma unroll
@@ -510,7 +496,7 @@ __global__ void __launch_bounds__(256, T::MINB) count_band_kernel(FusedParams a,
     uint32_t claim = 0;
     if (ctr && lane == 0) claim = atomicAdd(ctr, 1u);
     typename T::Counts cts;
-    T::counts(a, rb, cb, lane, cts, ring);
+    T::counts(a, rb, cb, lane, cts);
     const int64_t next = ctr ? waves_per_xcd + (int64_t)__builtin_amdgcn_readfirstlane(__shfl(claim, 0, 64))
                              : local + waves_per_xcd;
     if (seg_lo + next < seg_hi) {
@@ -562,18 +548,12 @@ static int launch_count_band(const PanelLayout& L, const pgld_binned_args* g,
     a.missing[p] = L.missing[p];
     a.K[p] = L.pop_size[p];
   }
-  const size_t lds = T::RING > 0 ? (size_t)4 * T::RING_BYTES : 0;
-  a.use_ring = T::RING > 0;
-  if (lds) {
-    PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(count_band_kernel<T>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  }
   int grid = g->grid > 0 ? g->grid : T::MINB * num_cus();
   grid = ((grid + 7) / 8) * 8;
   const int64_t need = ((g->n_tiles + 31) / 32) * 8;
   if (grid > need) grid = (int)need;
   if (grid < 8) grid = 8;
-  hipLaunchKernelGGL(count_band_kernel<T>, dim3(grid), dim3(256), lds, s, a, sums);
+  hipLaunchKernelGGL(count_band_kernel<T>, dim3(grid), dim3(256), 0, s, a, sums);
   PGLD_LAUNCH_CHECK();
   return PGLD_OK;
 }
@@ -728,7 +708,7 @@ __global__ void __launch_bounds__(256, T::MINB) window_kernel(WindowParams w) {
     }
     const int64_t lo = w.win[2 * win], hi = w.win[2 * win + 1];
     typename T::Counts cts;
-    T::counts(a, rb, cb, lane, cts, nullptr);  // one population: no LDS ring
+    T::counts(a, rb, cb, lane, cts);
     const int64_t i0 = rb * 32, j0 = cb * 32;
     const bool full = cb > rb && i0 >= lo && j0 + 32 <= hi;
     window_item<T>(w, srt, cts, lane, full, i0, j0, lo, hi, L0, L1, L2);

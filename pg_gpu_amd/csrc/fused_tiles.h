@@ -33,7 +33,6 @@ struct FusedParams {
   int32_t rt_off[PGLD_MAX_POPS];  // first entry of pop p in the LDS table
   int32_t rt_len;                 // total LDS table entries
   uint32_t* sched;                // per-XCD tile counters (null: static schedule)
-  int32_t use_ring;               // stream operands through the LDS ring (traits with RING)
 };
 
 // 16 packed count words per lane (one per MFMA output slot).  Kept as a
@@ -194,81 +193,12 @@ __device__ __forceinline__ void mfma_products9(const v4i* A, const v4i* B, int f
   }
 }
 
-// Operand ring in LDS: each wave streams its tile's fragments through a
-// private ring of RING k-steps x 4 fragments (1 KiB each) with
-// global_load_lds (lane l's 16 bytes land at byte 16 l of the slot), so loads
-// run RING k-steps ahead without holding registers.  The LDS writes of these
-// loads are tracked by vmcnt only, hence the explicit waits.
-typedef __attribute__((address_space(1))) void gmem_void;
-typedef __attribute__((address_space(3))) void lmem_void;
-
-__device__ __forceinline__ void lds_dma16(const v4i* g_lane, v4i* lds_slot) {
-  __builtin_amdgcn_global_load_lds((gmem_void*)g_lane, (lmem_void*)lds_slot, 16, 0, 0);
-}
-// Wait until at most n (0, 4, 8, 12 or 16) vector-memory operations of this
-// wave are outstanding; also a compiler barrier for memory accesses.
-__device__ __forceinline__ void wait_vm(int n) {
-  if (n >= 16) asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
-  else if (n >= 12) asm volatile("s_waitcnt vmcnt(12)" ::: "memory");
-  else if (n >= 8) asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
-  else if (n >= 4) asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
-  else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-}
-
-// The four fragments of one ring slot (1 KiB apart).  Read in one asm block
-// that waits for its own results: the compiler cannot tell these reads from
-// the in-flight ring loads of later k-steps and would otherwise wait for all
-// of them (vmcnt(0)) before every read.
-__device__ __forceinline__ void ring_read4(const v4i* slot_lane, v4i& f0, v4i& f1, v4i& f2,
-                                           v4i& f3) {
-  typedef const __attribute__((address_space(3))) v4i lds_v4i;
-  const uint32_t addr = (uint32_t)(uintptr_t)(lds_v4i*)slot_lane;  // LDS byte address
-  asm volatile(
-      "ds_read_b128 %0, %4\n\t"
-      "ds_read_b128 %1, %4 offset:1024\n\t"
-      "ds_read_b128 %2, %4 offset:2048\n\t"
-      "ds_read_b128 %3, %4 offset:3072\n\t"
-      "s_waitcnt lgkmcnt(0)"
-      : "=&v"(f0), "=&v"(f1), "=&v"(f2), "=&v"(f3)
-      : "v"(addr)
-      : "memory");
-}
-
-// hh, hv, vh, vv over the ks k-steps, operands streamed through the ring
-template <int RING>
-__device__ __forceinline__ void ring_products(const v4i* A, const v4i* Av, const v4i* B,
-                                              const v4i* Bv, int ks, int lane, v4i* ring,
-                                              v16f& hh, v16f& hv, v16f& vh, v16f& vv) {
-  const v4i* src[4] = {A, Av, B, Bv};
-  const int pre = ks < RING ? ks : RING;
-  for (int d = 0; d < pre; ++d) {
-#pragma unroll
-    for (int f = 0; f < 4; ++f) lds_dma16(src[f] + d * 64, ring + (d * 4 + f) * 64);
-  }
-  for (int s = 0; s < ks; ++s) {
-    const int ahead = ks - 1 - s < RING - 1 ? ks - 1 - s : RING - 1;  // later k-steps in flight
-    wait_vm(4 * ahead);
-    v4i ah, av, bh, bv;
-    ring_read4(ring + (s % RING) * 4 * 64 + lane, ah, av, bh, bv);
-    hh = pgld_mfma_fp4(ah, bh, hh);
-    hv = pgld_mfma_fp4(ah, bv, hv);
-    vh = pgld_mfma_fp4(av, bh, vh);
-    vv = pgld_mfma_fp4(av, bv, vv);
-    if (s + RING < ks) {
-      asm volatile("" ::: "memory");
-      v4i* dst = ring + (s % RING) * 4 * 64;
-#pragma unroll
-      for (int f = 0; f < 4; ++f) lds_dma16(src[f] + (s + RING) * 64, dst + f * 64);
-    }
-  }
-}
-
 // Counts of population p for tile (rb, cb), packed per slot as
-// w0 = n11 | n10 << 16, w1 = n01 | n00 << 16.  With RING > 0 (and missing
-// calls) the operands stream through the wave's LDS ring.
-template <bool TWO_PASS, int RING = 0, int PFM = PGLD_PF_MISSING>
+// w0 = n11 | n10 << 16, w1 = n01 | n00 << 16 (or the marginals, below); PFM
+// k-steps of operand loads in flight with missing calls.
+template <bool TWO_PASS, int PFM = PGLD_PF_MISSING>
 __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t rb, int64_t cb,
-                                            int lane, v16u& w0, v16u& w1, v4i* ring = nullptr) {
+                                            int lane, v16u& w0, v16u& w1) {
   // MARG: slot words hold the products' marginals (n11 | pA << 16, pB | n << 16
   // with pA = n11 + n10, pB = n11 + n01, n the valid pairs); otherwise the
   // four class counts (n11 | n10 << 16, n01 | n00 << 16).  Measured per
@@ -280,39 +210,9 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
   const int ks = a.ksteps[p];
   const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[p]) + lane;
   const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[p]) + lane;
-  if (RING > 0 && ring != nullptr && a.missing[p] && !TWO_PASS) {
-    v16f hh = zero16f(), hv = zero16f(), vh = zero16f(), vv = zero16f();
-    ring_products<(RING > 0 ? RING : 1)>(A, A + ks * 64, B, B + ks * 64, ks, lane, ring, hh, hv,
-                                         vh, vv);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int n11 = (int)hh[r], h_v = (int)hv[r], v_h = (int)vh[r], v_v = (int)vv[r];
-      w0[r] = (uint32_t)n11 | ((uint32_t)h_v << 16);
-      w1[r] = (uint32_t)v_h | ((uint32_t)v_v << 16);
-    }
-  } else if (a.missing[p] && !TWO_PASS) {
+  if (a.missing[p] && !TWO_PASS) {
     v16f hh, hv, vh, vv;
-    const v4i* Av = A + ks * 64;
-    const v4i* Bv = B + ks * 64;
-    if constexpr (RING == 0) {
-      mfma_products4<PFM>(A, Av, B, Bv, ks, hh, hv, vh, vv);
-    } else {
-      // fallback of the LDS ring traits: the form their register allocation
-      // was measured with
-      hh = hv = vh = vv = zero16f();
-      v4i ah, av, bh, bv;
-      if (ks > 0) { ah = A[0]; av = Av[0]; bh = B[0]; bv = Bv[0]; }
-      for (int s = 0; s < ks; ++s) {
-        hh = pgld_mfma_fp4(ah, bh, hh);
-        hv = pgld_mfma_fp4(ah, bv, hv);
-        vh = pgld_mfma_fp4(av, bh, vh);
-        vv = pgld_mfma_fp4(av, bv, vv);
-        if (s + 1 < ks) {
-          const int n = (s + 1) * 64;
-          ah = A[n]; av = Av[n]; bh = B[n]; bv = Bv[n];
-        }
-      }
-    }
+    mfma_products4<PFM>(A, A + ks * 64, B, B + ks * 64, ks, hh, hv, vh, vv);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int n11 = (int)hh[r], h_v = (int)hv[r], v_h = (int)vh[r], v_v = (int)vv[r];
@@ -509,9 +409,6 @@ struct HapTraits {
                             (MERGE_R2 ? 2 : 0);  // per group
   static constexpr int NSO = NS + (R2 ? 2 : 0);
   static constexpr int NGT = NG + (R2 && !MERGE_R2 ? 1 : 0);
-  // k-steps of the LDS operand ring per wave (0: register ring)
-  static constexpr int RING = P >= 4 ? 4 : 0;
-  static constexpr int RING_BYTES = RING * 4 * 1024;  // per wave
   // the group's sums hold statistic q times 2^e (raw numerators; 0 past the
   // statistics and for the r^2 slots)
   template <int G>
@@ -531,7 +428,7 @@ struct HapTraits {
     if constexpr (G < NG) return hap_stat_col<P, G>(q);
     else return NS + q;
   }
-  struct Counts { v16u w0[P], w1[P]; };  // slot words of tile_counts<., RING>
+  struct Counts { v16u w0[P], w1[P]; };  // slot words of tile_counts
   // Mailbox form of a tile's counts (the producer/consumer kernel): word k of
   // slot r of lane l at mb[(16 k + r) * 64 + l], k = 2p (w0) or 2p + 1 (w1);
   // every index a constant, so the counts never leave registers otherwise.
@@ -569,10 +466,9 @@ struct HapTraits {
     }
   }
   __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane,
-                                                Counts& c, v4i* ring) {
+                                                Counts& c) {
 #pragma unroll
-    for (int p = 0; p < P; ++p)
-      tile_counts<(MINB >= 3), RING>(a, p, rb, cb, lane, c.w0[p], c.w1[p], ring);
+    for (int p = 0; p < P; ++p) tile_counts<(MINB >= 3)>(a, p, rb, cb, lane, c.w0[p], c.w1[p]);
   }
   // the counting waves of the producer/consumer kernel hold no statistics
   // state, so their operand ring runs PC_PF k-steps deep
@@ -584,7 +480,7 @@ struct HapTraits {
                                                    int lane, Counts& c) {
 #pragma unroll
     for (int p = 0; p < P; ++p)
-      tile_counts<(MINB >= 3), 0, PC_PF>(a, p, rb, cb, lane, c.w0[p], c.w1[p], nullptr);
+      tile_counts<(MINB >= 3), PC_PF>(a, p, rb, cb, lane, c.w0[p], c.w1[p]);
   }
   // statistics G*GS .. G*GS+GS-1 of the pair in accumulator slot r
   template <int G>
@@ -648,8 +544,6 @@ struct GenoTraits {
                             (MERGE_R2 ? 2 : 0);
   static constexpr int NSO = NS + (R2 ? 2 : 0);
   static constexpr int NGT = NG + (R2 && !MERGE_R2 ? 1 : 0);
-  static constexpr int RING = 0;
-  static constexpr int RING_BYTES = 0;
   template <int G>
   __device__ __forceinline__ static int scale_exp(int) { return 0; }
   template <int G>
@@ -682,7 +576,7 @@ struct GenoTraits {
         for (int r = 0; r < 16; ++r) c.w[p][q][r] = mb[((W * p + q) * 16 + r) * 64 + lane];
   }
   __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane,
-                                                Counts& c, v4i*) {
+                                                Counts& c) {
 #pragma unroll
     for (int p = 0; p < P; ++p) geno_tile_counts<W, P == 1 ? 2 : 1>(a, p, rb, cb, lane, c.w[p]);
   }

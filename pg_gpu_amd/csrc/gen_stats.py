@@ -252,11 +252,10 @@ HAP_INNER = {
     "hap_pi2_ijij": 0.0625, "hap_pi2_shared": 0.0625, "hap_pi2_alldiff": 0.25,
 }
 HAP_AVG_INNER = 0.5
-# population counts whose groups accumulate raw numerators; with four
-# populations the scaled form compiles to fewer spilled registers (measured on
-# 4 populations of 40-800 haplotypes over 500k sites: 169 ms scaled, 175 ms
-# raw; after the marginal slot words 163 vs 179 ms)
-HAP_FOLD_P = (1, 2, 3)
+# population counts whose groups accumulate raw numerators: every count since
+# the population-set passes (4 populations of 40-800 haplotypes over 500k
+# sites: 159.1 ms scaled, 137.3 ms raw, both with the register operand ring)
+HAP_FOLD_P = (1, 2, 3, 4)
 
 
 def _log2_exact(x):
