@@ -5,9 +5,14 @@
 namespace pgld {
 
 int launch_binned_hap(const PanelLayout& L, const pgld_binned_args* args, hipStream_t s) {
-  if (args->with_r2) return launch_fused<HapTraits<1, true>>(L, args, s);  // one population
+  // one population without missing calls: constant reciprocals (HapNMTraits)
+  const bool nm = L.n_pops == 1 && !L.missing[0];
+  if (args->with_r2)  // one population
+    return nm ? launch_fused<HapNMTraits<true>>(L, args, s)
+              : launch_fused<HapTraits<1, true>>(L, args, s);
   switch (L.n_pops) {
-    case 1: return launch_fused<HapTraits<1>>(L, args, s);
+    case 1: return nm ? launch_fused<HapNMTraits<false>>(L, args, s)
+                      : launch_fused<HapTraits<1>>(L, args, s);
     case 2: return launch_fused<HapTraits<2>>(L, args, s);
     case 3: return launch_fused<HapTraits<3>>(L, args, s);
     case 4: return launch_fused<HapTraits<4>>(L, args, s);
@@ -29,7 +34,8 @@ int launch_count_band_hap(const PanelLayout& L, const pgld_binned_args* args,
 }
 
 int launch_windows_hap(const PanelLayout& L, const pgld_window_args* args, hipStream_t s) {
-  return launch_windows<HapTraits<1, true>>(L, args, s);
+  return L.missing[0] ? launch_windows<HapTraits<1, true>>(L, args, s)
+                      : launch_windows<HapNMTraits<true>>(L, args, s);
 }
 
 }  // namespace pgld

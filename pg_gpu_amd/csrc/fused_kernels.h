@@ -37,6 +37,7 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
   for (int e = tid; e < 4 * nb; e += blockDim.x) scnt[e] = 0;
   __syncthreads();
   TileCtx t;
+  t.fp = &a;
   t.sedges = sedges;
   t.srt = srt;
   t.stg = sstage + wave * nb * GS;
@@ -101,7 +102,7 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
     local = next;
   }
   // nothing may stay staged
-  stage_drain<T, 0, NS, T::template gsz<0>()>(t.stg, t.scw, slimb, lane, 0, nb - 1);
+  stage_drain<T, 0, NS, T::template gsz<0>()>(a, srt, t.stg, t.scw, slimb, lane, 0, nb - 1);
   __syncthreads();
   flush_global<NS>(slimb, a.out, nb);
 }
@@ -312,6 +313,7 @@ __global__ void __launch_bounds__(PcShape<T>::THREADS, 1) fused_pc_kernel(FusedP
       ncb = a.tiles[2 * (seg_lo + local) + 1];
     }
     TileCtx t;
+    t.fp = &a;
     t.sedges = sedges;
     t.lane = lane;
     int turn = 0;
@@ -381,6 +383,7 @@ __global__ void __launch_bounds__(PcShape<T>::THREADS, 1) fused_pc_kernel(FusedP
     // ---- statistics wave m
     const int m = wave - 4;
     TileCtx t;
+    t.fp = &a;
     t.sedges = sedges;
     t.srt = srt;
     t.stg = sstage + m * nb * GS;
@@ -413,7 +416,7 @@ __global__ void __launch_bounds__(PcShape<T>::THREADS, 1) fused_pc_kernel(FusedP
       t.blo = blo;
       tile_groups<T, 0, true>(a, t, cts, bw);
     }
-    stage_drain<T, 0, NS, T::template gsz<0>()>(t.stg, t.scw, slimb, lane, 0, nb - 1);
+    stage_drain<T, 0, NS, T::template gsz<0>()>(a, srt, t.stg, t.scw, slimb, lane, 0, nb - 1);
   }
   if (!ok && lane == 0) atomicAdd(a.out + (int64_t)NS * 3, 1ull << 62);
   __syncthreads();
@@ -454,7 +457,7 @@ static int launch_fused_pc(const PanelLayout& L, const pgld_binned_args* g, hipS
 // SIMD; form 0, or any other traits, the per-wave kernel.
 template <class T>
 static int launch_fused(const PanelLayout& L, const pgld_binned_args* g, hipStream_t s) {
-  if constexpr (T::MINB >= 2) {
+  if constexpr (T::MINB >= 2 && T::PC_OK) {
     if (g->form == 1) return launch_fused_pc<T>(L, g, s);
   }
   return launch_fused_waves<T>(L, g, s);
@@ -578,14 +581,17 @@ struct WindowParams {
 // unconditional add per limb (a conditional target would put the limbs in
 // scratch memory).
 template <class T, int NV, int LOG>
-__device__ __forceinline__ void window_add_limbs(double tot, int lane, long long& L0,
+__device__ __forceinline__ void window_add_limbs(const FusedParams& a, const double* srt,
+                                                 double tot, int lane, long long& L0,
                                                  long long& L1, long long& L2) {
   constexpr int GS = T::template gsz<0>();
   const int q = pgld_bitrev(lane & (NV - 1), LOG);
   const bool stat = lane < NV && q < GS;
   const bool cnt = lane < NV && q == GS;
   long long x2 = 0, x1 = 0, x0 = 0;
-  if (stat) pgld_to_limbs(tot, x2, x1, x0, pgld_limb_scale(T::template scale_exp<0>(q)));
+  if (stat)
+    pgld_to_limbs(tot * T::template stat_mult<0>(a, srt, q), x2, x1, x0,
+                  pgld_limb_scale(T::template scale_exp<0>(q)));
   L0 += x0 + (long long)(cnt ? tot : 0.0);
   L1 += x1;
   L2 += x2;
@@ -631,7 +637,7 @@ __device__ __forceinline__ void window_item(const WindowParams& w, const double*
   double v[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) v[q] = q < GS ? acc[q] : (q == GS ? (double)count : 0.0);
-  window_add_limbs<T, NV, LOG>(pgld_wave_sums<NV>(v, lane), lane, L0, L1, L2);
+  window_add_limbs<T, NV, LOG>(w.f, srt, pgld_wave_sums<NV>(v, lane), lane, L0, L1, L2);
 }
 
 // Adds a lane's running totals of window `win` to the output.
@@ -835,7 +841,7 @@ __device__ __forceinline__ void window_item16(const WindowParams& w, const doubl
   double v[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) v[q] = q < GS ? acc[q] : (q == GS ? (double)count : 0.0);
-  window_add_limbs<T, NV, 3>(pgld_wave_sums<NV>(v, lane), lane, L0, L1, L2);
+  window_add_limbs<T, NV, 3>(w.f, srt, pgld_wave_sums<NV>(v, lane), lane, L0, L1, L2);
 }
 
 // The window kernel over 16-site work items (blocks of 16 sites in d_work).
@@ -919,7 +925,9 @@ static int launch_windows(const PanelLayout& L, const pgld_window_args* g, hipSt
   if (grid > need) grid = (int)need;
   if (grid < 8) grid = 8;
   if (g->tile == 16) {
-    if constexpr (std::is_same<T, HapTraits<1, true>>::value) {
+    // the 16-site kernel covers haplotypes with and without missing calls
+    if constexpr (std::is_same<T, HapTraits<1, true>>::value ||
+                  std::is_same<T, HapNMTraits<true>>::value) {
       if (!gtab)
         PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(window16_kernel<false>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

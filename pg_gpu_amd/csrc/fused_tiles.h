@@ -193,6 +193,55 @@ __device__ __forceinline__ void mfma_products9(const v4i* A, const v4i* B, int f
   }
 }
 
+// Counts of a population without missing calls: n11 = H.H from the MFMAs,
+// the marginals from the site tallies.  MARG packs n11 | ki << 16 and
+// kj | K << 16 (ki, kj the allele counts of the row and column sites), else
+// the four class counts.  Returns kj.
+template <bool MARG>
+__device__ __forceinline__ int tile_counts_nomiss(const FusedParams& a, int p, int64_t rb,
+                                                  int64_t cb, int lane, const v4i* A,
+                                                  const v4i* B, int ks, v16u& w0, v16u& w1) {
+  v16f hh = zero16f();
+  constexpr int PF = 2;
+  v4i ah[PF], bh[PF];
+#pragma unroll
+  for (int d = 0; d < PF; ++d)
+    if (d < ks) { ah[d] = A[d * 64]; bh[d] = B[d * 64]; }
+  for (int s0 = 0; s0 < ks; s0 += PF) {
+#pragma unroll
+    for (int d = 0; d < PF; ++d) {
+      const int s = s0 + d;
+      if (s < ks) {
+        hh = pgld_mfma_fp4(ah[d], bh[d], hh);
+        if (s + PF < ks) { ah[d] = A[(s + PF) * 64]; bh[d] = B[(s + PF) * 64]; }
+      }
+    }
+  }
+  const int32_t* cnt1 = a.site_stats + (int64_t)p * 3 * a.V;
+  const int64_t j = cb * 32 + (lane & 31);
+  const int kj = j < a.V ? cnt1[j] : 0;
+  const int K = a.K[p];
+  // row i = ib + o, o = mfma_row(lane, r) - 4 (lane >> 5) a constant per r:
+  // immediate load offsets from one base, o < lim <=> i < V
+  const int64_t ib = rb * 32 + 4 * (lane >> 5);
+  const int32_t* ci = cnt1 + ib;
+  const int lim = (int)(a.V - ib < 32 ? a.V - ib : 32);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int o = 8 * (r >> 2) + (r & 3);
+    const int ki = o < lim ? ci[o] : 0;
+    const int n11 = (int)hh[r];
+    if constexpr (MARG) {
+      w0[r] = (uint32_t)n11 | ((uint32_t)ki << 16);
+      w1[r] = (uint32_t)kj | ((uint32_t)K << 16);
+    } else {
+      w0[r] = (uint32_t)n11 | ((uint32_t)(ki - n11) << 16);
+      w1[r] = (uint32_t)(kj - n11) | ((uint32_t)(K - ki - kj + n11) << 16);
+    }
+  }
+  return kj;
+}
+
 // Counts of population p for tile (rb, cb), packed per slot as
 // w0 = n11 | n10 << 16, w1 = n01 | n00 << 16 (or the marginals, below); PFM
 // k-steps of operand loads in flight with missing calls.
@@ -248,44 +297,7 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
       }
     }
   } else {
-    v16f hh = zero16f();
-    constexpr int PF = 2;
-    v4i ah[PF], bh[PF];
-#pragma unroll
-    for (int d = 0; d < PF; ++d)
-      if (d < ks) { ah[d] = A[d * 64]; bh[d] = B[d * 64]; }
-    for (int s0 = 0; s0 < ks; s0 += PF) {
-#pragma unroll
-      for (int d = 0; d < PF; ++d) {
-        const int s = s0 + d;
-        if (s < ks) {
-          hh = pgld_mfma_fp4(ah[d], bh[d], hh);
-          if (s + PF < ks) { ah[d] = A[(s + PF) * 64]; bh[d] = B[(s + PF) * 64]; }
-        }
-      }
-    }
-    const int32_t* cnt1 = a.site_stats + (int64_t)p * 3 * a.V;
-    const int64_t j = cb * 32 + (lane & 31);
-    const int kj = j < a.V ? cnt1[j] : 0;
-    const int K = a.K[p];
-    // row i = ib + o, o = mfma_row(lane, r) - 4 (lane >> 5) a constant per r:
-    // immediate load offsets from one base, o < lim <=> i < V
-    const int64_t ib = rb * 32 + 4 * (lane >> 5);
-    const int32_t* ci = cnt1 + ib;
-    const int lim = (int)(a.V - ib < 32 ? a.V - ib : 32);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int o = 8 * (r >> 2) + (r & 3);
-      const int ki = o < lim ? ci[o] : 0;
-      const int n11 = (int)hh[r];
-      if constexpr (MARG) {
-        w0[r] = (uint32_t)n11 | ((uint32_t)ki << 16);
-        w1[r] = (uint32_t)kj | ((uint32_t)K << 16);
-      } else {
-        w0[r] = (uint32_t)n11 | ((uint32_t)(ki - n11) << 16);
-        w1[r] = (uint32_t)(kj - n11) | ((uint32_t)(K - ki - kj + n11) << 16);
-      }
-    }
+    tile_counts_nomiss<MARG>(a, p, rb, cb, lane, A, B, ks, w0, w1);
   }
 }
 
@@ -416,6 +428,12 @@ struct HapTraits {
     if constexpr (G < NG) return hap_scale_group<P, G>(q);
     else return 0;
   }
+  // factor applied to a sum before it becomes limbs (sums here are values)
+  template <int G>
+  __device__ __forceinline__ static double stat_mult(const FusedParams&, const double*, int) {
+    return 1.0;
+  }
+  static constexpr bool PC_OK = true;   // has the producer/consumer form
   // statistics of pass G, and the output column of its q-th (the r^2 pair
   // after the group's statistics, or the r^2 group, lands at NS, NS + 1)
   template <int G>
@@ -524,6 +542,77 @@ struct HapTraits {
   }
 };
 
+// One haplotype population without missing calls.  Every pair has n = K
+// valid samples, so the reciprocal falling factorials are constants: each
+// tile's statistics are accumulated as exact integer numerators (below 2^53
+// for K below ~2,800 with 1,024 pairs per tile; above that the sums round
+// like any FP64 sum) and multiplied by 1/(K)_4 once per sum, where it becomes
+// limbs (stat_mult).  The column site's allele counts are tile constants and
+// a slot word holds n11 | ki << 16 only.
+template <bool R2 = false>
+struct HapNMTraits : HapTraits<1, R2> {
+  using Base = HapTraits<1, R2>;
+  static constexpr bool PC_OK = false;
+  // no class counts or reciprocal lookups: four waves per SIMD fit (<= 128
+  // registers)
+  static constexpr int MINB = 4;
+  struct Counts {
+    v16u w0[1], w1[1];     // n11 | ki << 16 (w1 unused)
+    double pB, qB, uB, PB, n;
+  };
+  __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb,
+                                                int lane, Counts& c) {
+    const int ks = a.ksteps[0];
+    const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[0]) + lane;
+    const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[0]) + lane;
+    const int kj = tile_counts_nomiss<true>(a, 0, rb, cb, lane, A, B, ks, c.w0[0], c.w1[0]);
+    c.n = pgld_count_f64(a.K[0]);
+    c.pB = pgld_count_f64(kj);
+    c.qB = c.n - c.pB;
+    c.uB = c.qB - c.pB;
+    c.PB = c.pB * c.qB;
+  }
+  // 1/(K)_4 for the three LD sums (raw numerators), 1 for the r^2 pair
+  template <int G>
+  __device__ __forceinline__ static double stat_mult(const FusedParams& a, const double* srt, int q) {
+    return q < 3 ? rt_entry(a, srt, 0, a.K[0])[3] : 1.0;
+  }
+  // the pair of slot r (counts, its basis fields)
+  __device__ __forceinline__ static HapPop pop(const Counts& c, int r) {
+    const uint32_t w = c.w0[0][r];
+    HapPop p;
+    p.c1 = pgld_count_f64((int)(w & 0xffffu));
+    p.pA = pgld_count_f64((int)(w >> 16));
+    p.n = c.n; p.pB = c.pB; p.qB = c.qB; p.uB = c.uB; p.PB = c.PB;
+    p.qA = p.n - p.pA;
+    p.c2 = p.pA - p.c1; p.c3 = p.pB - p.c1; p.c4 = p.qA - p.c3;
+    p.x = p.c1 * p.c4; p.y = p.c2 * p.c3;
+    p.D = p.y - p.x;
+    p.s14 = p.c1 + p.c4; p.s23 = p.c2 + p.c3;
+    p.uA = p.qA - p.pA;
+    p.PA = p.pA * p.qA;
+    p.w = fma(p.uA, p.uB, p.s23 - p.s14);
+    p.r1 = p.r2 = p.r3 = p.r4 = 0.0;   // not used: the sums are scaled by stat_mult
+    return p;
+  }
+  // acc[q] += wt * (raw numerator of statistic q; r^2 and its defined flag)
+  template <int G>
+  __device__ __forceinline__ static void accum(const FusedParams&, const double*, const Counts& c,
+                                               int r, double* acc, double wt) {
+    static_assert(Base::NGT == 1, "one statistic pass");
+    const HapPop p = pop(c, r);
+    acc[0] = fma(hap_dd_same_num(p), wt, acc[0]);
+    acc[1] = fma(hap_dz_same_num(p), wt, acc[1]);
+    acc[2] = fma(hap_pi2_same_num(p), wt, acc[2]);
+    if constexpr (R2) {
+      double x[2];
+      hap_r2(p, x);
+      acc[3] = fma(x[0], wt, acc[3]);
+      acc[4] = fma(x[1], wt, acc[4]);
+    }
+  }
+};
+
 // W = 3: nine 10-bit counts in three words per slot (population sizes up to
 // 1023); W = 5: 16-bit counts in five words (up to 32767).
 template <int P, int W, bool R2 = false>
@@ -546,6 +635,11 @@ struct GenoTraits {
   static constexpr int NGT = NG + (R2 && !MERGE_R2 ? 1 : 0);
   template <int G>
   __device__ __forceinline__ static int scale_exp(int) { return 0; }
+  template <int G>
+  __device__ __forceinline__ static double stat_mult(const FusedParams&, const double*, int) {
+    return 1.0;
+  }
+  static constexpr bool PC_OK = true;
   template <int G>
   __device__ static constexpr int gsz() {
     if constexpr (G < NG) return geno_group_size<P, G>() + (MERGE_R2 ? 2 : 0);
@@ -659,8 +753,9 @@ __device__ __forceinline__ void stage_add(double* stg, int* scnt, int bin, const
 // Move staged bins [blo, bhi] of this wave into the workgroup limbs of
 // statistics s0 .. s0+GS-1 (and the pair counters).
 template <class T, int G, int NS, int GS>
-__device__ __forceinline__ void stage_drain(double* stg, int* scnt, unsigned long long* slimb,
-                                            int lane, int blo, int bhi) {
+__device__ __forceinline__ void stage_drain(const FusedParams& a, const double* srt, double* stg,
+                                            int* scnt, unsigned long long* slimb, int lane,
+                                            int blo, int bhi) {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   const int E = (bhi - blo + 1) * (GS + 1);
@@ -672,7 +767,8 @@ __device__ __forceinline__ void stage_drain(double* stg, int* scnt, unsigned lon
         stg[b * GS + q] = 0.0;
         unsigned long long* dst = slimb + ((int64_t)b * (NS + 1) + T::template col<G>(q)) * 3;
         long long x2, x1, x0;
-        pgld_to_limbs(v, x2, x1, x0, pgld_limb_scale(T::template scale_exp<G>(q)));
+        pgld_to_limbs(v * T::template stat_mult<G>(a, srt, q), x2, x1, x0,
+                      pgld_limb_scale(T::template scale_exp<G>(q)));
         atomicAdd(dst + 0, (unsigned long long)x0);
         atomicAdd(dst + 1, (unsigned long long)x1);
         atomicAdd(dst + 2, (unsigned long long)x2);
@@ -715,6 +811,7 @@ __device__ __forceinline__ double coord_diff(const FusedParams& a, int64_t i, in
 }
 
 struct TileCtx {
+  const FusedParams* fp;
   const double* sedges;
   const double* srt;
   double* stg;   // this wave's staging [nb][GS]
@@ -748,7 +845,8 @@ __device__ __forceinline__ void reduce_to_limbs(const TileCtx& t, double (&acc)[
   if (lane < NV && q < GS) {
     unsigned long long* dst = t.slimb + ((int64_t)bin * (NS + 1) + T::template col<G>(q)) * 3;
     long long x2, x1, x0;
-    pgld_to_limbs(tot, x2, x1, x0, pgld_limb_scale(T::template scale_exp<G>(q)));
+    pgld_to_limbs(tot * T::template stat_mult<G>(*t.fp, t.srt, q), x2, x1, x0,
+                  pgld_limb_scale(T::template scale_exp<G>(q)));
     atomicAdd(dst + 0, (unsigned long long)x0);
     atomicAdd(dst + 1, (unsigned long long)x1);
     atomicAdd(dst + 2, (unsigned long long)x2);
@@ -906,7 +1004,7 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
       }
     }
     if (cur >= 0) stage_add<GS, G == 0>(t.stg, t.scw, cur, acc, count);
-    stage_drain<T, G, NS, GS>(t.stg, t.scw, t.slimb, lane, t.bmin < 0 ? 0 : t.bmin,
+    stage_drain<T, G, NS, GS>(a, t.srt, t.stg, t.scw, t.slimb, lane, t.bmin < 0 ? 0 : t.bmin,
                               t.bmax >= nb ? nb - 1 : t.bmax);
   }
 }
