@@ -59,6 +59,15 @@ def _time(plan, cfg, r2, reps):
     return best
 
 
+def pc_form_available(pan):
+    """Whether the library has the producer/consumer form for this panel set
+    (one haplotype population with missing calls, two haplotype populations,
+    one genotype population); elsewhere it runs form 0 whatever is asked."""
+    if pan.mode == 0:
+        return (pan.P == 1 and bool(pan.has_missing[0])) or pan.P == 2
+    return pan.P == 1
+
+
 def tune_binned(pan, positions, bins, max_dist=None, coords=None, r2=False, reps=3,
                 squares=SQUARES, grids=(0,), forms=(0, 1), use_cache=True):
     """-> (best config dict(square, dynamic, grid, form), [(config, ms)...] measured).
@@ -68,6 +77,7 @@ def tune_binned(pan, positions, bins, max_dist=None, coords=None, r2=False, reps
     if use_cache and key in _cache:
         return dict(_cache[key]), []
     timings = []
+    forms = [f for f in forms if f == 0 or pc_form_available(pan)] or [0]
     for sq in squares:
         plan = BinnedPlan(pan, positions, bins, max_dist, coords, square=sq)
         for dyn in (True, False):
