@@ -302,13 +302,18 @@ __device__ __forceinline__ void tile_counts(const FusedParams& a, int p, int64_t
 }
 
 // ---------------------------------------------------------------------------
-// Genotype tile counts.  Features per sample: b = [g==1], c = [g==2] and, with
-// missing calls, v = [g>=0].  The 9 products give the 9 class counts:
-//   n11 = b.b, n12 = b.c, n21 = c.b, n22 = c.c, n10 = b.v - n11 - n12,
-//   n20 = c.v - n21 - n22, n01 = v.b - n11 - n21, n02 = v.c - n12 - n22,
-//   n00 = v.v - (b.v + c.v) - n01 - n02.
-// Packed per slot in moments order g1..g9 (n22, n21, n20, n12, n11, n10, n02,
-// n01, n00) as five 16-bit pairs.
+// Genotype tile counts.  Features per sample (pgld_pack): u = g - 1 in
+// {-1, 0, 1}, q = u^2 and, with missing calls, v = [g >= 0] (u = q = 0 where
+// g is missing).  Over the samples valid at both sites the nine products are
+// the pair's centred dosage moments (row site first):
+//   u.u = ab11, u.q = ab12, q.u = ab21, q.q = ab22,
+//   u.v = a1, q.v = a2, v.u = b1, v.q = b2, v.v = n0.
+// Without missing calls the marginal sums come from the site tallies
+// (a1 = 2 n2 + n1 - K, a2 = K - n1, with n1, n2 the site's counts of
+// genotypes 1 and 2).  One population keeps the moments (signed fields);
+// several turn them into the nine class counts (moments order n22 n21 n20
+// n12 n11 n10 n02 n01 n00) with the exact inverse transform
+// [g=2] = (q + u)/2, [g=0] = (q - u)/2, [g=1] = v - q.
 // ---------------------------------------------------------------------------
 // Pack the nine class counts of slot r (moments order) into w[.][r].
 template <int W>
@@ -325,37 +330,88 @@ __device__ __forceinline__ void geno_pack_slot(v16u (&w)[W], int r, const int (&
   }
 }
 
-template <int W, int PF>
+// Pack the moments of slot r: W = 3 (|values| <= 1023) as 11-bit signed and
+// 10-bit unsigned fields, W = 5 as 16-bit fields (a1, b1 and the ab_kl with a
+// first power of u signed).
+template <int W>
+__device__ __forceinline__ void geno_pack_moments(v16u (&w)[W], int r, const GenoMoments& m) {
+  if constexpr (W == 3) {
+    w[0][r] = ((uint32_t)m.ab11 & 0x7ffu) | (((uint32_t)m.ab12 & 0x7ffu) << 11) |
+              ((uint32_t)m.a2 << 22);
+    w[1][r] = ((uint32_t)m.a1 & 0x7ffu) | (((uint32_t)m.ab21 & 0x7ffu) << 11) |
+              ((uint32_t)m.ab22 << 22);
+    w[2][r] = ((uint32_t)m.b1 & 0x7ffu) | ((uint32_t)m.b2 << 11) | ((uint32_t)m.n0 << 21);
+  } else {
+    w[0][r] = ((uint32_t)m.ab11 & 0xffffu) | ((uint32_t)m.ab12 << 16);
+    w[1][r] = ((uint32_t)m.a1 & 0xffffu) | ((uint32_t)m.ab21 << 16);
+    w[2][r] = ((uint32_t)m.b1 & 0xffffu) | ((uint32_t)m.a2 << 16);
+    w[3][r] = (uint32_t)m.ab22 | ((uint32_t)m.b2 << 16);
+    w[4][r] = (uint32_t)m.n0;
+  }
+}
+
+__device__ __forceinline__ int geno_sfield(uint32_t w, int off, int bits) {
+  return ((int)(w << (32 - off - bits))) >> (32 - bits);
+}
+
+// The moments of slot r packed by geno_pack_moments.
+template <int W>
+__device__ __forceinline__ GenoMoments geno_unpack_moments(const v16u (&w)[W], int r) {
+  GenoMoments m;
+  if constexpr (W == 3) {
+    const uint32_t x = w[0][r], y = w[1][r], z = w[2][r];
+    m.ab11 = geno_sfield(x, 0, 11); m.ab12 = geno_sfield(x, 11, 11); m.a2 = (int)(x >> 22);
+    m.a1 = geno_sfield(y, 0, 11); m.ab21 = geno_sfield(y, 11, 11); m.ab22 = (int)(y >> 22);
+    m.b1 = geno_sfield(z, 0, 11); m.b2 = (int)((z >> 11) & 0x3ffu); m.n0 = (int)(z >> 21);
+  } else {
+    const uint32_t x = w[0][r], y = w[1][r], z = w[2][r], t = w[3][r];
+    m.ab11 = geno_sfield(x, 0, 16); m.ab12 = geno_sfield(x, 16, 16);
+    m.a1 = geno_sfield(y, 0, 16); m.ab21 = geno_sfield(y, 16, 16);
+    m.b1 = geno_sfield(z, 0, 16); m.a2 = (int)(z >> 16);
+    m.ab22 = (int)(t & 0xffffu); m.b2 = (int)(t >> 16);
+    m.n0 = (int)w[4][r];
+  }
+  return m;
+}
+
+// CENTRED: pack the moments (one population); else the class counts.
+template <int W, int PF, bool CENTRED>
 __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, int64_t rb,
                                                  int64_t cb, int lane, v16u (&w)[W]) {
   const int ks = a.ksteps[p];
   const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[p]) + lane;
   const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[p]) + lane;
   const int fs = ks * 64;  // feature stride in v4i
-  v16f bb, bc, cbm, cc;
+  auto emit = [&](int r, const GenoMoments& m) {
+    if constexpr (CENTRED) {
+      geno_pack_moments(w, r, m);
+    } else {
+      int c[9];
+      geno_class_counts(m, c);
+      geno_pack_slot(w, r, c);
+    }
+  };
   if (a.missing[p]) {
     v16f acc[9];
     mfma_products9<PF>(A, B, fs, ks, acc);
-    bb = acc[0]; bc = acc[1];
-    const v16f bv = acc[2];
-    cbm = acc[3]; cc = acc[4];
-    const v16f cv = acc[5], vb = acc[6], vc = acc[7], vv = acc[8];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int n11 = (int)bb[r], n12 = (int)bc[r], n21 = (int)cbm[r], n22 = (int)cc[r];
-      const int b_v = (int)bv[r], c_v = (int)cv[r];
-      const int n10 = b_v - n11 - n12, n20 = c_v - n21 - n22;
-      const int n01 = (int)vb[r] - n11 - n21, n02 = (int)vc[r] - n12 - n22;
-      const int n00 = (int)vv[r] - b_v - c_v - n01 - n02;
-      const int c[9] = {n22, n21, n20, n12, n11, n10, n02, n01, n00};
-      geno_pack_slot(w, r, c);
+      GenoMoments m;
+      m.ab11 = (int)acc[0][r]; m.ab12 = (int)acc[1][r]; m.a1 = (int)acc[2][r];
+      m.ab21 = (int)acc[3][r]; m.ab22 = (int)acc[4][r]; m.a2 = (int)acc[5][r];
+      m.b1 = (int)acc[6][r]; m.b2 = (int)acc[7][r]; m.n0 = (int)acc[8][r];
+      emit(r, m);
     }
   } else {
-    mfma_products4<PGLD_PF_MISSING>(A, A + fs, B, B + fs, ks, bb, bc, cbm, cc);
+    v16f uu, uq, qu, qq;
+    mfma_products4<PGLD_PF_MISSING>(A, A + fs, B, B + fs, ks, uu, uq, qu, qq);
     const int32_t* st = a.site_stats + (int64_t)p * 3 * a.V;
     const int64_t j = cb * 32 + (lane & 31);
-    const int vbj = j < a.V ? st[j] : 0, vcj = j < a.V ? st[a.V + j] : 0;
     const int K = a.K[p];
+    // a padded site (past the last one) counts as all heterozygous (u = q = 0,
+    // matching its zero products), so every class count stays non-negative
+    const int n1j = j < a.V ? st[j] : K, n2j = j < a.V ? st[a.V + j] : 0;
+    const int b1 = 2 * n2j + n1j - K, b2 = K - n1j;
     // rows as in tile_counts: i = ib + o with o constant per r
     const int64_t ib = rb * 32 + 4 * (lane >> 5);
     const int32_t* si = st + ib;
@@ -364,13 +420,12 @@ __device__ __forceinline__ void geno_tile_counts(const FusedParams& a, int p, in
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int o = 8 * (r >> 2) + (r & 3);
-      const int bvi = o < lim ? si[o] : 0, cvi = o < lim ? sci[o] : 0;
-      const int n11 = (int)bb[r], n12 = (int)bc[r], n21 = (int)cbm[r], n22 = (int)cc[r];
-      const int n10 = bvi - n11 - n12, n20 = cvi - n21 - n22;
-      const int n01 = vbj - n11 - n21, n02 = vcj - n12 - n22;
-      const int n00 = K - bvi - cvi - n01 - n02;
-      const int c[9] = {n22, n21, n20, n12, n11, n10, n02, n01, n00};
-      geno_pack_slot(w, r, c);
+      const int n1i = o < lim ? si[o] : K, n2i = o < lim ? sci[o] : 0;
+      GenoMoments m;
+      m.ab11 = (int)uu[r]; m.ab12 = (int)uq[r]; m.ab21 = (int)qu[r]; m.ab22 = (int)qq[r];
+      m.a1 = 2 * n2i + n1i - K; m.a2 = K - n1i;
+      m.b1 = b1; m.b2 = b2; m.n0 = K;
+      emit(r, m);
     }
   }
 }
@@ -622,6 +677,9 @@ struct GenoTraits {
   // One population fits two waves per SIMD; more need the full register file
   // of a one-wave-per-SIMD launch.
   static constexpr int MINB = P == 1 ? 2 : 1;
+  // one population keeps the centred moments in its slot words; several keep
+  // the class counts (their cross-population estimators use them)
+  static constexpr bool CENTRED = P == 1;
   static constexpr int NS = P == 1 ? GENO_NSTATS_P1 : P == 2 ? GENO_NSTATS_P2 : P == 3 ? GENO_NSTATS_P3
                                                                                         : GENO_NSTATS_P4;
   static constexpr int NG = P == 1 ? GENO_NGROUPS_P1 : P == 2 ? GENO_NGROUPS_P2 : P == 3 ? GENO_NGROUPS_P3
@@ -672,7 +730,7 @@ struct GenoTraits {
   __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane,
                                                 Counts& c) {
 #pragma unroll
-    for (int p = 0; p < P; ++p) geno_tile_counts<W, P == 1 ? 2 : 1>(a, p, rb, cb, lane, c.w[p]);
+    for (int p = 0; p < P; ++p) geno_tile_counts<W, P == 1 ? 2 : 1, CENTRED>(a, p, rb, cb, lane, c.w[p]);
   }
   // counting waves of the producer/consumer kernel (see HapTraits)
 #ifndef PGLD_PC_PF_GENO
@@ -682,14 +740,22 @@ struct GenoTraits {
   __device__ __forceinline__ static void counts_pc(const FusedParams& a, int64_t rb, int64_t cb,
                                                    int lane, Counts& c) {
 #pragma unroll
-    for (int p = 0; p < P; ++p) geno_tile_counts<W, PC_PF>(a, p, rb, cb, lane, c.w[p]);
+    for (int p = 0; p < P; ++p) geno_tile_counts<W, PC_PF, CENTRED>(a, p, rb, cb, lane, c.w[p]);
   }
   template <int G>
   __device__ __forceinline__ static void stats(const FusedParams& a, const double* srt, const Counts& c, int r,
                                double* x) {
     GenoPop pp[P];
+    if constexpr (CENTRED) {
+      // the slot words hold the moments themselves (the one-population
+      // estimators and r^2 use nothing else)
+      const GenoMoments m = geno_unpack_moments<W>(c.w[0], r);
+      const int y[9] = {m.n0, m.b1, m.b2, m.a1, m.ab11, m.ab12, m.a2, m.ab21, m.ab22};
+      const int g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      pp[0] = geno_pop(g, y, 0, rt_entry(a, srt, 0, m.n0));
+    }
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
+    for (int p = 0; p < (CENTRED ? 0 : P); ++p) {
       int g[9];
       if constexpr (W == 3) {
 #pragma unroll

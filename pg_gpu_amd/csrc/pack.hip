@@ -264,22 +264,31 @@ __global__ void __launch_bounds__(256) pack_kernel(
     uint32_t x[8];  // samples s0 + 4j .. s0 + 4j + 3, s0 = 32 (lane >> 5)
 #pragma unroll
     for (int e = 0; e < 8; ++e) x[e] = src[e];
-    // feature code: hap {0: [v==1], 1: [v>=0]}; geno {0: [v==1], 1: [v==2], 2: [v>=0]}
+    // feature code: hap {0: [v==1], 1: [v>=0]}; geno (centred dosages)
+    // {0: u = v - 1 in {-1, 0, 1}, 1: q = u^2, 2: [v>=0]}, u and q 0 where
+    // missing, so the products of the features are the centred dosage moments
     const bool valid_f = f == n_feat - 1 && f > 0 && (mode == PGLD_MODE_HAP ? f == 1 : f == 2);
-    const uint32_t c = (mode == PGLD_MODE_GENO && f == 1) ? 0x02020202u : 0x01010101u;
+    const bool geno = mode == PGLD_MODE_GENO;
     uint32_t word[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      uint32_t m0, m1;
+      // nibble i of word q <-> sample s0 + 8q + i; 1.0 in E2M1 is 0x2, -1.0 0xA
       if (valid_f) {
-        m0 = (~x[2 * q] & 0x80808080u) >> 7;
-        m1 = (~x[2 * q + 1] & 0x80808080u) >> 7;
+        const uint32_t m0 = (~x[2 * q] & 0x80808080u) >> 7;
+        const uint32_t m1 = (~x[2 * q + 1] & 0x80808080u) >> 7;
+        word[q] = (flags_to_nibbles(m0) | (flags_to_nibbles(m1) << 16)) * PGLD_FP4_ONE;
+      } else if (geno) {
+        const uint32_t z = flags_to_nibbles(bytes_eq(x[2 * q], 0u)) |
+                           (flags_to_nibbles(bytes_eq(x[2 * q + 1], 0u)) << 16);
+        const uint32_t t = flags_to_nibbles(bytes_eq(x[2 * q], 0x02020202u)) |
+                           (flags_to_nibbles(bytes_eq(x[2 * q + 1], 0x02020202u)) << 16);
+        // per nibble at most one of z, t is set: no carries
+        word[q] = f == 0 ? t * PGLD_FP4_ONE + z * PGLD_FP4_NEG_ONE : (z | t) * PGLD_FP4_ONE;
       } else {
-        m0 = bytes_eq(x[2 * q], c);
-        m1 = bytes_eq(x[2 * q + 1], c);
+        const uint32_t m0 = bytes_eq(x[2 * q], 0x01010101u);
+        const uint32_t m1 = bytes_eq(x[2 * q + 1], 0x01010101u);
+        word[q] = (flags_to_nibbles(m0) | (flags_to_nibbles(m1) << 16)) * PGLD_FP4_ONE;
       }
-      // nibble i of word q <-> sample s0 + 8q + i; 1.0 in E2M1 is 0x2
-      word[q] = (flags_to_nibbles(m0) | (flags_to_nibbles(m1) << 16)) * PGLD_FP4_ONE;
     }
     int8_t* dst = panels + (blk0 + bl) * block_bytes + pop_off +
                   ((int64_t)f * ksteps + ks) * PGLD_FRAG_BYTES + lane * 16;

@@ -119,17 +119,20 @@ __global__ void __launch_bounds__(64) pair_block_geno_kernel(PairBlockArgs a) {
   for (int r = 0; r < 16; ++r) {
     const int64_t i = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     if (i >= a.i0 + a.ni || j >= a.j0 + a.nj || i >= a.V || j >= a.V) continue;
-    // sums over the individuals valid at both sites (features b = [g==1],
-    // c = [g==2], v = [g>=0]; dosage x = b + 2c)
-    const double bb = pr[0][0][r], bc = pr[0][1][r], cb_ = pr[1][0][r], cc = pr[1][1][r];
-    double bv, cv, vb, vc, vv;
+    // sums over the individuals valid at both sites from the centred
+    // feature products (u = x - 1, v = [x >= 0]; dosage x = u + 1):
+    // sum x = u.v + v.v, sum x y = u.u + u.v + v.u + v.v
+    const double uu = pr[0][0][r];
+    double uv, vu, vv;
     if (a.missing) {
-      bv = pr[0][2][r]; cv = pr[1][2][r]; vb = pr[2][0][r]; vc = pr[2][1][r]; vv = pr[2][2][r];
+      uv = pr[0][2][r]; vu = pr[2][0][r]; vv = pr[2][2][r];
     } else {
-      bv = a.st[i]; cv = a.st[a.V + i]; vb = a.st[j]; vc = a.st[a.V + j]; vv = a.K;
+      uv = 2.0 * a.st[a.V + i] + a.st[i] - a.K;
+      vu = 2.0 * a.st[a.V + j] + a.st[j] - a.K;
+      vv = a.K;
     }
-    const double sx = bv + 2.0 * cv, sy = vb + 2.0 * vc;
-    const double sxy = bb + 2.0 * (bc + cb_) + 4.0 * cc;
+    const double sx = uv + vv, sy = vu + vv;
+    const double sxy = uu + uv + vu + vv;
     // per-site means and sums of squares over each site's valid individuals
     const double ni_v = (double)(a.K - a.st[2 * a.V + i]), nj_v = (double)(a.K - a.st[2 * a.V + j]);
     const double si = a.st[i] + 2.0 * a.st[a.V + i], sj = a.st[j] + 2.0 * a.st[a.V + j];

@@ -96,6 +96,63 @@ __device__ __forceinline__ void feature_products(const int8_t* panels, int64_t b
     for (int b = 0; b < NF; ++b) prod[a][b] >>= 2;
 }
 
+// 0/1 nibble masks of 4-bit E2M1 feature values: where the value is 1.0
+// (0x2), -1.0 (0xA), or either.
+__device__ __forceinline__ uint32_t nib_pos(uint32_t w) { return (w >> 1) & ~(w >> 3) & 0x11111111u; }
+__device__ __forceinline__ uint32_t nib_neg(uint32_t w) { return (w >> 1) & (w >> 3) & 0x11111111u; }
+__device__ __forceinline__ uint32_t nib_one(uint32_t w) { return (w >> 1) & 0x11111111u; }
+__device__ __forceinline__ uint4 map4(uint4 a, uint32_t (*f)(uint32_t)) {
+  return make_uint4(f(a.x), f(a.y), f(a.z), f(a.w));
+}
+// Dot product of 32 0/1 nibbles.
+__device__ __forceinline__ int dot32m(uint4 a, uint4 b, int acc) {
+  acc = __builtin_amdgcn_udot8(a.x, b.x, acc, false);
+  acc = __builtin_amdgcn_udot8(a.y, b.y, acc, false);
+  acc = __builtin_amdgcn_udot8(a.z, b.z, acc, false);
+  acc = __builtin_amdgcn_udot8(a.w, b.w, acc, false);
+  return acc;
+}
+
+// The centred moments of genotype sites i and j from the packed features
+// u (signed: the products split into the +1 and -1 parts), q and v.
+__device__ __forceinline__ GenoMoments geno_pair_moments(const int8_t* panels, int64_t block_bytes,
+                                                         int64_t pop_off, int ks, bool missing,
+                                                         int64_t i, int64_t j) {
+  int pp = 0, nn = 0, pn = 0, np_ = 0;          // u.u parts
+  int pq = 0, nq = 0, qp = 0, qn = 0, qq = 0;   // u.q, q.u, q.q parts
+  int pv = 0, nv = 0, qv = 0, vp = 0, vn = 0, vq = 0, vv = 0;
+  for (int s = 0; s < ks; ++s) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint4 ui = *site_half(panels, block_bytes, pop_off, ks, 0, s, i, h);
+      const uint4 uj = *site_half(panels, block_bytes, pop_off, ks, 0, s, j, h);
+      const uint4 Qi = map4(*site_half(panels, block_bytes, pop_off, ks, 1, s, i, h), nib_one);
+      const uint4 Qj = map4(*site_half(panels, block_bytes, pop_off, ks, 1, s, j, h), nib_one);
+      const uint4 Pi = map4(ui, nib_pos), Ni = map4(ui, nib_neg);
+      const uint4 Pj = map4(uj, nib_pos), Nj = map4(uj, nib_neg);
+      pp = dot32m(Pi, Pj, pp); nn = dot32m(Ni, Nj, nn);
+      pn = dot32m(Pi, Nj, pn); np_ = dot32m(Ni, Pj, np_);
+      pq = dot32m(Pi, Qj, pq); nq = dot32m(Ni, Qj, nq);
+      qp = dot32m(Qi, Pj, qp); qn = dot32m(Qi, Nj, qn);
+      qq = dot32m(Qi, Qj, qq);
+      if (missing) {
+        const uint4 Vi = map4(*site_half(panels, block_bytes, pop_off, ks, 2, s, i, h), nib_one);
+        const uint4 Vj = map4(*site_half(panels, block_bytes, pop_off, ks, 2, s, j, h), nib_one);
+        pv = dot32m(Pi, Vj, pv); nv = dot32m(Ni, Vj, nv); qv = dot32m(Qi, Vj, qv);
+        vp = dot32m(Vi, Pj, vp); vn = dot32m(Vi, Nj, vn); vq = dot32m(Vi, Qj, vq);
+        vv = dot32m(Vi, Vj, vv);
+      }
+    }
+  }
+  GenoMoments m;
+  m.ab11 = pp + nn - pn - np_;
+  m.ab12 = pq - nq;
+  m.ab21 = qp - qn;
+  m.ab22 = qq;
+  m.a1 = pv - nv; m.a2 = qv; m.b1 = vp - vn; m.b2 = vq; m.n0 = vv;
+  return m;
+}
+
 struct CountArgs {
   const int8_t* panels;
   int64_t block_bytes, pop_off;
@@ -128,30 +185,21 @@ __global__ void __launch_bounds__(256) count_pairs_kernel(CountArgs a) {
     c[0] = hh; c[1] = hv - hh; c[2] = vh - hh; c[3] = vv - hv - vh + hh;
     if (a.nvalid) a.nvalid[t] = vv;
   } else {
-    // features: b = [g==1], c = [g==2], v = [g>=0]
-    int bb, bc, cb, cc, bv, cv, vb, vc, vv;
-    if (a.missing) {
-      int pr[3][3];
-      feature_products<3>(a.panels, a.block_bytes, a.pop_off, a.ks, i, j, pr);
-      bb = pr[0][0]; bc = pr[0][1]; bv = pr[0][2];
-      cb = pr[1][0]; cc = pr[1][1]; cv = pr[1][2];
-      vb = pr[2][0]; vc = pr[2][1]; vv = pr[2][2];
-    } else {
-      int pr[2][2];
-      feature_products<2>(a.panels, a.block_bytes, a.pop_off, a.ks, i, j, pr);
-      bb = pr[0][0]; bc = pr[0][1]; cb = pr[1][0]; cc = pr[1][1];
-      bv = a.site_stats[i]; cv = a.site_stats[a.V + i];
-      vb = a.site_stats[j]; vc = a.site_stats[a.V + j];
-      vv = a.K;
+    // centred features u = g - 1, q = u^2, v = [g >= 0]: the pair's moments,
+    // then its class counts by the exact inverse transform
+    GenoMoments m = geno_pair_moments(a.panels, a.block_bytes, a.pop_off, a.ks, a.missing != 0,
+                                      i, j);
+    if (!a.missing) {
+      const int* st = a.site_stats;
+      m.a1 = 2 * st[a.V + i] + st[i] - a.K; m.a2 = a.K - st[i];
+      m.b1 = 2 * st[a.V + j] + st[j] - a.K; m.b2 = a.K - st[j]; m.n0 = a.K;
     }
-    const int n10 = bv - bb - bc, n20 = cv - cb - cc;
-    const int n01 = vb - bb - cb, n02 = vc - bc - cc;
-    const int n00 = vv - (bv + cv) - n01 - n02;
+    int mc[9];
+    geno_class_counts(m, mc);   // moments order n22 .. n00
     int32_t* c = a.counts + 9 * t;
-    c[0] = n00; c[1] = n01; c[2] = n02;
-    c[3] = n10; c[4] = bb; c[5] = bc;
-    c[6] = n20; c[7] = cb; c[8] = cc;
-    if (a.nvalid) a.nvalid[t] = vv;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c[k] = mc[8 - k];   // n00, n01, ..., n22
+    if (a.nvalid) a.nvalid[t] = m.n0;
   }
 }
 

@@ -9,6 +9,7 @@
 #define PGLD_KSTEP 64           // samples per packed k-step (MFMA 32x32x64 K)
 #define PGLD_FRAG_BYTES 1024    // one 32 sites x 64 samples fragment of 4-bit values
 #define PGLD_FP4_ONE 2u         // 1.0 in FP4 E2M1 (sign 0, exponent 01, mantissa 0)
+#define PGLD_FP4_NEG_ONE 0xAu   // -1.0 in FP4 E2M1 (sign bit set)
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v8i __attribute__((ext_vector_type(8)));
@@ -203,4 +204,25 @@ __device__ __forceinline__ int pgld_bin_scan(double d, const double* edges, int 
   while (b < n_bins - 1 && d >= edges[b + 1]) ++b;
   (void)hi;
   return b;
+}
+
+// Centred dosage moments of a site pair over the samples valid at both
+// (genotype panels: products of the features u = g - 1, q = u^2, v = [g >= 0];
+// row site first): ab11 = u.u, ab12 = u.q, ab21 = q.u, ab22 = q.q, a1 = u.v,
+// a2 = q.v, b1 = v.u, b2 = v.q, n0 = v.v.
+struct GenoMoments { int n0, a1, a2, b1, b2, ab11, ab12, ab21, ab22; };
+
+// The nine class counts (moments order) of a pair from its moments; every
+// numerator is an exact multiple of its divisor and non-negative.
+__device__ __forceinline__ void geno_class_counts(const GenoMoments& m, int (&c)[9]) {
+  const int uu = m.ab11, uq = m.ab12, qu = m.ab21, qq = m.ab22;
+  c[0] = (qq + qu + uq + uu) >> 2;           // n22
+  c[1] = (m.a2 + m.a1 - qq - uq) >> 1;       // n21 = (qv + uv - qq - uq) / 2
+  c[2] = (qq - qu + uq - uu) >> 2;           // n20
+  c[3] = (m.b2 + m.b1 - qq - qu) >> 1;       // n12 = (vq + vu - qq - qu) / 2
+  c[4] = m.n0 - m.b2 - m.a2 + qq;            // n11 = vv - vq - qv + qq
+  c[5] = (m.b2 - m.b1 - qq + qu) >> 1;       // n10 = (vq - vu - qq + qu) / 2
+  c[6] = (qq + qu - uq - uu) >> 2;           // n02
+  c[7] = (m.a2 - m.a1 - qq + uq) >> 1;       // n01 = (qv - uv - qq + uq) / 2
+  c[8] = (qq - qu - uq + uu) >> 2;           // n00
 }

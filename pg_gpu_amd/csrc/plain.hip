@@ -331,19 +331,20 @@ __global__ void __launch_bounds__(64) count_block_geno_kernel(
   for (int r = 0; r < 16; ++r) {
     const int64_t i = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     if (i >= i0 + ni || j >= j0 + nj || i >= V || j >= V) continue;
-    const int n11 = (int)pr[0][0][r], n12 = (int)pr[0][1][r];
-    const int n21 = (int)pr[1][0][r], n22 = (int)pr[1][1][r];
-    int bv, cv, vb, vc, vv;
+    // the centred moments from the feature products (u, q, v), then the
+    // class counts by the exact inverse transform
+    GenoMoments m;
+    m.ab11 = (int)pr[0][0][r]; m.ab12 = (int)pr[0][1][r];
+    m.ab21 = (int)pr[1][0][r]; m.ab22 = (int)pr[1][1][r];
     if (missing) {
-      bv = (int)pr[0][2][r]; cv = (int)pr[1][2][r];
-      vb = (int)pr[2][0][r]; vc = (int)pr[2][1][r]; vv = (int)pr[2][2][r];
+      m.a1 = (int)pr[0][2][r]; m.a2 = (int)pr[1][2][r];
+      m.b1 = (int)pr[2][0][r]; m.b2 = (int)pr[2][1][r]; m.n0 = (int)pr[2][2][r];
     } else {
-      bv = st[i]; cv = st[V + i]; vb = st[j]; vc = st[V + j]; vv = K;
+      m.a1 = 2 * st[V + i] + st[i] - K; m.a2 = K - st[i];
+      m.b1 = 2 * st[V + j] + st[j] - K; m.b2 = K - st[j]; m.n0 = K;
     }
-    const int n10 = bv - n11 - n12, n20 = cv - n21 - n22;
-    const int n01 = vb - n11 - n21, n02 = vc - n12 - n22;
-    const int n00 = vv - bv - cv - n01 - n02;
-    const int c[9] = {n22, n21, n20, n12, n11, n10, n02, n01, n00};
+    int c[9];
+    geno_class_counts(m, c);   // moments order
     const int64_t o = (i - i0) * nj + (j - j0);
     const int64_t plane = ni * nj;
     for (int m = 0; m < 9; ++m) out[m * plane + o] = c[m];
