@@ -343,6 +343,8 @@ __global__ void __launch_bounds__(PcShape<T>::THREADS, 1) fused_pc_kernel(FusedP
       if (!fast) {
         t.i0 = i0;
         t.j0 = j0;
+        t.bmin = bmin;
+        t.bmax = bmax;
         tile_bins(a, t, bw);
       }
       // the first empty mailbox of this wave's statistics waves

@@ -923,40 +923,86 @@ __device__ __forceinline__ void reduce_to_limbs(const TileCtx& t, double (&acc)[
 
 // Bins of the lane's 16 pairs of a tile that is not single-bin: 8 bits per
 // slot r in word r >> 2, 0xFF where the pair is not counted (i >= j, beyond
-// the cutoff, outside the bins).  All loads are issued before the epilogue.
+// the cutoff, outside the bins).  The corner pairs bound every pair's bin to
+// [t.bmin, t.bmax], so a pair's bin is the window's first bin plus the number
+// of the window's inner edges at or below its distance (the digitize rule for
+// non-decreasing edges): wave-uniform edges, no per-lane search.  The 16 row
+// coordinates are loaded at once (indices clamped into the sites).
+#ifndef PGLD_BIN_WINDOW
+#define PGLD_BIN_WINDOW 24  // widest window of inner edges counted; wider: per-lane scan
+#endif
 __device__ __forceinline__ void tile_bins(const FusedParams& a, const TileCtx& t,
                                           uint32_t (&bw)[4]) {
   const int nb = a.n_bins, lane = t.lane;
   const int64_t j = t.j0 + (lane & 31);
   const bool jok = j < a.V;
-  const double pos_j = jok ? a.pos[j] : 0.0;
-  const double co_j = (jok && !a.coord_is_pos) ? a.coord[j] : pos_j;
-  int guess = nb - 1;
-  // row i = ib + o with o = 8q + k a constant (mfma_row(lane, 4q + k)): the
-  // loads take immediate offsets from one base and i < j compares o with jr
+  const int64_t jc = jok ? j : a.V - 1;
+  const double pos_j = a.pos[jc];
+  const double co_j = a.coord_is_pos ? pos_j : a.coord[jc];
+  // row i = ib + o with o = 8q + k a constant (mfma_row(lane, 4q + k))
   const int64_t ib = t.i0 + 4 * (lane >> 5);
-  const double* pos_i = a.pos + ib;
-  const double* co_i = a.coord + ib;
-  const int jr = (int)(j - ib < 32 ? j - ib : 32);  // o < jr <=> i < j (o < 32)
+  const int jr = jok ? (int)(j - ib < 32 ? j - ib : 32) : 0;  // o < jr <=> i < j (o < 32)
+  // candidate bins [lo, hi] of the tile (wave-uniform); hi < lo: none
+  const int lo = __builtin_amdgcn_readfirstlane(t.bmin < 0 ? 0 : t.bmin);
+  const int hi = __builtin_amdgcn_readfirstlane(t.bmax >= nb ? nb - 1 : t.bmax);
+  const bool window = hi - lo <= PGLD_BIN_WINDOW;  // inner edges lo + 1 .. hi
+  int guess = nb - 1;
+  // two halves of 8 slots (words 2h, 2h + 1) keep the live distances few
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    uint32_t w = 0;
+  for (int h = 0; h < 2; ++h) {
+    double d[8];
+    uint32_t in_cut = 0;  // bit s: pos_j <= pos_i + max_dist
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int o = 8 * q + k;
-      uint32_t b8 = 0xFF;
-      if (jok && o < jr) {
-        const double pi = pos_i[o];
-        const double d = a.coord_is_pos ? (pos_j - pi) : fabs(co_j - co_i[o]);
-        const int b = pgld_bin_scan(d, t.sedges, nb, guess, nb - 1);
-        if (pos_j <= pi + a.max_dist && b >= 0 && b < nb) {
-          b8 = (uint32_t)b;
-          guess = b;
-        }
-      }
-      w |= b8 << (8 * k);
+    for (int s = 0; s < 8; ++s) {
+      const int r = 8 * h + s;
+      const int64_t i = ib + 8 * (r >> 2) + (r & 3);
+      const int64_t ic = i < a.V ? i : a.V - 1;
+      const double pi = a.pos[ic];
+      d[s] = a.coord_is_pos ? pos_j - pi : fabs(co_j - a.coord[ic]);
+      in_cut |= (pos_j <= pi + a.max_dist ? 1u : 0u) << s;
     }
-    bw[q] = w;
+    if (window) {
+      int c[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) c[s] = lo;
+      for (int e = lo + 1; e <= hi; ++e) {
+        const double ee = a.edges[e];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) c[s] += d[s] >= ee ? 1 : 0;
+      }
+      // lo <= n_bins and hi + 1 >= 0 index real edges (n_bins + 1 of them)
+      const double e_lo = a.edges[lo], e_hi = a.edges[hi + 1];
+#pragma unroll
+      for (int w2 = 0; w2 < 2; ++w2) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int s = 4 * w2 + k, o = 8 * (2 * h + w2) + k;
+          const bool ok = o < jr && ((in_cut >> s) & 1u) && d[s] >= e_lo && d[s] < e_hi;
+          w |= (ok ? (uint32_t)c[s] : 0xFFu) << (8 * k);
+        }
+        bw[2 * h + w2] = w;
+      }
+    } else {
+#pragma unroll
+      for (int w2 = 0; w2 < 2; ++w2) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int s = 4 * w2 + k, o = 8 * (2 * h + w2) + k;
+          uint32_t b8 = 0xFF;
+          if (o < jr) {
+            const int b = pgld_bin_scan(d[s], t.sedges, nb, guess, nb - 1);
+            if (((in_cut >> s) & 1u) && b >= 0 && b < nb) {
+              b8 = (uint32_t)b;
+              guess = b;
+            }
+          }
+          w |= b8 << (8 * k);
+        }
+        bw[2 * h + w2] = w;
+      }
+    }
   }
 }
 
