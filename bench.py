@@ -99,6 +99,24 @@ def utilisation_from_profile(name):
     return out or None
 
 
+def kernel_name(pan, geno, windows, r2, form):
+    """The kernel instantiation the library dispatches for this job (the
+    traits class: one haplotype population without missing calls has its own,
+    one genotype population keeps its moment accumulators in the per-wave
+    form)."""
+    tail = ", r2" if r2 else ""
+    nomiss = pan.P == 1 and not bool(pan.has_missing[0])
+    if windows:
+        t = "GenoTraits<1, r2>" if geno else ("HapNMTraits<r2>" if nomiss else "HapTraits<1, r2>")
+        return f"window_kernel<{t}>"
+    if geno:
+        t = (f"GenoRawTraits<{'r2' if r2 else ''}>" if pan.P == 1 and form != 1
+             else f"GenoTraits<{pan.P}{tail}>")
+    else:
+        t = f"HapNMTraits<{'r2' if r2 else ''}>" if nomiss else f"HapTraits<{pan.P}{tail}>"
+    return f"fused_kernel<{t}>"
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -545,8 +563,7 @@ def main():
         log(f"end to end: {e2e['ms']:.2f} ms ({e2e['value']:.3e} pairs/s)")
 
     achieved = ops_per_pair * n_pairs / (k_ms * 1e-3) / 1e12
-    kname = (f"window_kernel<{'Geno' if geno else 'Hap'}Traits<1, r2>>" if windows else
-             f"fused_kernel<{'Geno' if geno else 'Hap'}Traits<{pan.P}{', r2' if r2 else ''}>>")
+    kname = kernel_name(pan, geno, windows, r2, launch.get("form", 0))
     prof = f"{'windows' if windows else 'fused'}_{w.mode}{pan.P}_{w.name}"
     traffic = traffic_from_profile(prof) if args.scale == 1.0 else None
     util = utilisation_from_profile(prof) if args.scale == 1.0 else None

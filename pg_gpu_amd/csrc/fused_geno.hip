@@ -1,11 +1,18 @@
-// Genotype instantiations of the fused binned and window kernels: nine
-// 10-bit counts in three words per slot for populations of at most 1,023
-// individuals, 16-bit counts in five words above.
+// Genotype instantiations of the fused binned and window kernels.  One
+// population in the per-wave form keeps its nine moment products in the MFMA
+// accumulators; otherwise the counts are packed per slot: nine 10-bit fields
+// in three words for populations of at most 1,023 individuals, 16-bit fields
+// in five words above.
 #include "fused_kernels.h"
 
 namespace pgld {
 
 int launch_binned_geno(const PanelLayout& L, const pgld_binned_args* args, hipStream_t s) {
+  // one population: the per-wave form reads the moment accumulators directly
+  // (any population size); the producer/consumer form hands packed words over
+  if (L.n_pops == 1 && args->form != 1)
+    return args->with_r2 ? launch_fused<GenoRawTraits<true>>(L, args, s)
+                         : launch_fused<GenoRawTraits<false>>(L, args, s);
   if (args->with_r2)  // one population
     return L.pop_size[0] <= 1023 ? launch_fused<GenoTraits<1, 3, true>>(L, args, s)
                                  : launch_fused<GenoTraits<1, 5, true>>(L, args, s);

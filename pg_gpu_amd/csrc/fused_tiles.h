@@ -796,6 +796,106 @@ struct GenoTraits {
   }
 };
 
+// One genotype population in the per-wave kernel: the tile's nine moment
+// products stay in their FP32 MFMA accumulators (exact integers) and the
+// epilogue reads them directly, with no packing into count words.  The nine
+// accumulators take 144 registers, which two waves per SIMD still hold.
+template <bool R2>
+struct GenoRawTraits {
+  using Packed = GenoTraits<1, 3, R2>;
+  static constexpr int NP = 1;
+  static constexpr bool WITH_R2 = R2;
+  static constexpr int MINB = 2;
+  static constexpr int NS = Packed::NS;
+  static constexpr int NG = Packed::NG;
+  static constexpr bool MERGE_R2 = Packed::MERGE_R2;
+  static constexpr int GS = Packed::GS;
+  static constexpr int NSO = Packed::NSO;
+  static constexpr int NGT = Packed::NGT;
+  static constexpr bool PC_OK = false;
+  template <int G>
+  __device__ __forceinline__ static int scale_exp(int q) { return Packed::template scale_exp<G>(q); }
+  template <int G>
+  __device__ __forceinline__ static double stat_mult(const FusedParams& a, const double* srt, int q) {
+    return Packed::template stat_mult<G>(a, srt, q);
+  }
+  template <int G>
+  __device__ static constexpr int gsz() { return Packed::template gsz<G>(); }
+  template <int G>
+  __device__ __forceinline__ static int col(int q) { return Packed::template col<G>(q); }
+  // m[3 f + g] = feature f of the row site . feature g of the column site
+  // (u, q, v): ab11, ab12, a1, ab21, ab22, a2, b1, b2, n0
+  struct Counts { v16f m[9]; };
+  __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb, int lane,
+                                                Counts& c) {
+    const int ks = a.ksteps[0];
+    const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[0]) + lane;
+    const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[0]) + lane;
+    const int fs = ks * 64;  // feature stride in v4i
+    if (a.missing[0]) {
+      mfma_products9<2>(A, B, fs, ks, c.m);
+      return;
+    }
+    mfma_products4<PGLD_PF_MISSING>(A, A + fs, B, B + fs, ks, c.m[0], c.m[1], c.m[3], c.m[4]);
+    // no missing calls: the marginal sums from the site tallies (a padded
+    // site counts as all heterozygous, u = q = 0)
+    const int32_t* st = a.site_stats;
+    const int K = a.K[0];
+    const int64_t j = cb * 32 + (lane & 31);
+    const int n1j = j < a.V ? st[j] : K, n2j = j < a.V ? st[a.V + j] : 0;
+    const float b1 = (float)(2 * n2j + n1j - K), b2 = (float)(K - n1j);
+    // opaque per tile: a loop-invariant 16-register splat would be hoisted
+    // out of the tile loop and held (or spilled) across it
+    float n0 = (float)K;
+    asm volatile("" : "+v"(n0));
+    const int64_t ib = rb * 32 + 4 * (lane >> 5);
+    const int32_t* si = st + ib;
+    const int32_t* sci = st + a.V + ib;
+    const int64_t lim = a.V - ib;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int o = 8 * (r >> 2) + (r & 3);
+      const int n1i = o < lim ? si[o] : K, n2i = o < lim ? sci[o] : 0;
+      c.m[2][r] = (float)(2 * n2i + n1i - K);
+      c.m[5][r] = (float)(K - n1i);
+      c.m[6][r] = b1;
+      c.m[7][r] = b2;
+      c.m[8][r] = n0;
+    }
+  }
+  template <int G>
+  __device__ __forceinline__ static void stats(const FusedParams& a, const double* srt, const Counts& c, int r,
+                                               double* x) {
+    GenoPop p;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) p.g[k] = 0.0;  // the one-population forms read only moments
+    p.ab11 = (double)c.m[0][r]; p.ab12 = (double)c.m[1][r]; p.a1 = (double)c.m[2][r];
+    p.ab21 = (double)c.m[3][r]; p.ab22 = (double)c.m[4][r]; p.a2 = (double)c.m[5][r];
+    p.b1 = (double)c.m[6][r]; p.b2 = (double)c.m[7][r];
+    const float n0f = c.m[8][r];
+    p.n0 = (double)n0f;
+    p.n = p.n0;
+    const double2* t = reinterpret_cast<const double2*>(rt_entry(a, srt, 0, (int)n0f));
+    const double2 ra = t[0], rb = t[1];
+    p.r1 = ra.x; p.r2 = ra.y; p.r3 = rb.x; p.r4 = rb.y;
+    if constexpr (G < NG) {
+      geno_stats_group<1, G>(&p, x);
+      if constexpr (MERGE_R2) geno_r2(p, x + GS - 2);
+    } else {
+      geno_r2(p, x);
+    }
+  }
+  template <int G>
+  __device__ __forceinline__ static void accum(const FusedParams& a, const double* srt,
+                                               const Counts& c, int r, double* acc, double wt) {
+    constexpr int GSZ = gsz<G>();
+    double x[GSZ];
+    stats<G>(a, srt, c, r, x);
+#pragma unroll
+    for (int q = 0; q < GSZ; ++q) acc[q] = fma(x[q], wt, acc[q]);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Binning.  Each wave owns an FP64 staging array stage[bin][slot] for the GS
 // statistics of the group being evaluated, and a pair counter per bin, in
