@@ -36,6 +36,9 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
   for (int e = tid; e < nb * (NS + 1) * 3; e += blockDim.x) slimb[e] = 0ull;
   for (int e = tid; e < 4 * nb; e += blockDim.x) scnt[e] = 0;
   __syncthreads();
+  // one bin edge per lane for the tile classification (pgld_bin_ballot)
+  const double lane_edge = lane <= nb ? sedges[lane] : __builtin_inf();
+  const double e64 = nb == 64 ? sedges[64] : 0.0;
   TileCtx t;
   t.fp = &a;
   t.sedges = sedges;
@@ -91,8 +94,8 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
     // (i0, j1) bound the bins of every pair in the tile.
     t.i0 = i0;
     t.j0 = j0;
-    t.bmin = pgld_bin_scan(d_min, sedges, nb, 0, nb - 1);
-    t.bmax = pgld_bin_scan(d_max, sedges, nb, t.bmin < 0 ? 0 : t.bmin, nb - 1);
+    t.bmin = pgld_bin_ballot(d_min, lane_edge, nb, e64);
+    t.bmax = pgld_bin_ballot(d_max, lane_edge, nb, e64);
     t.fast = (cb > rb) && (j1 == j0 + 31) && in_cut &&
              t.bmin == t.bmax && t.bmin >= 0 && t.bmin < nb;
     t.blo = t.bmin < 0 ? 0 : t.bmin;
@@ -316,6 +319,8 @@ __global__ void __launch_bounds__(PcShape<T>::THREADS, 1) fused_pc_kernel(FusedP
     t.fp = &a;
     t.sedges = sedges;
     t.lane = lane;
+    const double lane_edge = lane <= nb ? sedges[lane] : __builtin_inf();
+    const double e64 = nb == 64 ? sedges[64] : 0.0;
     int turn = 0;
     while (seg_lo + local < seg_hi) {
       const int64_t rb = __builtin_amdgcn_readfirstlane(nrb), cb = __builtin_amdgcn_readfirstlane(ncb);
@@ -333,8 +338,8 @@ __global__ void __launch_bounds__(PcShape<T>::THREADS, 1) fused_pc_kernel(FusedP
         nrb = a.tiles[2 * (seg_lo + next)];
         ncb = a.tiles[2 * (seg_lo + next) + 1];
       }
-      const int bmin = pgld_bin_scan(d_min, sedges, nb, 0, nb - 1);
-      const int bmax = pgld_bin_scan(d_max, sedges, nb, bmin < 0 ? 0 : bmin, nb - 1);
+      const int bmin = pgld_bin_ballot(d_min, lane_edge, nb, e64);
+      const int bmax = pgld_bin_ballot(d_max, lane_edge, nb, e64);
       const bool fast = (cb > rb) && (j1 == j0 + 31) && in_cut && bmin == bmax && bmin >= 0 &&
                         bmin < nb;
       const int blo = bmin < 0 ? 0 : bmin;

@@ -206,6 +206,16 @@ __device__ __forceinline__ int pgld_bin_scan(double d, const double* edges, int 
   return b;
 }
 
+// The same digitize for a wave-uniform distance d, from one edge per lane
+// (lane l holds edges[l], +inf past the last edge; e64 = edges[64], the one
+// edge no lane holds, when n_bins = 64): the number of edges at or below d,
+// minus one.  Every lane of the wave must be active.
+__device__ __forceinline__ int pgld_bin_ballot(double d, double lane_edge, int n_bins, double e64) {
+  int cnt = __popcll(__ballot(d >= lane_edge));
+  if (n_bins == 64 && d >= e64) ++cnt;
+  return cnt - 1;
+}
+
 // Centred dosage moments of a site pair over the samples valid at both
 // (genotype panels: products of the features u = g - 1, q = u^2, v = [g >= 0];
 // row site first): ab11 = u.u, ab12 = u.q, ab21 = q.u, ab22 = q.q, a1 = u.v,
