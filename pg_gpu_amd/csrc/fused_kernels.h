@@ -96,8 +96,8 @@ __global__ void __launch_bounds__(256, T::MINB) fused_kernel(FusedParams a) {
     t.j0 = j0;
     t.bmin = pgld_bin_ballot(d_min, lane_edge, nb, e64);
     t.bmax = pgld_bin_ballot(d_max, lane_edge, nb, e64);
-    t.fast = (cb > rb) && (j1 == j0 + 31) && in_cut &&
-             t.bmin == t.bmax && t.bmin >= 0 && t.bmin < nb;
+    t.clean = (cb > rb) && (j1 == j0 + 31) && in_cut && t.bmin >= 0 && t.bmax < nb;
+    t.fast = t.clean && t.bmin == t.bmax;
     t.blo = t.bmin < 0 ? 0 : t.bmin;
     t.dual = !t.fast && t.blo < nb && (t.bmax >= nb ? nb - 1 : t.bmax) - t.blo <= 1;
     const uint32_t none[4] = {0u, 0u, 0u, 0u};
@@ -350,7 +350,8 @@ __global__ void __launch_bounds__(PcShape<T>::THREADS, 1) fused_pc_kernel(FusedP
         t.j0 = j0;
         t.bmin = bmin;
         t.bmax = bmax;
-        tile_bins(a, t, bw);
+        t.clean = (cb > rb) && (j1 == j0 + 31) && in_cut && bmin >= 0 && bmax < nb;
+        tile_bins<clean_bins<T>()>(a, t, bw);
       }
       // the first empty mailbox of this wave's statistics waves
       int m = -1;

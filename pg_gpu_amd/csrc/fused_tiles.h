@@ -985,6 +985,7 @@ struct TileCtx {
   unsigned long long* slimb;
   int lane, bmin, bmax;
   bool fast;   // every pair valid and in bin bmin
+  bool clean;  // every pair valid (i < j, inside the cutoff and the bins)
   bool dual;   // otherwise: every valid pair in bin blo or blo + 1
   int blo;
   int64_t i0, j0;
@@ -1031,6 +1032,48 @@ __device__ __forceinline__ void reduce_to_limbs(const TileCtx& t, double (&acc)[
 #ifndef PGLD_BIN_WINDOW
 #define PGLD_BIN_WINDOW 24  // widest window of inner edges counted; wider: per-lane scan
 #endif
+// tile_bins for a clean tile: off the diagonal, all 32 x 32 sites real,
+// inside the cutoff and inside the bins (bins [lo, hi] wave-uniform), so every
+// pair is counted.  The row coordinates are the same for all lanes of a half
+// wave: scalar loads, one select per pair.
+__device__ __forceinline__ void tile_bins_clean(const FusedParams& a, const TileCtx& t, int lo,
+                                                int hi, uint32_t (&bw)[4]) {
+  const int lane = t.lane;
+  const double* c = a.coord_is_pos ? a.pos : a.coord;
+  const double cj = c[t.j0 + (lane & 31)];
+  const double* crow = c + t.i0;
+  const bool upper = lane >= 32;  // rows o + 4
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    double d[8];
+    int cnt[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int r = 8 * h + s, o = 8 * (r >> 2) + (r & 3);
+      const double ci = upper ? crow[o + 4] : crow[o];
+      d[s] = cj - ci;  // coordinates are non-decreasing: the distance for i < j
+      cnt[s] = lo;
+    }
+    for (int e = lo + 1; e <= hi; ++e) {
+      const double ee = a.edges[e];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) cnt[s] += d[s] >= ee ? 1 : 0;
+    }
+#pragma unroll
+    for (int w2 = 0; w2 < 2; ++w2)
+      bw[2 * h + w2] = (uint32_t)cnt[4 * w2] | ((uint32_t)cnt[4 * w2 + 1] << 8) |
+                       ((uint32_t)cnt[4 * w2 + 2] << 16) | ((uint32_t)cnt[4 * w2 + 3] << 24);
+  }
+}
+
+// CLEAN: clean tiles take tile_bins_clean.  Where its registers fit: one or
+// two populations at two or four waves per SIMD (the three-wave and
+// one-wave kernels spill with it).
+template <class T>
+__host__ __device__ constexpr bool clean_bins() {
+  return T::NP <= 2 && (T::MINB == 2 || T::MINB == 4);
+}
+template <bool CLEAN>
 __device__ __forceinline__ void tile_bins(const FusedParams& a, const TileCtx& t,
                                           uint32_t (&bw)[4]) {
   const int nb = a.n_bins, lane = t.lane;
@@ -1046,6 +1089,12 @@ __device__ __forceinline__ void tile_bins(const FusedParams& a, const TileCtx& t
   const int lo = __builtin_amdgcn_readfirstlane(t.bmin < 0 ? 0 : t.bmin);
   const int hi = __builtin_amdgcn_readfirstlane(t.bmax >= nb ? nb - 1 : t.bmax);
   const bool window = hi - lo <= PGLD_BIN_WINDOW;  // inner edges lo + 1 .. hi
+  if constexpr (CLEAN) {
+    if (t.clean && window) {
+      tile_bins_clean(a, t, lo, hi, bw);
+      return;
+    }
+  }
   int guess = nb - 1;
   // two halves of 8 slots (words 2h, 2h + 1) keep the live distances few
 #pragma unroll
@@ -1147,7 +1196,7 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
 #pragma unroll
       for (int q = 0; q < 4; ++q) bw[q] = pre[q];
     } else {
-      tile_bins(a, t, bw);
+      tile_bins<clean_bins<T>()>(a, t, bw);
     }
     double acc1[GS];
 #pragma unroll
@@ -1190,7 +1239,7 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
 #pragma unroll
       for (int q = 0; q < 4; ++q) bw[q] = pre[q];
     } else {
-      tile_bins(a, t, bw);
+      tile_bins<clean_bins<T>()>(a, t, bw);
     }
     int cur = -1, count = 0;
 #pragma unroll 1
