@@ -118,18 +118,23 @@ __device__ __forceinline__ uint4 pack_load16(const int8_t* mat, const int8_t* ma
 }
 
 // ---------------------------------------------------------------------------
-// Site tallies.  A workgroup covers 512 consecutive sites (32 groups of 16)
-// of one population and splits the population's rows into 8 interleaved
-// slices: thread (slice, group) reads 16 bytes of each of its rows (one
-// coalesced 512-byte row segment per wave and load) and counts the bytes
-// equal to 1, 2 and -1 with bytewise (SWAR) tests into per-byte counters,
-// flushed to integers every 255 rows.  The slices meet in LDS; the tallies are
-// written once, coalesced.  A byte of a real site outside the mode's alphabet
-// sets flags[0].
+// Site tallies.  A workgroup covers 256 consecutive sites (16 groups of 16)
+// of one population and splits the population's rows into 32 interleaved
+// slices: thread (slice, group) reads 16 bytes of each of its rows (a wave
+// reads four rows' 256-byte segments per load) and counts the bytes equal to
+// 1, 2 and -1 with bytewise (SWAR) tests into per-byte counters, flushed to
+// integers every 255 rows.  The four slices of a wave are summed with two
+// lane exchanges, the eight waves through LDS; the tallies are written once,
+// coalesced.  (Many slices and small site blocks keep enough loads in flight:
+// one workgroup per 512 sites with 8 slices ran below one wave per SIMD on
+// 100k sites.)  A byte of a real site outside the mode's alphabet sets
+// flags[0].
 // ---------------------------------------------------------------------------
-constexpr int SCAN_SLICES = 8;
-constexpr int SCAN_GROUPS = 32;
+constexpr int SCAN_SLICES = 32;
+constexpr int SCAN_GROUPS = 16;
 constexpr int SCAN_SITES = SCAN_GROUPS * 16;
+constexpr int SCAN_THREADS = SCAN_SLICES * SCAN_GROUPS;
+constexpr int SCAN_WAVES = SCAN_THREADS / 64;
 
 struct ScanParams {
   const int8_t* mat;
@@ -138,20 +143,22 @@ struct ScanParams {
   const int32_t* rows;
   int32_t offs[PGLD_MAX_POPS + 1];
   int32_t mode;
+  int32_t aligned;  // matrix and leading dimension 16-byte aligned
   int32_t* stats;   // [P][3][n_sites]
   int32_t* flags;
 };
 
-__global__ void __launch_bounds__(256) scan_sites_kernel(ScanParams a) {
-  __shared__ int part[3 * SCAN_SITES];
-  const int tid = threadIdx.x;
+__global__ void __launch_bounds__(SCAN_THREADS) scan_sites_kernel(ScanParams a) {
+  __shared__ int part[SCAN_WAVES][3 * SCAN_SITES];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int p = blockIdx.y;
-  for (int e = tid; e < 3 * SCAN_SITES; e += 256) part[e] = 0;
-  __syncthreads();
   const int g = tid % SCAN_GROUPS, slice = tid / SCAN_GROUPS;
   const int64_t base = (int64_t)blockIdx.x * SCAN_SITES;
   const int64_t site0 = base + 16 * g;
   const bool geno = a.mode == PGLD_MODE_GENO;
+  int c[16][3];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) c[k][0] = c[k][1] = c[k][2] = 0;
   if (site0 < a.n_sites) {
     // bit 8b of vm[e] set where site site0 + 4e + b is a real site
     uint32_t vm[4];
@@ -162,16 +169,10 @@ __global__ void __launch_bounds__(256) scan_sites_kernel(ScanParams a) {
       for (int b = 0; b < 4; ++b) m |= (site0 + 4 * e + b < a.n_sites ? 1u : 0u) << (8 * b);
       vm[e] = m;
     }
-    int c[16][3];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) c[k][0] = c[k][1] = c[k][2] = 0;
     uint32_t a1[4] = {0, 0, 0, 0}, a2[4] = {0, 0, 0, 0}, am[4] = {0, 0, 0, 0};
     uint32_t bad = 0;
-    int pending = 0;
-    const int r1 = a.offs[p + 1];
-    for (int r = a.offs[p] + slice; r < r1; r += SCAN_SLICES) {
-      const int8_t* src = a.mat + (int64_t)a.rows[r] * a.ld + site0;
-      const uint4 q = pack_load16(a.mat, a.mat_end, src, site0, a.n_sites);
+    int pending = 0;  // rows in the byte counters (flushed before 255)
+    auto tally = [&](const uint4 q) {
       const uint32_t w[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -182,18 +183,49 @@ __global__ void __launch_bounds__(256) scan_sites_kernel(ScanParams a) {
         am[e] += em;
         bad |= ~(e0 | e1 | em | (geno ? e2 : 0u)) & vm[e];
       }
-      if (++pending == 255) {
+    };
+    auto flush = [&]() {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-          const int e = k >> 2, sh = 8 * (k & 3);
-          c[k][0] += (a1[e] >> sh) & 0xFFu;
-          c[k][1] += (a2[e] >> sh) & 0xFFu;
-          c[k][2] += (am[e] >> sh) & 0xFFu;
-        }
-#pragma unroll
-        for (int e = 0; e < 4; ++e) a1[e] = a2[e] = am[e] = 0u;
-        pending = 0;
+      for (int k = 0; k < 16; ++k) {
+        const int e = k >> 2, sh = 8 * (k & 3);
+        c[k][0] += (a1[e] >> sh) & 0xFFu;
+        c[k][1] += (a2[e] >> sh) & 0xFFu;
+        c[k][2] += (am[e] >> sh) & 0xFFu;
       }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) a1[e] = a2[e] = am[e] = 0u;
+      pending = 0;
+    };
+    const int r0 = a.offs[p], r1 = a.offs[p + 1];
+    const int last = r1 - 1;
+    int r = r0 + slice;
+    if (a.aligned && site0 + 16 <= a.n_sites) {
+      // rows 16-byte aligned and this thread's 16 sites all real: four rows'
+      // plain loads in flight, the next four row ids loaded one step ahead
+      // (a separate loop, so no alignment branch sits between the loads)
+      int id[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) id[k] = r0 < r1 ? a.rows[min(r + k * SCAN_SLICES, last)] : 0;
+      // (the last step's ids past the population repeat its last row and are
+      // not tallied)
+      for (; r < r1; r += 4 * SCAN_SLICES) {
+        uint4 q[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          q[k] = *reinterpret_cast<const uint4*>(a.mat + (int64_t)id[k] * a.ld + site0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) id[k] = a.rows[min(r + (4 + k) * SCAN_SLICES, last)];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (r + k * SCAN_SLICES < r1) tally(q[k]);
+        pending += 4;
+        if (pending > 250) flush();
+      }
+    }
+    for (; r < r1; r += SCAN_SLICES) {
+      tally(pack_load16(a.mat, a.mat_end, a.mat + (int64_t)a.rows[r] * a.ld + site0, site0,
+                        a.n_sites));
+      if (++pending == 255) flush();
     }
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
@@ -202,18 +234,26 @@ __global__ void __launch_bounds__(256) scan_sites_kernel(ScanParams a) {
       c[k][1] += (a2[e] >> sh) & 0xFFu;
       c[k][2] += (am[e] >> sh) & 0xFFu;
     }
-#pragma unroll
-    for (int k = 0; k < 16; ++k)
-#pragma unroll
-      for (int q = 0; q < 3; ++q)
-        if (c[k][q]) atomicAdd(&part[q * SCAN_SITES + 16 * g + k], c[k][q]);
     if (bad) atomicOr(a.flags, 1);
   }
+  // lanes g, g + 16, g + 32, g + 48 of a wave hold the same sites
+#pragma unroll
+  for (int k = 0; k < 16; ++k)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      int v = c[k][q];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      if (lane < 16) part[wave][q * SCAN_SITES + 16 * g + k] = v;
+    }
   __syncthreads();
   int32_t* st = a.stats + (int64_t)p * 3 * a.n_sites;
-  for (int e = tid; e < 3 * SCAN_SITES; e += 256) {
+  for (int e = tid; e < 3 * SCAN_SITES; e += SCAN_THREADS) {
     const int q = e / SCAN_SITES, k = e - q * SCAN_SITES;
-    if (base + k < a.n_sites) st[(int64_t)q * a.n_sites + base + k] = part[e];
+    int v = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_WAVES; ++w) v += part[w][e];
+    if (base + k < a.n_sites) st[(int64_t)q * a.n_sites + base + k] = v;
   }
 }
 
@@ -337,10 +377,11 @@ extern "C" int pgld_scan_sites(const int8_t* d_mat, int64_t n_rows, int64_t n_si
   a.rows = d_rows;
   for (int p = 0; p <= PGLD_MAX_POPS; ++p) a.offs[p] = p <= desc->n_pops ? pop_offsets[p] : 0;
   a.mode = desc->mode;
+  a.aligned = (reinterpret_cast<uintptr_t>(d_mat) % 16 == 0) && (ld % 16 == 0);
   a.stats = d_site_stats;
   a.flags = d_flags;
   const unsigned grid = (unsigned)((n_sites + SCAN_SITES - 1) / SCAN_SITES);
-  hipLaunchKernelGGL(scan_sites_kernel, dim3(grid, (unsigned)desc->n_pops), dim3(256), 0,
+  hipLaunchKernelGGL(scan_sites_kernel, dim3(grid, (unsigned)desc->n_pops), dim3(SCAN_THREADS), 0,
                      as_stream(stream), a);
   PGLD_LAUNCH_CHECK();
   return PGLD_OK;
