@@ -489,6 +489,8 @@ struct HapTraits {
     return 1.0;
   }
   static constexpr bool PC_OK = true;   // has the producer/consumer form
+  // single-bin tiles with the slot loop fully unrolled
+  static constexpr bool FULL_FAST = false;
   // statistics of pass G, and the output column of its q-th (the r^2 pair
   // after the group's statistics, or the r^2 group, lands at NS, NS + 1)
   template <int G>
@@ -698,6 +700,7 @@ struct GenoTraits {
     return 1.0;
   }
   static constexpr bool PC_OK = true;
+  static constexpr bool FULL_FAST = false;
   template <int G>
   __device__ static constexpr int gsz() {
     if constexpr (G < NG) return geno_group_size<P, G>() + (MERGE_R2 ? 2 : 0);
@@ -813,6 +816,9 @@ struct GenoRawTraits {
   static constexpr int NSO = Packed::NSO;
   static constexpr int NGT = Packed::NGT;
   static constexpr bool PC_OK = false;
+  // the single-bin slot loop fully unrolled: the accumulators are read with
+  // static indices (0.722 -> 0.698 ms on 1,000 diploids, 100k sites)
+  static constexpr bool FULL_FAST = true;
   template <int G>
   __device__ __forceinline__ static int scale_exp(int q) { return Packed::template scale_exp<G>(q); }
   template <int G>
@@ -1177,7 +1183,10 @@ __device__ __forceinline__ void tile_group(const FusedParams& a, const TileCtx& 
   for (int q = 0; q < GS; ++q) acc[q] = 0.0;
   if (t.fast) {
     // every pair of the tile is valid and falls in bin bmin
-    if constexpr (U2) {  // 2 x 500 haplotypes, 200k sites: 3.21 -> 3.18 ms
+    if constexpr (T::FULL_FAST) {  // static slot indices (no index-mode moves)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) T::template accum<G>(a, t.srt, cts, r, acc, 1.0);
+    } else if constexpr (U2) {  // 2 x 500 haplotypes, 200k sites: 3.21 -> 3.18 ms
 #pragma unroll 2
       for (int r = 0; r < 16; ++r) T::template accum<G>(a, t.srt, cts, r, acc, 1.0);
     } else {
