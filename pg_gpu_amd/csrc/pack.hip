@@ -82,6 +82,26 @@ constexpr int PACK_SB = 8;  // site blocks per workgroup (256 sites, 16 groups o
 // halves of the groups one wave writes (phase 1) on distinct banks.
 constexpr int PACK_PITCH = 19;
 
+// The 16 bytes at src from the two aligned 16-byte words around them (a
+// funnel shift; m = src mod 16, 0 included).  The 32 bytes from src - m must
+// lie inside the matrix.
+__device__ __forceinline__ uint4 load16_shifted(const int8_t* src) {
+  const int m = (int)(reinterpret_cast<uintptr_t>(src) & 15u);
+  // named values, not arrays: a select between neighbouring array elements
+  // would become an indexed load through scratch memory
+  const uint4* base = reinterpret_cast<const uint4*>(src - m);
+  const uint4 q0 = base[0], q1 = base[1];
+  const bool s4 = (m & 4) != 0, s8 = (m & 8) != 0;
+  const uint32_t t0 = s4 ? q0.y : q0.x, t1 = s4 ? q0.z : q0.y, t2 = s4 ? q0.w : q0.z,
+                 t3 = s4 ? q1.x : q0.w, t4 = s4 ? q1.y : q1.x, t5 = s4 ? q1.z : q1.y,
+                 t6 = s4 ? q1.w : q1.z;
+  const uint32_t u0 = s8 ? t2 : t0, u1 = s8 ? t3 : t1, u2 = s8 ? t4 : t2, u3 = s8 ? t5 : t3,
+                 u4 = s8 ? t6 : t4;
+  const uint32_t sh = (uint32_t)(m & 3);
+  return make_uint4(__builtin_amdgcn_alignbyte(u1, u0, sh), __builtin_amdgcn_alignbyte(u2, u1, sh),
+                    __builtin_amdgcn_alignbyte(u3, u2, sh), __builtin_amdgcn_alignbyte(u4, u3, sh));
+}
+
 // Sites site0 .. site0+15 of the row at src, as four little-endian dwords;
 // sites at or past n_sites read as -2 (no feature set).
 __device__ __forceinline__ uint4 pack_load16(const int8_t* mat, const int8_t* mat_end,
@@ -91,20 +111,7 @@ __device__ __forceinline__ uint4 pack_load16(const int8_t* mat, const int8_t* ma
   if (m == 0 && site0 + 16 <= n_sites) {
     return *reinterpret_cast<const uint4*>(src);
   } else if (site0 + 16 <= n_sites && src - m >= mat && src - m + 32 <= mat_end) {
-    // two aligned 16-byte loads around the segment, then a funnel shift
-    // (named values, not arrays: a select between neighbouring array
-    // elements would become an indexed load through scratch memory)
-    const uint4* base = reinterpret_cast<const uint4*>(src - m);
-    const uint4 q0 = base[0], q1 = base[1];
-    const bool s4 = (m & 4) != 0, s8 = (m & 8) != 0;
-    const uint32_t t0 = s4 ? q0.y : q0.x, t1 = s4 ? q0.z : q0.y, t2 = s4 ? q0.w : q0.z,
-                   t3 = s4 ? q1.x : q0.w, t4 = s4 ? q1.y : q1.x, t5 = s4 ? q1.z : q1.y,
-                   t6 = s4 ? q1.w : q1.z;
-    const uint32_t u0 = s8 ? t2 : t0, u1 = s8 ? t3 : t1, u2 = s8 ? t4 : t2, u3 = s8 ? t5 : t3,
-                   u4 = s8 ? t6 : t4;
-    const uint32_t sh = (uint32_t)(m & 3);
-    return make_uint4(__builtin_amdgcn_alignbyte(u1, u0, sh), __builtin_amdgcn_alignbyte(u2, u1, sh),
-                      __builtin_amdgcn_alignbyte(u3, u2, sh), __builtin_amdgcn_alignbyte(u4, u3, sh));
+    return load16_shifted(src);
   } else {
 #pragma unroll
     for (int e = 0; e < 4; ++e) w[e] = 0xFEFEFEFEu;
@@ -130,8 +137,14 @@ __device__ __forceinline__ uint4 pack_load16(const int8_t* mat, const int8_t* ma
 // 100k sites.)  A byte of a real site outside the mode's alphabet sets
 // flags[0].
 // ---------------------------------------------------------------------------
-constexpr int SCAN_SLICES = 32;
-constexpr int SCAN_GROUPS = 16;
+#ifndef PGLD_SCAN_SLICES
+#define PGLD_SCAN_SLICES 32
+#endif
+#ifndef PGLD_SCAN_GROUPS
+#define PGLD_SCAN_GROUPS 16
+#endif
+constexpr int SCAN_SLICES = PGLD_SCAN_SLICES;
+constexpr int SCAN_GROUPS = PGLD_SCAN_GROUPS;
 constexpr int SCAN_SITES = SCAN_GROUPS * 16;
 constexpr int SCAN_THREADS = SCAN_SLICES * SCAN_GROUPS;
 constexpr int SCAN_WAVES = SCAN_THREADS / 64;
@@ -221,6 +234,39 @@ __global__ void __launch_bounds__(SCAN_THREADS) scan_sites_kernel(ScanParams a) 
         pending += 4;
         if (pending > 250) flush();
       }
+    } else if (site0 + 16 <= a.n_sites) {
+      // rows not 16-byte aligned: every row through the branch-free shifted
+      // load (a shift of 0 for a row that happens to be aligned), so lanes of
+      // different rows do not diverge; the generic load only where the two
+      // words would pass the end of the matrix
+      int id[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) id[k] = r0 < r1 ? a.rows[min(r + k * SCAN_SLICES, last)] : 0;
+      for (; r < r1; r += 4 * SCAN_SLICES) {
+        const int8_t* src[4];
+        bool inside = true;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          src[k] = a.mat + (int64_t)id[k] * a.ld + site0;
+          const int8_t* b0 = src[k] - (reinterpret_cast<uintptr_t>(src[k]) & 15u);
+          inside = inside && b0 >= a.mat && b0 + 32 <= a.mat_end;
+        }
+        uint4 q[4];
+        if (inside) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) q[k] = load16_shifted(src[k]);
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) q[k] = pack_load16(a.mat, a.mat_end, src[k], site0, a.n_sites);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) id[k] = a.rows[min(r + (4 + k) * SCAN_SLICES, last)];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (r + k * SCAN_SLICES < r1) tally(q[k]);
+        pending += 4;
+        if (pending > 250) flush();
+      }
     }
     for (; r < r1; r += SCAN_SLICES) {
       tally(pack_load16(a.mat, a.mat_end, a.mat + (int64_t)a.rows[r] * a.ld + site0, site0,
@@ -236,15 +282,15 @@ __global__ void __launch_bounds__(SCAN_THREADS) scan_sites_kernel(ScanParams a) 
     }
     if (bad) atomicOr(a.flags, 1);
   }
-  // lanes g, g + 16, g + 32, g + 48 of a wave hold the same sites
+  // lanes g, g + SCAN_GROUPS, ... of a wave hold the same sites
 #pragma unroll
   for (int k = 0; k < 16; ++k)
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       int v = c[k][q];
-      v += __shfl_xor(v, 16, 64);
-      v += __shfl_xor(v, 32, 64);
-      if (lane < 16) part[wave][q * SCAN_SITES + 16 * g + k] = v;
+#pragma unroll
+      for (int m = SCAN_GROUPS; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
+      if (lane < SCAN_GROUPS) part[wave][q * SCAN_SITES + 16 * g + k] = v;
     }
   __syncthreads();
   int32_t* st = a.stats + (int64_t)p * 3 * a.n_sites;
