@@ -181,6 +181,15 @@ def test_four_pops_64_bins_split_launches(cuda):
     run_case(cuda, hap, pos, pops, synth.log_bins(20, 9_000, 64), 9_000.0)
 
 
+def test_two_pops_64_bins_one_launch(cuda):
+    """The largest bin count in one launch (65 edges: one per lane plus the
+    last), with the cutoff past the last edge so that pairs beyond the bins
+    are skipped by the tile classification and the bin words."""
+    rng = np.random.default_rng(34)
+    hap, pos, pops = small_pops(rng, 0, 400, 30_000, 0.0, sizes=[30, 41])
+    run_case(cuda, hap, pos, pops, synth.log_bins(20, 8_000, 64), 9_000.0)
+
+
 def test_four_pops_workload_c4_scaled(cuda):
     """40/120/300/800 haplotypes, 3% missing, rare-allele sites: the 105-statistic set."""
     w = synth.make("c4", scale=0.0008)
