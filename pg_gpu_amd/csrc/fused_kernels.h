@@ -628,14 +628,21 @@ __device__ __forceinline__ void window_item(const WindowParams& w, const double*
   const int j_o = (int)(j - ib < -1 ? -1 : j - ib > 32 ? 32 : j - ib);
   const bool jin = j < hi;
   // two slots per iteration: two pairs' FP64 chains interleave
+  if (full) {
+    // every pair of the item is in the window: plain sums
 #pragma unroll 2
-  for (int r = 0; r < 16; ++r) {
-    const int o = 8 * (r >> 2) + (r & 3);
-    // every slot's statistics are finite (padded sites have zero counts), so
-    // slots outside the window take weight 0 instead of a divergent branch
-    const bool in = full || (o >= lo_o && o < j_o && jin);
-    T::template accum<0>(w.f, srt, cts, r, acc, in ? 1.0 : 0.0);
-    count += in;
+    for (int r = 0; r < 16; ++r) T::template accum<0>(w.f, srt, cts, r, acc, 1.0);
+    count = 16;
+  } else {
+#pragma unroll 2
+    for (int r = 0; r < 16; ++r) {
+      const int o = 8 * (r >> 2) + (r & 3);
+      // every slot's statistics are finite (padded sites have zero counts), so
+      // slots outside the window take weight 0 instead of a divergent branch
+      const bool in = o >= lo_o && o < j_o && jin;
+      T::template accum<0>(w.f, srt, cts, r, acc, in ? 1.0 : 0.0);
+      count += in;
+    }
   }
   // butterfly reduce-scatter (see reduce_to_limbs); the pair count rides in
   // the first padding slot (an exact small integer)
@@ -837,12 +844,18 @@ __device__ __forceinline__ void window_item16(const WindowParams& w, const doubl
   const int64_t j = j0 + (lane & 15);
   const int64_t ib = i0 + 4 * (lane >> 4);
   const bool jin = j < hi;
+  if (full) {
 #pragma unroll 2
-  for (int r = 0; r < 4; ++r) {
-    const int64_t i = ib + r;
-    const bool in = full || (i >= lo && i < j && jin);
-    T::accum(w.f, srt, cts, r, acc, in ? 1.0 : 0.0);
-    count += in;
+    for (int r = 0; r < 4; ++r) T::accum(w.f, srt, cts, r, acc, 1.0);
+    count = 4;
+  } else {
+#pragma unroll 2
+    for (int r = 0; r < 4; ++r) {
+      const int64_t i = ib + r;
+      const bool in = i >= lo && i < j && jin;
+      T::accum(w.f, srt, cts, r, acc, in ? 1.0 : 0.0);
+      count += in;
+    }
   }
   constexpr int NV = 8;
   static_assert(GS < NV, "count slot");
