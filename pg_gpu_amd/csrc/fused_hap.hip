@@ -35,7 +35,7 @@ int launch_count_band_hap(const PanelLayout& L, const pgld_binned_args* args,
 
 int launch_windows_hap(const PanelLayout& L, const pgld_window_args* args, hipStream_t s) {
   return L.missing[0] ? launch_windows<HapTraits<1, true>>(L, args, s)
-                      : launch_windows<HapNMTraits<true>>(L, args, s);
+                      : launch_windows<HapNMRawTraits<true>>(L, args, s);
 }
 
 }  // namespace pgld

@@ -948,7 +948,8 @@ static int launch_windows(const PanelLayout& L, const pgld_window_args* g, hipSt
   if (g->tile == 16) {
     // the 16-site kernel covers haplotypes with and without missing calls
     if constexpr (std::is_same<T, HapTraits<1, true>>::value ||
-                  std::is_same<T, HapNMTraits<true>>::value) {
+                  std::is_same<T, HapNMTraits<true>>::value ||
+                  std::is_same<T, HapNMRawTraits<true>>::value) {
       if (!gtab)
         PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(window16_kernel<false>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -193,14 +193,9 @@ __device__ __forceinline__ void mfma_products9(const v4i* A, const v4i* B, int f
   }
 }
 
-// Counts of a population without missing calls: n11 = H.H from the MFMAs,
-// the marginals from the site tallies.  MARG packs n11 | ki << 16 and
-// kj | K << 16 (ki, kj the allele counts of the row and column sites), else
-// the four class counts.  Returns kj.
-template <bool MARG>
-__device__ __forceinline__ int tile_counts_nomiss(const FusedParams& a, int p, int64_t rb,
-                                                  int64_t cb, int lane, const v4i* A,
-                                                  const v4i* B, int ks, v16u& w0, v16u& w1) {
+// H.H over the ks k-steps of row operand A and column operand B, operand
+// loads two k-steps ahead.
+__device__ __forceinline__ v16f hh_products(const v4i* A, const v4i* B, int ks) {
   v16f hh = zero16f();
   constexpr int PF = 2;
   v4i ah[PF], bh[PF];
@@ -217,6 +212,18 @@ __device__ __forceinline__ int tile_counts_nomiss(const FusedParams& a, int p, i
       }
     }
   }
+  return hh;
+}
+
+// Counts of a population without missing calls: n11 = H.H from the MFMAs,
+// the marginals from the site tallies.  MARG packs n11 | ki << 16 and
+// kj | K << 16 (ki, kj the allele counts of the row and column sites), else
+// the four class counts.  Returns kj.
+template <bool MARG>
+__device__ __forceinline__ int tile_counts_nomiss(const FusedParams& a, int p, int64_t rb,
+                                                  int64_t cb, int lane, const v4i* A,
+                                                  const v4i* B, int ks, v16u& w0, v16u& w1) {
+  const v16f hh = hh_products(A, B, ks);
   const int32_t* cnt1 = a.site_stats + (int64_t)p * 3 * a.V;
   const int64_t j = cb * 32 + (lane & 31);
   const int kj = j < a.V ? cnt1[j] : 0;
@@ -657,6 +664,72 @@ struct HapNMTraits : HapTraits<1, R2> {
   __device__ __forceinline__ static void accum(const FusedParams&, const double*, const Counts& c,
                                                int r, double* acc, double wt) {
     static_assert(Base::NGT == 1, "one statistic pass");
+    const HapPop p = pop(c, r);
+    acc[0] = fma(hap_dd_same_num(p), wt, acc[0]);
+    acc[1] = fma(hap_dz_same_num(p), wt, acc[1]);
+    acc[2] = fma(hap_pi2_same_num(p), wt, acc[2]);
+    if constexpr (R2) {
+      double x[2];
+      hap_r2(p, x);
+      acc[3] = fma(x[0], wt, acc[3]);
+      acc[4] = fma(x[1], wt, acc[4]);
+    }
+  }
+};
+
+// HapNMTraits for the window kernel, whose registers have room for it: n11
+// stays in its FP32 accumulator and the row sites' allele counts in words of
+// their own, so nothing is packed per work item or unpacked per pair.
+template <bool R2>
+struct HapNMRawTraits : HapNMTraits<R2> {
+  struct Counts {
+    v16f n11;
+    v16u ki;
+    double pB, qB, uB, PB, n;
+  };
+  __device__ __forceinline__ static void counts(const FusedParams& a, int64_t rb, int64_t cb,
+                                                int lane, Counts& c) {
+    const int ks = a.ksteps[0];
+    const v4i* A = reinterpret_cast<const v4i*>(a.panels + rb * a.block_bytes + a.pop_off[0]) + lane;
+    const v4i* B = reinterpret_cast<const v4i*>(a.panels + cb * a.block_bytes + a.pop_off[0]) + lane;
+    c.n11 = hh_products(A, B, ks);
+    const int32_t* cnt1 = a.site_stats;
+    const int64_t j = cb * 32 + (lane & 31);
+    const int kj = j < a.V ? cnt1[j] : 0;
+    // row i = ib + o, o = mfma_row(lane, r) - 4 (lane >> 5), as in tile_counts
+    const int64_t ib = rb * 32 + 4 * (lane >> 5);
+    const int32_t* ci = cnt1 + ib;
+    const int lim = (int)(a.V - ib < 32 ? a.V - ib : 32);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int o = 8 * (r >> 2) + (r & 3);
+      c.ki[r] = o < lim ? (uint32_t)ci[o] : 0u;
+    }
+    c.n = pgld_count_f64(a.K[0]);
+    c.pB = pgld_count_f64(kj);
+    c.qB = c.n - c.pB;
+    c.uB = c.qB - c.pB;
+    c.PB = c.pB * c.qB;
+  }
+  __device__ __forceinline__ static HapPop pop(const Counts& c, int r) {
+    HapPop p;
+    p.c1 = (double)c.n11[r];
+    p.pA = pgld_count_f64(c.ki[r]);
+    p.n = c.n; p.pB = c.pB; p.qB = c.qB; p.uB = c.uB; p.PB = c.PB;
+    p.qA = p.n - p.pA;
+    p.c2 = p.pA - p.c1; p.c3 = p.pB - p.c1; p.c4 = p.qA - p.c3;
+    p.x = p.c1 * p.c4; p.y = p.c2 * p.c3;
+    p.D = p.y - p.x;
+    p.s14 = p.c1 + p.c4; p.s23 = p.c2 + p.c3;
+    p.uA = p.qA - p.pA;
+    p.PA = p.pA * p.qA;
+    p.w = fma(p.uA, p.uB, p.s23 - p.s14);
+    p.r1 = p.r2 = p.r3 = p.r4 = 0.0;   // not used: the sums are scaled by stat_mult
+    return p;
+  }
+  template <int G>
+  __device__ __forceinline__ static void accum(const FusedParams&, const double*, const Counts& c,
+                                               int r, double* acc, double wt) {
     const HapPop p = pop(c, r);
     acc[0] = fma(hap_dd_same_num(p), wt, acc[0]);
     acc[1] = fma(hap_dz_same_num(p), wt, acc[1]);
