@@ -5,13 +5,14 @@
 namespace pgld {
 
 int launch_binned_hap(const PanelLayout& L, const pgld_binned_args* args, hipStream_t s) {
-  // one population without missing calls: constant reciprocals (HapNMTraits)
+  // one population without missing calls: constant reciprocals, counts kept
+  // as they come (HapNMRawTraits)
   const bool nm = L.n_pops == 1 && !L.missing[0];
   if (args->with_r2)  // one population
-    return nm ? launch_fused<HapNMTraits<true>>(L, args, s)
+    return nm ? launch_fused<HapNMRawTraits<true>>(L, args, s)
               : launch_fused<HapTraits<1, true>>(L, args, s);
   switch (L.n_pops) {
-    case 1: return nm ? launch_fused<HapNMTraits<false>>(L, args, s)
+    case 1: return nm ? launch_fused<HapNMRawTraits<false>>(L, args, s)
                       : launch_fused<HapTraits<1>>(L, args, s);
     case 2: return launch_fused<HapTraits<2>>(L, args, s);
     case 3: return launch_fused<HapTraits<3>>(L, args, s);

@@ -677,9 +677,12 @@ struct HapNMTraits : HapTraits<1, R2> {
   }
 };
 
-// HapNMTraits for the window kernel, whose registers have room for it: n11
-// stays in its FP32 accumulator and the row sites' allele counts in words of
-// their own, so nothing is packed per work item or unpacked per pair.
+// One haplotype population without missing calls, as launched (fused and
+// window kernels): n11 stays in its FP32 accumulator and the row sites'
+// allele counts in words of their own, so nothing is packed per tile or work
+// item and nothing unpacked per pair.  Both kernels keep four waves per SIMD
+// (122-127 registers); HapNMTraits' packed words remain the layout of the
+// 16-site window tiles.
 template <bool R2>
 struct HapNMRawTraits : HapNMTraits<R2> {
   struct Counts {
