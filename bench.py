@@ -107,13 +107,13 @@ def kernel_name(pan, geno, windows, r2, form):
     tail = ", r2" if r2 else ""
     nomiss = pan.P == 1 and not bool(pan.has_missing[0])
     if windows:
-        t = "GenoTraits<1, r2>" if geno else ("HapNMTraits<r2>" if nomiss else "HapTraits<1, r2>")
+        t = "GenoTraits<1, r2>" if geno else ("HapNMRawTraits<r2>" if nomiss else "HapTraits<1, r2>")
         return f"window_kernel<{t}>"
     if geno:
         t = (f"GenoRawTraits<{'r2' if r2 else ''}>" if pan.P == 1 and form != 1
              else f"GenoTraits<{pan.P}{tail}>")
     else:
-        t = f"HapNMTraits<{'r2' if r2 else ''}>" if nomiss else f"HapTraits<{pan.P}{tail}>"
+        t = f"HapNMRawTraits<{'r2' if r2 else ''}>" if nomiss else f"HapTraits<{pan.P}{tail}>"
     return f"fused_kernel<{t}>"
 
 
