@@ -37,8 +37,8 @@ int launch_binned_geno(const PanelLayout& L, const pgld_binned_args* args, hipSt
 }
 
 int launch_windows_geno(const PanelLayout& L, const pgld_window_args* args, hipStream_t s) {
-  return L.pop_size[0] <= 1023 ? launch_windows<GenoTraits<1, 3, true>>(L, args, s)
-                               : launch_windows<GenoTraits<1, 5, true>>(L, args, s);
+  // the moment accumulators read directly, as in the per-wave binned kernel
+  return launch_windows<GenoRawTraits<true>>(L, args, s);
 }
 
 }  // namespace pgld
