@@ -1,8 +1,8 @@
 // Genotype instantiations of the fused binned and window kernels.  One
-// population in the per-wave form keeps its nine moment products in the MFMA
-// accumulators; otherwise the counts are packed per slot: nine 10-bit fields
-// in three words for populations of at most 1,023 individuals, 16-bit fields
-// in five words above.
+// population (the per-wave binned form and the windows) keeps its nine
+// moment products in the MFMA accumulators; otherwise the counts are packed
+// per slot: nine 10-bit fields in three words for populations of at most
+// 1,023 individuals, 16-bit fields in five words above.
 #include "fused_kernels.h"
 
 namespace pgld {
