@@ -74,11 +74,11 @@ def decode_limbs(raw, n_stats):
     # to bin 0's pair count: no real count comes near it
     if nb and raw[0, n_stats, 0] >= (1 << 60):
         raise RuntimeError("fused kernel: a mailbox wait timed out; the result is invalid")
-    sums = np.zeros((nb, n_stats))
-    for b in range(nb):
-        for s in range(n_stats):
-            l0, l1, l2 = (int(x) for x in raw[b, s])
-            sums[b, s] = (l2 * (1 << 64) + l1 * (1 << 32) + l0) / (1 << 72)
+    # exact integer sum of the limbs, one correctly rounded division each
+    # (Python integers in an object array: the loop runs in C)
+    r = raw[:, :n_stats, :].astype(object)
+    v = (r[..., 2] << 64) + (r[..., 1] << 32) + r[..., 0]
+    sums = (v / (1 << 72)).astype(np.float64) if v.size else np.zeros((nb, n_stats))
     return sums, raw[:, n_stats, 0].copy()
 
 
