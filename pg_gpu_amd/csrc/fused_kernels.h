@@ -627,13 +627,14 @@ __device__ __forceinline__ void window_item(const WindowParams& w, const double*
   const int lo_o = (int)(lo - ib < -1 ? -1 : lo - ib > 32 ? 32 : lo - ib);
   const int j_o = (int)(j - ib < -1 ? -1 : j - ib > 32 ? 32 : j - ib);
   const bool jin = j < hi;
-  // two slots per iteration: two pairs' FP64 chains interleave
   if (full) {
-    // every pair of the item is in the window: plain sums
-#pragma unroll 2
+    // every pair of the item is in the window: plain sums, the slot loop
+    // unrolled (static slot indices)
+#pragma unroll
     for (int r = 0; r < 16; ++r) T::template accum<0>(w.f, srt, cts, r, acc, 1.0);
     count = 16;
   } else {
+    // two slots per iteration: two pairs' FP64 chains interleave
 #pragma unroll 2
     for (int r = 0; r < 16; ++r) {
       const int o = 8 * (r >> 2) + (r & 3);
@@ -845,7 +846,7 @@ __device__ __forceinline__ void window_item16(const WindowParams& w, const doubl
   const int64_t ib = i0 + 4 * (lane >> 4);
   const bool jin = j < hi;
   if (full) {
-#pragma unroll 2
+#pragma unroll
     for (int r = 0; r < 4; ++r) T::accum(w.f, srt, cts, r, acc, 1.0);
     count = 4;
   } else {
