@@ -328,6 +328,8 @@ def main():
                     help="use the default launch shape instead of the autotuned one")
     ap.add_argument("--form", type=int, default=0,
                     help="fused kernel form with --no-tune: 0 per-wave, 1 producer/consumer")
+    ap.add_argument("--window-group", type=int, default=0,
+                    help="window kernel work items per wave turn with --no-tune (0 = default)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB three times "
                          "(the known-size kernel that scales the HBM byte counters)")
@@ -374,7 +376,7 @@ def main():
     windows = w.windows is not None
     r2 = w.name in R2_WORKLOADS
     if windows:
-        launch = {"tile": 32, "group": 0, "tuned": False}
+        launch = {"tile": 32, "group": args.window_group, "tuned": False}
         if not args.no_tune:
             from pg_gpu_amd import autotune
 

@@ -207,14 +207,16 @@ typedef struct {
   long long* d_out;            /* [n_windows][n_stats + 3][3] limbs, zeroed */
   int32_t grid;                /* workgroups (0 = auto) */
   int32_t n_lo[PGLD_MAX_POPS]; /* as in pgld_binned_args */
-  uint32_t* d_sched;           /* not used: the window kernel deals each XCD's eighth of
-                                  the work list to its waves in fixed groups */
+  uint32_t* d_sched;           /* not used: the window kernel cuts each XCD's eighth of
+                                  the work list into equal turns for its waves */
   int32_t tile;                /* sites per side of the work items' blocks: 32 (or 0) for
                                   32 x 32 tiles, 16 for 16 x 16 tiles (haplotypes; the
                                   v_mfma_scale_f32_16x16x128_f8f6f4 path) */
-  int32_t group;               /* consecutive work items per wave turn, 1..4096 (0 = 16
-                                  for 32-site items, 32 for 16-site items); the result
-                                  does not depend on it */
+  int32_t group;               /* work items per wave turn, 1..4096 (0 = 16 for 32-site
+                                  items, 32 for 16-site items): each XCD's eighth of the
+                                  work list is cut into R * m equal turns (R waves on the
+                                  XCD, m its items / (R * group), rounded, at least 1);
+                                  the result does not depend on it */
 } pgld_window_args;
 
 /* Per-window sums over every i < j pair of sites inside each window (no

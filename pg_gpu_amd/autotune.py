@@ -109,10 +109,26 @@ def window_job_key(pan, windows):
     return (int(pan.mode), tuple(pan.pop_sizes), tuple(pan.has_missing), q)
 
 
-def tune_windows(pan, windows, reps=3, tiles=(32, 16), groups=(8, 16, 64), use_cache=True):
+WINDOW_TIE = 0.02
+
+
+def pick_window_launch(timings, tie=WINDOW_TIE):
+    """The launch to keep from [(dict(tile, group), ms)...]: among those
+    within ``tie`` (relative) of the fastest, the one with the smallest
+    group.  An XCD's waves work on about (waves x group) neighbouring items
+    at a time: short turns keep those items' panels in the XCD's L2, long
+    turns flush fewer atomics but re-read panels from HBM (on the ragged-window
+    workload, 64 items per turn run as fast as 16 and fetch 5x the bytes)."""
+    fastest = min(t for _, t in timings)
+    near = [(c, t) for c, t in timings if t <= fastest * (1.0 + tie)]
+    return min(near, key=lambda ct: (ct[0]["group"], ct[1]))[0]
+
+
+def tune_windows(pan, windows, reps=3, tiles=(32, 16), groups=(16, 32, 64), use_cache=True):
     """-> (best dict(tile, group), [(config, ms)...]): times the window kernel
     with 32 x 32 and (haplotypes) 16 x 16 MFMA tiles and several work-item
-    group sizes on the job, cached per ``window_job_key``."""
+    group sizes (items per wave turn) on the job, keeps the launch that
+    ``pick_window_launch`` names, cached per ``window_job_key``."""
     from .windows import WindowPlan
 
     key = window_job_key(pan, windows)
@@ -137,6 +153,6 @@ def tune_windows(pan, windows, reps=3, tiles=(32, 16), groups=(8, 16, 64), use_c
                 pend.finish()
                 best = min(best, a.elapsed_time(b))
             timings.append((dict(tile=int(tile), group=int(group)), best))
-    best = min(timings, key=lambda t: t[1])[0]
+    best = pick_window_launch(timings)
     _win_cache[key] = dict(best)
     return dict(best), timings

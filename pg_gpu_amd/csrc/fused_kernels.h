@@ -677,23 +677,34 @@ __device__ __forceinline__ void window_flush(const WindowParams& w, int64_t win,
 }
 
 // Work items of one XCD (workgroup b runs on XCD b % 8) are a contiguous
-// eighth of the work list; inside it wave r of the XCD's R waves takes groups
-// r, r + R, r + 2R, ... of `group` consecutive items.  At any time the XCD's
-// waves work on neighbouring items (the same windows' panels, held in that
-// XCD's L2), and a wave keeps its window's totals in registers while its
-// items stay in that window: global atomics once per (wave, group, window)
-// instead of per item.  Every bound is wave-uniform.
+// eighth of the work list, cut into R * m turns of equal length to within one
+// item (R waves on the XCD; m = the XCD's items / (R * group), rounded, at
+// least 1); wave r takes turns r, r + R, ..., r + (m - 1) R, so every wave
+// holds the same number of items to within m.  At any time the XCD's waves
+// work on neighbouring items (the same windows' panels, held in that XCD's
+// L2), and a wave keeps its window's totals in registers while its items
+// stay in that window: global atomics once per (wave, turn, window) instead
+// of per item.  Every bound is wave-uniform.
 struct WinSched {
-  int64_t first, stride, end;
+  int64_t lo, n, parts, R, r;
+  int m;
   __device__ WinSched(int64_t n_work, int wave, int group) {
     constexpr int nx = 8;  // the grid is a multiple of 8
     const int64_t xcd = blockIdx.x % nx;
-    const int64_t R = (int64_t)(gridDim.x / nx) * 4;
-    const int64_t r = (int64_t)(blockIdx.x / nx) * 4 + wave;
-    const int64_t lo = n_work * xcd / nx;
-    end = n_work * (xcd + 1) / nx;
-    first = lo + r * group;
-    stride = R * group;
+    R = (int64_t)(gridDim.x / nx) * 4;
+    r = (int64_t)(blockIdx.x / nx) * 4 + wave;
+    lo = n_work * xcd / nx;
+    n = n_work * (xcd + 1) / nx - lo;
+    const int64_t per = R * group;
+    const int64_t mm = (n + per / 2) / per;
+    m = (int)(mm < 1 ? 1 : mm);
+    parts = R * m;
+  }
+  // items [k0, k1) of the wave's turn t
+  __device__ void turn(int t, int64_t& k0, int64_t& k1) const {
+    const int64_t p = (int64_t)t * R + r;
+    k0 = lo + n * p / parts;
+    k1 = lo + n * (p + 1) / parts;
   }
 };
 
@@ -714,26 +725,28 @@ __global__ void __launch_bounds__(256, T::MINB) window_kernel(WindowParams w) {
     }
     __syncthreads();
   }
-  const int group = w.group;
-  const WinSched ws(w.n_work, wave, group);
+  const WinSched ws(w.n_work, wave, w.group);
   int64_t cur = -1;
   long long L0 = 0, L1 = 0, L2 = 0;
-  for (int64_t g0 = ws.first; g0 < ws.end; g0 += ws.stride)
-  for (int64_t k = g0, k1 = min(g0 + group, ws.end); k < k1; ++k) {
-    const int64_t win = __builtin_amdgcn_readfirstlane(w.work[3 * k]);
-    const int64_t rb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 1]);
-    const int64_t cb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 2]);
-    if (win != cur) {
-      if (cur >= 0) window_flush<T>(w, cur, lane, L0, L1, L2);
-      cur = win;
-      L0 = L1 = L2 = 0;
+  for (int t = 0; t < ws.m; ++t) {
+    int64_t k0, k1;
+    ws.turn(t, k0, k1);
+    for (int64_t k = k0; k < k1; ++k) {
+      const int64_t win = __builtin_amdgcn_readfirstlane(w.work[3 * k]);
+      const int64_t rb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 1]);
+      const int64_t cb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 2]);
+      if (win != cur) {
+        if (cur >= 0) window_flush<T>(w, cur, lane, L0, L1, L2);
+        cur = win;
+        L0 = L1 = L2 = 0;
+      }
+      const int64_t lo = w.win[2 * win], hi = w.win[2 * win + 1];
+      typename T::Counts cts;
+      T::counts(a, rb, cb, lane, cts);
+      const int64_t i0 = rb * 32, j0 = cb * 32;
+      const bool full = cb > rb && i0 >= lo && j0 + 32 <= hi;
+      window_item<T>(w, srt, cts, lane, full, i0, j0, lo, hi, L0, L1, L2);
     }
-    const int64_t lo = w.win[2 * win], hi = w.win[2 * win + 1];
-    typename T::Counts cts;
-    T::counts(a, rb, cb, lane, cts);
-    const int64_t i0 = rb * 32, j0 = cb * 32;
-    const bool full = cb > rb && i0 >= lo && j0 + 32 <= hi;
-    window_item<T>(w, srt, cts, lane, full, i0, j0, lo, hi, L0, L1, L2);
   }
   if (cur >= 0) window_flush<T>(w, cur, lane, L0, L1, L2);
 }
@@ -880,26 +893,28 @@ __global__ void __launch_bounds__(256, 3) window16_kernel(WindowParams w) {
     for (int e = tid; e < 4 * len; e += blockDim.x) stab[e] = a.rtab[4 * a.rt_lo[0] + e];
     __syncthreads();
   }
-  const int group = w.group;
-  const WinSched ws(w.n_work, wave, group);
+  const WinSched ws(w.n_work, wave, w.group);
   int64_t cur = -1;
   long long L0 = 0, L1 = 0, L2 = 0;
-  for (int64_t g0 = ws.first; g0 < ws.end; g0 += ws.stride)
-  for (int64_t k = g0, k1 = min(g0 + group, ws.end); k < k1; ++k) {
-    const int64_t win = __builtin_amdgcn_readfirstlane(w.work[3 * k]);
-    const int64_t rb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 1]);
-    const int64_t cb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 2]);
-    if (win != cur) {
-      if (cur >= 0) window_flush<Hap1Tile16>(w, cur, lane, L0, L1, L2);
-      cur = win;
-      L0 = L1 = L2 = 0;
+  for (int t = 0; t < ws.m; ++t) {
+    int64_t k0, k1;
+    ws.turn(t, k0, k1);
+    for (int64_t k = k0; k < k1; ++k) {
+      const int64_t win = __builtin_amdgcn_readfirstlane(w.work[3 * k]);
+      const int64_t rb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 1]);
+      const int64_t cb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 2]);
+      if (win != cur) {
+        if (cur >= 0) window_flush<Hap1Tile16>(w, cur, lane, L0, L1, L2);
+        cur = win;
+        L0 = L1 = L2 = 0;
+      }
+      const int64_t lo = w.win[2 * win], hi = w.win[2 * win + 1];
+      Hap1Tile16::Counts cts;
+      tile16_counts(a, rb, cb, lane, cts.w0, cts.w1);
+      const int64_t i0 = rb * 16, j0 = cb * 16;
+      const bool full = cb > rb && i0 >= lo && j0 + 16 <= hi;
+      window_item16(w, srt, cts, lane, full, i0, j0, lo, hi, L0, L1, L2);
     }
-    const int64_t lo = w.win[2 * win], hi = w.win[2 * win + 1];
-    Hap1Tile16::Counts cts;
-    tile16_counts(a, rb, cb, lane, cts.w0, cts.w1);
-    const int64_t i0 = rb * 16, j0 = cb * 16;
-    const bool full = cb > rb && i0 >= lo && j0 + 16 <= hi;
-    window_item16(w, srt, cts, lane, full, i0, j0, lo, hi, L0, L1, L2);
   }
   if (cur >= 0) window_flush<Hap1Tile16>(w, cur, lane, L0, L1, L2);
 }
@@ -928,6 +943,9 @@ static int launch_windows(const PanelLayout& L, const pgld_window_args* g, hipSt
   w.n_work = g->n_work;
   w.win = g->d_windows;
   w.out = reinterpret_cast<unsigned long long*>(g->d_out);
+  // items per wave turn: an XCD's R waves work on about R * group
+  // neighbouring items at a time, whose panels stay in that XCD's L2 at 16
+  // items per turn (longer turns flush fewer atomics but re-read panels)
   w.group = g->group > 0 ? g->group : (g->tile == 16 ? 32 : 16);
   size_t lds = (size_t)a.rt_len * 4 * sizeof(double);
   // a table wider than the LDS (a large population with heavily missing

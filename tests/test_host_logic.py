@@ -406,3 +406,19 @@ def test_ctypes_structs_match_the_header(cname, pyname):
         stub = re.search(r"class " + pyname + r"\(ctypes.Structure\):\n\s*_fields_ = \[(.*?)\]\n",
                          doc, flags=re.S).group(1)
         assert re.findall(r'\("(\w+)"', stub) == want
+
+
+def test_window_launch_pick_prefers_short_turns_within_the_tie():
+    from pg_gpu_amd.autotune import WINDOW_TIE, pick_window_launch
+
+    def c(tile, group):
+        return dict(tile=tile, group=group)
+
+    t = [(c(32, 16), 1.03), (c(32, 32), 1.015), (c(32, 64), 1.00), (c(16, 64), 2.0)]
+    assert pick_window_launch(t) == c(32, 32)           # 16 is outside the 2 % tie
+    t = [(c(32, 64), 1.00), (c(32, 16), 1.0 + 0.99 * WINDOW_TIE)]
+    assert pick_window_launch(t) == c(32, 16)
+    t = [(c(16, 16), 1.01), (c(32, 16), 1.00)]          # equal groups: the faster
+    assert pick_window_launch(t) == c(32, 16)
+    assert pick_window_launch(t, tie=0.0) == c(32, 16)
+    assert pick_window_launch([(c(32, 16), 1.5), (c(32, 64), 1.0)], tie=0.0) == c(32, 64)
