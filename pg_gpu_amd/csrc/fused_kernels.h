@@ -581,8 +581,18 @@ struct WindowParams {
   int64_t n_work;
   const int64_t* win;      // [n_windows][2]
   unsigned long long* out; // [n_windows][NSO + 1][3]
-  int32_t group;           // consecutive work items per wave turn
+  int32_t group;           // work items per wave turn
+  int32_t slot_off;        // byte offset of the window slots in dynamic LDS
 };
+
+// A workgroup's turn touches a few consecutive windows: the first
+// PGLD_WIN_SLOTS of them collect the four waves' limbs in LDS,
+// [turn in batch][slot][q < 16][limb], and go to the output once per batch
+// of PGLD_WIN_BATCH turns.
+#define PGLD_WIN_SLOTS 4
+#define PGLD_WIN_BATCH 8
+#define PGLD_WIN_SET_WORDS (PGLD_WIN_SLOTS * 16 * 3)
+#define PGLD_WIN_SLOT_WORDS (PGLD_WIN_BATCH * PGLD_WIN_SET_WORDS)
 
 // Adds one item's wave total `tot` to the lane's running limbs: statistic
 // lanes add its fixed-point limbs, the count lane the exact pair count.  One
@@ -676,6 +686,39 @@ __device__ __forceinline__ void window_flush(const WindowParams& w, int64_t win,
   }
 }
 
+// Window kernels that combine their waves' limbs in LDS slots: 32-site
+// haplotype items without missing calls run as fast either way and write 8x
+// less with slots; items with missing calls, genotype items and 16-site
+// items run 1-4 % faster with per-wave atomics.
+template <class T>
+constexpr bool window_slots() {
+  return std::is_same<T, HapNMRawTraits<true>>::value;
+}
+
+// Adds a wave's limbs of window `win` to LDS slot `slot` of its workgroup
+// turn's slot set, or straight to the output when the window has no slot.
+// `slot` is wave-uniform.
+template <class T, bool SLOTS>
+__device__ __forceinline__ void window_put(const WindowParams& w, unsigned long long* slots,
+                                           int64_t win, int64_t slot, int lane, long long L0,
+                                           long long L1, long long L2) {
+  constexpr int GS = T::template gsz<0>();
+  constexpr int NV = GS <= 2 ? 2 : GS <= 4 ? 4 : GS <= 8 ? 8 : GS <= 16 ? 16 : GS <= 32 ? 32 : 64;
+  constexpr int LOG = NV == 2 ? 1 : NV == 4 ? 2 : NV == 8 ? 3 : NV == 16 ? 4 : NV == 32 ? 5 : 6;
+  static_assert(GS < 16, "a window slot holds 16 columns");
+  if (!SLOTS || slot < 0 || slot >= PGLD_WIN_SLOTS) {
+    window_flush<T>(w, win, lane, L0, L1, L2);
+    return;
+  }
+  const int q = pgld_bitrev(lane & (NV - 1), LOG);
+  if (lane < NV && q <= GS) {
+    unsigned long long* d = slots + ((int)slot * 16 + q) * 3;
+    if (L0) atomicAdd(d + 0, (unsigned long long)L0);
+    if (L1) atomicAdd(d + 1, (unsigned long long)L1);
+    if (L2) atomicAdd(d + 2, (unsigned long long)L2);
+  }
+}
+
 // Work items of one XCD (workgroup b runs on XCD b % 8) are a contiguous
 // eighth of the work list, cut into R * m turns of equal length to within one
 // item (R waves on the XCD; m = the XCD's items / (R * group), rounded, at
@@ -706,7 +749,70 @@ struct WinSched {
     k0 = lo + n * p / parts;
     k1 = lo + n * (p + 1) / parts;
   }
+  // the first item of the workgroup's turn t (its four waves take
+  // neighbouring turns r - wave, ..., r - wave + 3)
+  __device__ int64_t block_first(int t, int wave) const {
+    return lo + n * ((int64_t)t * R + r - wave) / parts;
+  }
 };
+
+// Runs `item(k, win, L0, L1, L2)` over the wave's work items turn by turn.
+// With SLOTS the limbs of each window go to the LDS slot set of the
+// workgroup's turn (slot = window offset from the turn's first window), and
+// the sets are added to the output and cleared after every PGLD_WIN_BATCH
+// turns: global atomics once per (workgroup, turn, window), two barriers
+// per batch.  The turn count is the same for the four waves (one XCD), so
+// every wave reaches every barrier.  Without SLOTS each wave adds its limbs
+// to the output once per (wave, turn, window).
+template <class T, bool SLOTS, class Item>
+__device__ __forceinline__ void window_turns(const WindowParams& w, unsigned long long* slots,
+                                             int tid, int wave, Item item) {
+  constexpr int NS = T::NSO;
+  constexpr int GS = T::template gsz<0>();
+  const int lane = tid & 63;
+  const WinSched ws(w.n_work, wave, w.group);
+  if constexpr (SLOTS) {
+    for (int e = tid; e < PGLD_WIN_SLOT_WORDS; e += blockDim.x) slots[e] = 0;
+    __syncthreads();
+  }
+  for (int t = 0; t < ws.m; ++t) {
+    int64_t k0, k1;
+    ws.turn(t, k0, k1);
+    const int64_t b0 = ws.block_first(t, wave);
+    const int64_t w0 = b0 < w.n_work ? (int64_t)__builtin_amdgcn_readfirstlane(w.work[3 * b0]) : 0;
+    unsigned long long* set = slots + (t % PGLD_WIN_BATCH) * PGLD_WIN_SET_WORDS;
+    int64_t cur = -1;
+    long long L0 = 0, L1 = 0, L2 = 0;
+    for (int64_t k = k0; k < k1; ++k) {
+      const int64_t win = __builtin_amdgcn_readfirstlane(w.work[3 * k]);
+      if (win != cur) {
+        if (cur >= 0) window_put<T, SLOTS>(w, set, cur, cur - w0, lane, L0, L1, L2);
+        cur = win;
+        L0 = L1 = L2 = 0;
+      }
+      item(k, win, L0, L1, L2);
+    }
+    if (cur >= 0) window_put<T, SLOTS>(w, set, cur, cur - w0, lane, L0, L1, L2);
+    if (SLOTS && (t % PGLD_WIN_BATCH == PGLD_WIN_BATCH - 1 || t == ws.m - 1)) {
+      __syncthreads();
+      const int t0 = t - t % PGLD_WIN_BATCH;
+      for (int e = tid; e < (t - t0 + 1) * PGLD_WIN_SET_WORDS; e += blockDim.x) {
+        const unsigned long long v = slots[e];
+        if (v) {
+          slots[e] = 0;
+          const int j = e / PGLD_WIN_SET_WORDS, f = e % PGLD_WIN_SET_WORDS;
+          const int slot = f / 48, q = (f / 3) % 16, limb = f % 3;
+          // a set with a nonzero word belongs to a turn with items, whose
+          // first item lies inside the work list
+          const int64_t first = w.work[3 * ws.block_first(t0 + j, wave)];
+          const int c = q < GS ? T::template col<0>(q) : NS;
+          atomicAdd(w.out + ((first + slot) * (NS + 1) + c) * 3 + limb, v);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
 
 template <class T, bool GTAB = false>
 __global__ void __launch_bounds__(256, T::MINB) window_kernel(WindowParams w) {
@@ -725,30 +831,18 @@ __global__ void __launch_bounds__(256, T::MINB) window_kernel(WindowParams w) {
     }
     __syncthreads();
   }
-  const WinSched ws(w.n_work, wave, w.group);
-  int64_t cur = -1;
-  long long L0 = 0, L1 = 0, L2 = 0;
-  for (int t = 0; t < ws.m; ++t) {
-    int64_t k0, k1;
-    ws.turn(t, k0, k1);
-    for (int64_t k = k0; k < k1; ++k) {
-      const int64_t win = __builtin_amdgcn_readfirstlane(w.work[3 * k]);
-      const int64_t rb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 1]);
-      const int64_t cb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 2]);
-      if (win != cur) {
-        if (cur >= 0) window_flush<T>(w, cur, lane, L0, L1, L2);
-        cur = win;
-        L0 = L1 = L2 = 0;
-      }
-      const int64_t lo = w.win[2 * win], hi = w.win[2 * win + 1];
-      typename T::Counts cts;
-      T::counts(a, rb, cb, lane, cts);
-      const int64_t i0 = rb * 32, j0 = cb * 32;
-      const bool full = cb > rb && i0 >= lo && j0 + 32 <= hi;
-      window_item<T>(w, srt, cts, lane, full, i0, j0, lo, hi, L0, L1, L2);
-    }
-  }
-  if (cur >= 0) window_flush<T>(w, cur, lane, L0, L1, L2);
+  unsigned long long* slots = reinterpret_cast<unsigned long long*>(smem + w.slot_off);
+  window_turns<T, window_slots<T>()>(w, slots, tid, wave,
+                  [&](int64_t k, int64_t win, long long& L0, long long& L1, long long& L2) {
+    const int64_t rb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 1]);
+    const int64_t cb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 2]);
+    const int64_t lo = w.win[2 * win], hi = w.win[2 * win + 1];
+    typename T::Counts cts;
+    T::counts(a, rb, cb, lane, cts);
+    const int64_t i0 = rb * 32, j0 = cb * 32;
+    const bool full = cb > rb && i0 >= lo && j0 + 32 <= hi;
+    window_item<T>(w, srt, cts, lane, full, i0, j0, lo, hi, L0, L1, L2);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -893,30 +987,19 @@ __global__ void __launch_bounds__(256, 3) window16_kernel(WindowParams w) {
     for (int e = tid; e < 4 * len; e += blockDim.x) stab[e] = a.rtab[4 * a.rt_lo[0] + e];
     __syncthreads();
   }
-  const WinSched ws(w.n_work, wave, w.group);
-  int64_t cur = -1;
-  long long L0 = 0, L1 = 0, L2 = 0;
-  for (int t = 0; t < ws.m; ++t) {
-    int64_t k0, k1;
-    ws.turn(t, k0, k1);
-    for (int64_t k = k0; k < k1; ++k) {
-      const int64_t win = __builtin_amdgcn_readfirstlane(w.work[3 * k]);
-      const int64_t rb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 1]);
-      const int64_t cb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 2]);
-      if (win != cur) {
-        if (cur >= 0) window_flush<Hap1Tile16>(w, cur, lane, L0, L1, L2);
-        cur = win;
-        L0 = L1 = L2 = 0;
-      }
-      const int64_t lo = w.win[2 * win], hi = w.win[2 * win + 1];
-      Hap1Tile16::Counts cts;
-      tile16_counts(a, rb, cb, lane, cts.w0, cts.w1);
-      const int64_t i0 = rb * 16, j0 = cb * 16;
-      const bool full = cb > rb && i0 >= lo && j0 + 16 <= hi;
-      window_item16(w, srt, cts, lane, full, i0, j0, lo, hi, L0, L1, L2);
-    }
-  }
-  if (cur >= 0) window_flush<Hap1Tile16>(w, cur, lane, L0, L1, L2);
+  unsigned long long* slots = reinterpret_cast<unsigned long long*>(smem + w.slot_off);
+  window_turns<Hap1Tile16, false>(w, slots, tid, wave,
+                           [&](int64_t k, int64_t win, long long& L0, long long& L1,
+                               long long& L2) {
+    const int64_t rb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 1]);
+    const int64_t cb = __builtin_amdgcn_readfirstlane(w.work[3 * k + 2]);
+    const int64_t lo = w.win[2 * win], hi = w.win[2 * win + 1];
+    Hap1Tile16::Counts cts;
+    tile16_counts(a, rb, cb, lane, cts.w0, cts.w1);
+    const int64_t i0 = rb * 16, j0 = cb * 16;
+    const bool full = cb > rb && i0 >= lo && j0 + 16 <= hi;
+    window_item16(w, srt, cts, lane, full, i0, j0, lo, hi, L0, L1, L2);
+  });
 }
 
 template <class T>
@@ -947,18 +1030,26 @@ static int launch_windows(const PanelLayout& L, const pgld_window_args* g, hipSt
   // neighbouring items at a time, whose panels stay in that XCD's L2 at 16
   // items per turn (longer turns flush fewer atomics but re-read panels)
   w.group = g->group > 0 ? g->group : (g->tile == 16 ? 32 : 16);
-  size_t lds = (size_t)a.rt_len * 4 * sizeof(double);
-  // a table wider than the LDS (a large population with heavily missing
-  // sites) is read from global memory, indexed by n
-  const bool gtab = lds > 160 * 1024;
+  const size_t slot_bytes =
+      window_slots<T>() && g->tile != 16 ? PGLD_WIN_SLOT_WORDS * sizeof(unsigned long long) : 0;
+  size_t tab = ((size_t)a.rt_len * 4 * sizeof(double) + 15) / 16 * 16;
+  // a table that leaves no room for the window slots in the LDS (a large
+  // population with heavily missing sites) is read from global memory,
+  // indexed by n
+  const bool gtab = tab + slot_bytes > 160 * 1024;
   if (gtab) {
     a.rt_lo[0] = 0;
     a.rt_len = 0;
-    lds = 0;
-  } else {
+    tab = 0;
+  }
+  w.slot_off = (int32_t)tab;
+  const size_t lds = tab + slot_bytes;
+  if (gtab)
+    PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(window_kernel<T, true>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  else
     PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(window_kernel<T, false>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  }
   int grid = g->grid > 0 ? g->grid : T::MINB * num_cus();
   grid = ((grid + 7) / 8) * 8;
   const int64_t need = ((g->n_work + 31) / 32) * 8;
@@ -969,11 +1060,12 @@ static int launch_windows(const PanelLayout& L, const pgld_window_args* g, hipSt
     if constexpr (std::is_same<T, HapTraits<1, true>>::value ||
                   std::is_same<T, HapNMTraits<true>>::value ||
                   std::is_same<T, HapNMRawTraits<true>>::value) {
-      if (!gtab)
-        PGLD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(window16_kernel<false>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      PGLD_HIP_CHECK(hipFuncSetAttribute(
+          gtab ? reinterpret_cast<const void*>(window16_kernel<true>)
+               : reinterpret_cast<const void*>(window16_kernel<false>),
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       if (gtab)
-        hipLaunchKernelGGL((window16_kernel<true>), dim3(grid), dim3(256), 0, s, w);
+        hipLaunchKernelGGL((window16_kernel<true>), dim3(grid), dim3(256), lds, s, w);
       else
         hipLaunchKernelGGL((window16_kernel<false>), dim3(grid), dim3(256), lds, s, w);
       PGLD_LAUNCH_CHECK();
@@ -983,7 +1075,7 @@ static int launch_windows(const PanelLayout& L, const pgld_window_args* g, hipSt
     }
   }
   if (gtab)
-    hipLaunchKernelGGL((window_kernel<T, true>), dim3(grid), dim3(256), 0, s, w);
+    hipLaunchKernelGGL((window_kernel<T, true>), dim3(grid), dim3(256), lds, s, w);
   else
     hipLaunchKernelGGL((window_kernel<T, false>), dim3(grid), dim3(256), lds, s, w);
   PGLD_LAUNCH_CHECK();
